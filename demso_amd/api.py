@@ -1,0 +1,337 @@
+"""Python mirror of DemSo's step interface (src/functions.cuh:29-89) over the
+C ABI of libdemso.so.
+
+The stage functions keep the reference's names, argument order and meaning
+(calcHash, sortParticles, reorderDataAndFindCellStart, collide,
+integrateSystem, set_gravity, allocateVectors/desAllocateVectors), with a
+Context first; arrays are torch tensors on the GPU whose memory is handed to
+the HIP kernels as raw device pointers.  Errors raise DemError with the
+library's message instead of being ignored.
+
+Simulation owns the double-buffered state and runs whole steps (eagerly or
+by hipGraph replay), like SimLooping (src/main.cu:371-565) without the
+renderer.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_float, c_uint32, c_void_p
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import DemError, InstanceProps, ParticleProps, SystemProps, check
+from .scene import Scene
+
+__all__ = [
+    "Context",
+    "Simulation",
+    "DeviceState",
+    "calcHash",
+    "sortParticles",
+    "reorderDataAndFindCellStart",
+    "collide",
+    "integrateSystem",
+    "set_gravity",
+    "allocateVectors",
+    "desAllocateVectors",
+    "DemError",
+]
+
+
+def _ptr(t) -> Optional[int]:
+    """Device address of a tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("arrays must be GPU tensors")
+    if not t.is_contiguous():
+        raise ValueError("arrays must be contiguous")
+    return t.data_ptr()
+
+
+def _np_ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("host arrays must be C-contiguous")
+    return a.ctypes.data
+
+
+def system_props(scene: Scene) -> SystemProps:
+    inst0 = scene.instances[0]
+    sp = SystemProps()
+    sp.num_particles = scene.n
+    sp.cube_dimension[:] = inst0.box
+    sp.grid_size[:] = inst0.grid
+    sp.num_cells = inst0.grid[0] * inst0.grid[1]
+    sp.time_step = scene.dt
+    sp.gravity[:] = scene.gravity
+    sp.boundary_normal_stiffness = scene.bkn
+    sp.boundary_shear_stiffness = scene.bkt
+    sp.grid_extent[:] = inst0.extent
+    sp.flags = scene.flags
+    return sp
+
+
+def particle_props(scene: Scene, single_instance: bool) -> ctypes.Array:
+    arr = (ParticleProps * len(scene.materials))()
+    for k, m in enumerate(scene.materials):
+        p = arr[k]
+        p.radius, p.mass, p.inertia = m.radius, m.mass, m.inertia
+        p.normal_stiffness, p.shear_stiffness, p.normal_damping = m.kn, m.kt, m.cn
+        p.boundary_damping, p.friction_coefficient, p.attract_coefficient = m.bdamp, m.mu, m.attract
+        p.color_r = p.color_g = p.color_b = 255.0
+        p.areal_density = m.areal
+        p.instance = 0 if single_instance else m.instance
+    return arr
+
+
+def instance_props(scene: Scene) -> ctypes.Array:
+    arr = (InstanceProps * len(scene.instances))()
+    for k, g in enumerate(scene.instances):
+        arr[k].cube_dimension[:] = g.box
+        arr[k].grid_extent[:] = g.extent
+        arr[k].grid_size[:] = g.grid
+        arr[k].cell_base = g.cell_base
+    return arr
+
+
+class Context:
+    """dem_context: device, stream and the parameter tables."""
+
+    def __init__(self, device: int = 0, stream: Optional[int] = None):
+        self.lib = _lib.load()
+        h = c_void_p()
+        check(self.lib.dem_create(byref(h), device, stream), None)
+        self.handle = h.value
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.dem_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, code: int) -> None:
+        check(code, self.handle)
+
+    def set_params(self, sys: SystemProps, types: ctypes.Array) -> None:
+        self._check(self.lib.dem_set_params(self.handle, byref(sys), types, len(types)))
+
+    def set_instances(self, inst: ctypes.Array) -> None:
+        self._check(self.lib.dem_set_instances(self.handle, inst, len(inst)))
+
+    def set_materials(self, types: ctypes.Array) -> None:
+        self._check(self.lib.dem_set_materials(self.handle, types, len(types)))
+
+    def load_scene(self, scene: Scene) -> None:
+        """Upload the scene's parameters: one system, or an ensemble (geometry
+        first, then the materials that name their instances)."""
+        self.set_params(system_props(scene), particle_props(scene, single_instance=True))
+        if len(scene.instances) > 1:
+            self.set_instances(instance_props(scene))
+            self.set_materials(particle_props(scene, single_instance=False))
+
+    def set_gravity(self, gx: float, gy: float) -> None:
+        self._check(self.lib.dem_set_gravity(self.handle, gx, gy))
+
+    def reserve(self, n: int) -> None:
+        self._check(self.lib.dem_reserve(self.handle, n))
+
+    def synchronize(self) -> None:
+        self._check(self.lib.dem_synchronize(self.handle))
+
+
+# ---- the five stages, reference names ----------------------------------------
+
+def calcHash(ctx: Context, pos, gridParticleIndex, gridParticleHash, numParticles: int, type=None) -> None:
+    """calcHash (src/functions.cu:315-327)."""
+    if type is None:
+        ctx._check(ctx.lib.dem_calc_hash(ctx.handle, _ptr(pos), _ptr(gridParticleIndex),
+                                         _ptr(gridParticleHash), numParticles))
+    else:
+        ctx._check(ctx.lib.dem_calc_hash_ex(ctx.handle, _ptr(pos), _ptr(type), _ptr(gridParticleIndex),
+                                            _ptr(gridParticleHash), numParticles))
+
+
+def sortParticles(ctx: Context, gridParticleHash, gridParticleIndex, numParticles: int) -> None:
+    """sortParticles (src/functions.cu:336-341): stable key/index sort, in place."""
+    ctx._check(ctx.lib.dem_sort_particles(ctx.handle, _ptr(gridParticleHash), _ptr(gridParticleIndex),
+                                          numParticles))
+
+
+def reorderDataAndFindCellStart(ctx: Context, cellStart, cellEnd, sortedPos, sortedVel, sortedTheta,
+                                sortedOmega, sortedID, sortedLoc, sortedType, gridParticleHash,
+                                gridParticleIndex, oldPos, oldVel, oldTheta, oldOmega, oldID, oldType,
+                                numParticles: int, numCells: int, sortedRadius=None, oldRadius=None) -> None:
+    """reorderDataAndFindCellStart (src/functions.cu:347-393)."""
+    ctx._check(ctx.lib.dem_reorder_ex(
+        ctx.handle, _ptr(cellStart), _ptr(cellEnd), _ptr(sortedPos), _ptr(sortedVel), _ptr(sortedTheta),
+        _ptr(sortedOmega), _ptr(sortedID), _ptr(sortedLoc), _ptr(sortedType), _ptr(sortedRadius),
+        _ptr(gridParticleHash), _ptr(gridParticleIndex), _ptr(oldPos), _ptr(oldVel), _ptr(oldTheta),
+        _ptr(oldOmega), _ptr(oldID), _ptr(oldType), _ptr(oldRadius), numParticles, numCells))
+
+
+def collide(ctx: Context, oldPos, oldVel, newAcc, oldOmega, newAlpha, oldType, cellStart, cellEnd,
+            numParticles: int, numCells: int, pressure, radius=None) -> None:
+    """collide (src/functions.cu:398-521, without the controlled particle)."""
+    ctx._check(ctx.lib.dem_collide_ex(
+        ctx.handle, _ptr(oldPos), _ptr(oldVel), _ptr(newAcc), _ptr(oldOmega), _ptr(newAlpha), _ptr(oldType),
+        _ptr(radius), _ptr(cellStart), _ptr(cellEnd), numParticles, numCells, _ptr(pressure)))
+
+
+def integrateSystem(ctx: Context, pos, vel, acc, theta, omega, alpha, type, pressure, numParticles: int,
+                    radius=None) -> None:
+    """integrateSystem (src/functions.cu:529-551)."""
+    ctx._check(ctx.lib.dem_integrate_ex(
+        ctx.handle, _ptr(pos), _ptr(vel), _ptr(acc), _ptr(theta), _ptr(omega), _ptr(alpha), _ptr(type),
+        _ptr(radius), _ptr(pressure), numParticles))
+
+
+def set_gravity(ctx: Context, gravity) -> None:
+    """set_gravity (src/functions.cu:663-668)."""
+    ctx.set_gravity(float(gravity[0]), float(gravity[1]))
+
+
+class DeviceState:
+    """The per-particle arrays of ParticlesValues (src/main.cuh:58-104) for
+    one buffer set, as torch tensors."""
+
+    FIELDS = ("pos", "vel", "theta", "omega", "id", "type", "loc", "radius")
+
+    def __init__(self, n: int, with_radius: bool, device: str = "cuda"):
+        import torch
+
+        f32, i32 = torch.float32, torch.int32
+        self.n = n
+        self.pos = torch.zeros((n, 2), dtype=f32, device=device)
+        self.vel = torch.zeros((n, 2), dtype=f32, device=device)
+        self.theta = torch.zeros(n, dtype=f32, device=device)
+        self.omega = torch.zeros(n, dtype=f32, device=device)
+        self.id = torch.zeros(n, dtype=i32, device=device)
+        self.type = torch.zeros(n, dtype=i32, device=device)
+        self.loc = torch.zeros(n, dtype=i32, device=device)
+        self.radius = torch.zeros(n, dtype=f32, device=device) if with_radius else None
+
+    def upload(self, scene: Scene) -> None:
+        import torch
+
+        for f in self.FIELDS:
+            src = getattr(scene, f)
+            dst = getattr(self, f)
+            if dst is None:
+                continue
+            dst.copy_(torch.from_numpy(np.ascontiguousarray(src).view(np.int32 if src.dtype == np.uint32 else src.dtype)))
+
+
+def allocateVectors(scene: Scene, ctx: Context) -> dict:
+    """allocateVectors (src/functions.cu:68-149): device arrays for both
+    buffer sets plus acc/alpha/pressure/cell/key arrays, parameters uploaded
+    to ctx.  Returns a dict of tensors (freed by desAllocateVectors)."""
+    import torch
+
+    n, cells = scene.n, scene.total_cells
+    rad = scene.radius is not None
+    ctx.load_scene(scene)
+    ctx.reserve(n)
+    v = {
+        "s1": DeviceState(n, rad),
+        "s2": DeviceState(n, rad),
+        "acc": torch.zeros((n, 2), dtype=torch.float32, device="cuda"),
+        "alpha": torch.zeros(n, dtype=torch.float32, device="cuda"),
+        "pressure": torch.zeros(n, dtype=torch.float32, device="cuda"),
+        "cellStart": torch.zeros(cells, dtype=torch.int32, device="cuda"),
+        "cellEnd": torch.zeros(cells, dtype=torch.int32, device="cuda"),
+        "gridParticleIndex": torch.zeros(n, dtype=torch.int32, device="cuda"),
+        "gridParticleHash": torch.zeros(n, dtype=torch.int32, device="cuda"),
+    }
+    return v
+
+
+def desAllocateVectors(values: dict) -> None:
+    """desAllocateVectors (src/functions.cu:154-194)."""
+    values.clear()
+
+
+class Simulation:
+    """dem_sim: owned double-buffered device state and whole steps."""
+
+    def __init__(self, ctx: Context, scene: Scene):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.n = scene.n
+        self.cells = scene.total_cells
+        self.with_radius = scene.radius is not None
+        ctx.load_scene(scene)
+        h = c_void_p()
+        ctx._check(self.lib.dem_sim_create(ctx.handle, self.n, self.cells, int(self.with_radius), byref(h)))
+        self.handle = h.value
+        self.upload(scene)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.dem_sim_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene: Scene) -> None:
+        r = scene.radius if self.with_radius else None
+        self.ctx._check(self.lib.dem_sim_upload(
+            self.handle, _np_ptr(scene.pos), _np_ptr(scene.vel), _np_ptr(scene.theta), _np_ptr(scene.omega),
+            _np_ptr(scene.id), _np_ptr(scene.type), _np_ptr(scene.loc), _np_ptr(r)))
+
+    def download(self, extra: bool = True) -> dict:
+        n, c = self.n, self.cells
+        out = {
+            "pos": np.empty((n, 2), np.float32), "vel": np.empty((n, 2), np.float32),
+            "theta": np.empty(n, np.float32), "omega": np.empty(n, np.float32),
+            "id": np.empty(n, np.uint32), "type": np.empty(n, np.uint32), "loc": np.empty(n, np.uint32),
+            "radius": np.empty(n, np.float32) if self.with_radius else None,
+        }
+        if extra:
+            out.update({
+                "acc": np.empty((n, 2), np.float32), "alpha": np.empty(n, np.float32),
+                "pressure": np.empty(n, np.float32), "hash": np.empty(n, np.uint32),
+                "index": np.empty(n, np.uint32), "cellStart": np.empty(c, np.uint32),
+                "cellEnd": np.empty(c, np.uint32),
+            })
+        g = lambda k: _np_ptr(out.get(k))
+        self.ctx._check(self.lib.dem_sim_download(
+            self.handle, g("pos"), g("vel"), g("theta"), g("omega"), g("id"), g("type"), g("loc"), g("radius"),
+            g("acc"), g("alpha"), g("pressure"), g("hash"), g("index"), g("cellStart"), g("cellEnd")))
+        return out
+
+    def step(self, steps: int = 1) -> None:
+        self.ctx._check(self.lib.dem_sim_step(self.handle, steps))
+
+    def step_graph(self, steps: int) -> None:
+        self.ctx._check(self.lib.dem_sim_step_graph(self.handle, steps))
+
+    def stage_times(self, reps: int = 5) -> dict:
+        ms = (c_float * 7)()
+        self.ctx._check(self.lib.dem_sim_stage_times(self.handle, reps, ms))
+        names = ("hash", "sort", "cell_fill", "reorder", "collide", "integrate", "step")
+        return {k: float(ms[i]) for i, k in enumerate(names)}
+
+    def autotune(self) -> dict:
+        b = (c_uint32 * 5)()
+        self.ctx._check(self.lib.dem_sim_autotune(self.handle, b))
+        return {"hash": b[0], "reorder": b[1], "collide": b[2], "integrate": b[3], "sort_items": b[4]}
+
+    def sort_passes(self) -> int:
+        return int(self.lib.dem_sim_sort_passes(self.handle))
+
+    def synchronize(self) -> None:
+        self.ctx.synchronize()

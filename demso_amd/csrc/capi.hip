@@ -1,0 +1,816 @@
+// capi.hip -- the extern "C" boundary of include/demso.h.
+//
+// Host-side replacements for DemSo's launch wrappers (src/functions.cu) and
+// step loop (src/main.cu:371-565): argument checking, parameter tables in
+// device memory instead of __constant__ symbols, scratch management, the
+// owned double-buffered simulation, hipGraph capture/replay, HIP-event stage
+// timing and block-size autotuning.  No call here exits the process.
+#include "dem_internal.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+using dem::Instance;
+using dem::Material;
+using dem::SortScratch;
+using dem::StepConst;
+
+struct dem_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool params_set = false;
+  StepConst c{};
+  Material* d_mat = nullptr;
+  uint32_t mat_cap = 0;
+  Instance* d_inst = nullptr;
+  uint32_t inst_cap = 0;
+  std::vector<Material> h_mat;
+  std::vector<Instance> h_inst;
+  uint32_t total_cells = 0;  // every key produced by hashing is < total_cells
+  uint32_t key_bits = 32;
+  SortScratch sort;
+  uint32_t* d_copy_tmp = nullptr;
+  int bs_hash = 256, bs_reorder = 256, bs_collide = 128, bs_integrate = 256;
+  uint64_t generation = 1;  // bumped by every parameter change
+};
+
+struct dem_sim {
+  dem_context* ctx = nullptr;
+  uint32_t n = 0, cells = 0;
+  bool rad = false;
+  float2 *pos[2] = {}, *vel[2] = {};
+  float *theta[2] = {}, *omega[2] = {}, *radius[2] = {};
+  uint32_t *id[2] = {}, *type[2] = {}, *loc[2] = {};
+  float2* acc = nullptr;
+  float *alpha = nullptr, *pressure = nullptr;
+  uint32_t *key = nullptr, *idx = nullptr;
+  uint32_t *sorted_key = nullptr, *sorted_idx = nullptr;
+  uint32_t *cell_start = nullptr, *cell_end = nullptr;
+  SortScratch sort;
+  int cur = 0;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  uint64_t graph_gen = 0;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(dem_context* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+int hip_fail(dem_context* ctx, hipError_t e, const char* what) {
+  return fail(ctx, e == hipErrorOutOfMemory ? DEM_ERR_ALLOC : DEM_ERR_LAUNCH,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define DEM_HIP(ctx, call, what)                     \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+  } while (0)
+
+bool is_pow2(uint32_t v) { return v && !(v & (v - 1u)); }
+
+uint32_t bits_for(uint32_t total_cells) {
+  if (total_cells <= 1) return 0;
+  uint32_t v = total_cells - 1u, b = 0;
+  while (v) { ++b; v >>= 1; }
+  return b;
+}
+
+int check_ctx(dem_context* ctx) {
+  if (!ctx) return DEM_ERR_INVALID;
+  if (!ctx->params_set) return fail(ctx, DEM_ERR_STATE, "parameters not set (call dem_set_params)");
+  return DEM_OK;
+}
+
+int ensure_device(dem_context* ctx) {
+  DEM_HIP(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  return DEM_OK;
+}
+
+template <class T>
+int grow(dem_context* ctx, T** p, uint32_t* cap, uint32_t need) {
+  if (*cap >= need && *p) return DEM_OK;
+  if (*p) { (void)hipFree(*p); *p = nullptr; *cap = 0; }
+  DEM_HIP(ctx, hipMalloc((void**)p, sizeof(T) * (size_t)std::max(need, 1u)), "hipMalloc");
+  *cap = need;
+  return DEM_OK;
+}
+
+void free_sort(SortScratch& s) {
+  if (s.keys_alt) (void)hipFree(s.keys_alt);
+  if (s.vals_alt) (void)hipFree(s.vals_alt);
+  if (s.block_counts) (void)hipFree(s.block_counts);
+  if (s.row_totals) (void)hipFree(s.row_totals);
+  s = SortScratch{};
+}
+
+int alloc_sort(dem_context* ctx, SortScratch& s, uint32_t n) {
+  if (s.capacity >= n && s.keys_alt) return DEM_OK;
+  free_sort(s);
+  const size_t cnt = dem::sort_block_count_elems(std::max(n, 1u));
+  DEM_HIP(ctx, hipMalloc((void**)&s.keys_alt, sizeof(uint32_t) * std::max(n, 1u)), "hipMalloc(sort keys)");
+  DEM_HIP(ctx, hipMalloc((void**)&s.vals_alt, sizeof(uint32_t) * std::max(n, 1u)), "hipMalloc(sort vals)");
+  DEM_HIP(ctx, hipMalloc((void**)&s.block_counts, sizeof(uint32_t) * cnt), "hipMalloc(sort counts)");
+  DEM_HIP(ctx, hipMalloc((void**)&s.row_totals, sizeof(uint32_t) * 256), "hipMalloc(sort totals)");
+  s.capacity = n;
+  return DEM_OK;
+}
+
+int upload_tables(dem_context* ctx) {
+  const uint32_t nt = (uint32_t)ctx->h_mat.size(), ni = (uint32_t)ctx->h_inst.size();
+  if (int r = grow(ctx, &ctx->d_mat, &ctx->mat_cap, nt)) return r;
+  if (int r = grow(ctx, &ctx->d_inst, &ctx->inst_cap, ni)) return r;
+  DEM_HIP(ctx, hipMemcpyAsync(ctx->d_mat, ctx->h_mat.data(), sizeof(Material) * nt, hipMemcpyHostToDevice, ctx->stream),
+          "upload materials");
+  DEM_HIP(ctx, hipMemcpyAsync(ctx->d_inst, ctx->h_inst.data(), sizeof(Instance) * ni, hipMemcpyHostToDevice, ctx->stream),
+          "upload instances");
+  DEM_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  ctx->c.mat = ctx->d_mat;
+  ctx->c.inst = ctx->d_inst;
+  ctx->c.num_types = nt;
+  ctx->c.num_instances = ni;
+  ctx->c.inst0 = ctx->h_inst[0];
+  ctx->generation++;
+  return DEM_OK;
+}
+
+int validate_instances(dem_context* ctx, const std::vector<Instance>& inst, uint32_t flags,
+                       uint32_t* total_cells) {
+  uint64_t top = 0;
+  for (size_t k = 0; k < inst.size(); ++k) {
+    const Instance& in = inst[k];
+    if (in.grid[0] == 0 || in.grid[1] == 0) return fail(ctx, DEM_ERR_INVALID, "grid size must be >= 1");
+    if (!(flags & DEM_GRID_CLAMP) && !(is_pow2(in.grid[0]) && is_pow2(in.grid[1])))
+      return fail(ctx, DEM_ERR_INVALID, "wrapping grids need power-of-two sizes (set DEM_GRID_CLAMP)");
+    if (!(in.extent[0] > 0.0f) || !(in.extent[1] > 0.0f) || !isfinite(in.extent[0]) || !isfinite(in.extent[1]))
+      return fail(ctx, DEM_ERR_INVALID, "grid extent must be finite and > 0");
+    if (!isfinite(in.box[0]) || !isfinite(in.box[1])) return fail(ctx, DEM_ERR_INVALID, "box must be finite");
+    const uint64_t end = (uint64_t)in.cell_base + (uint64_t)in.grid[0] * in.grid[1];
+    if (end >= 0xFFFFFFFFull) return fail(ctx, DEM_ERR_INVALID, "cell keys exceed 32 bits");
+    top = std::max(top, end);
+  }
+  *total_cells = (uint32_t)top;
+  return DEM_OK;
+}
+
+Instance make_instance(const float box[2], const float extent[2], const uint32_t grid[2], uint32_t base) {
+  Instance in{};
+  in.box[0] = box[0];
+  in.box[1] = box[1];
+  in.extent[0] = (extent[0] == 0.0f && extent[1] == 0.0f) ? box[0] : extent[0];
+  in.extent[1] = (extent[0] == 0.0f && extent[1] == 0.0f) ? box[1] : extent[1];
+  in.grid[0] = grid[0];
+  in.grid[1] = grid[1];
+  in.cell_base = base;
+  return in;
+}
+
+int check_cells(dem_context* ctx, uint32_t num_cells) {
+  if (num_cells < ctx->total_cells)
+    return fail(ctx, DEM_ERR_INVALID, "numCells is smaller than the configured grid");
+  return DEM_OK;
+}
+
+bool multi(const dem_context* ctx) { return ctx->c.num_instances > 1; }
+
+__global__ void k_copy2(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                        uint32_t* __restrict__ da, uint32_t* __restrict__ db, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    da[i] = a[i];
+    db[i] = b[i];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dem_abi_version(void) { return DEMSO_ABI_VERSION; }
+
+const char* dem_last_error(const dem_context* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int dem_create(dem_context** out, int device, void* hip_stream) {
+  if (!out) return fail(nullptr, DEM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) return fail(nullptr, DEM_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(nullptr, DEM_ERR_NO_DEVICE, "device index out of range");
+  auto* ctx = new (std::nothrow) dem_context();
+  if (!ctx) return fail(nullptr, DEM_ERR_ALLOC, "host allocation failed");
+  ctx->device = device;
+  ctx->stream = (hipStream_t)hip_stream;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete ctx;
+    return fail(nullptr, DEM_ERR_NO_DEVICE, "hipSetDevice failed");
+  }
+  *out = ctx;
+  return DEM_OK;
+}
+
+int dem_destroy(dem_context* ctx) {
+  if (!ctx) return DEM_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->d_mat) (void)hipFree(ctx->d_mat);
+  if (ctx->d_inst) (void)hipFree(ctx->d_inst);
+  if (ctx->d_copy_tmp) (void)hipFree(ctx->d_copy_tmp);
+  free_sort(ctx->sort);
+  delete ctx;
+  return DEM_OK;
+}
+
+int dem_set_stream(dem_context* ctx, void* hip_stream) {
+  if (!ctx) return DEM_ERR_INVALID;
+  ctx->stream = (hipStream_t)hip_stream;
+  ctx->generation++;
+  return DEM_OK;
+}
+
+int dem_synchronize(dem_context* ctx) {
+  if (!ctx) return DEM_ERR_INVALID;
+  if (int r = ensure_device(ctx)) return r;
+  DEM_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return DEM_OK;
+}
+
+static int convert_materials(dem_context* ctx, const dem_particle_props* types, uint32_t num_types,
+                      uint32_t num_instances, std::vector<Material>& mat, bool& attract) {
+  if (!types) return fail(ctx, DEM_ERR_INVALID, "types is NULL");
+  if (num_types == 0 || num_types > DEM_MAX_TYPES) return fail(ctx, DEM_ERR_INVALID, "num_types out of range");
+  mat.assign(num_types, Material{});
+  attract = false;
+  for (uint32_t t = 0; t < num_types; ++t) {
+    const dem_particle_props& p = types[t];
+    if (p.instance >= num_instances) return fail(ctx, DEM_ERR_INVALID, "a material names a missing instance");
+    Material m{};
+    m.radius = p.radius; m.mass = p.mass; m.inertia = p.inertia;
+    m.kn = p.normal_stiffness; m.kt = p.shear_stiffness; m.cn = p.normal_damping;
+    m.bdamp = p.boundary_damping; m.mu = p.friction_coefficient; m.attract = p.attract_coefficient;
+    m.areal = p.areal_density; m.inst = p.instance;
+    attract = attract || (p.attract_coefficient != 0.0f);
+    mat[t] = m;
+  }
+  return DEM_OK;
+}
+
+int dem_set_params(dem_context* ctx, const dem_system_props* sys, const dem_particle_props* types,
+                   uint32_t num_types) {
+  if (!ctx) return DEM_ERR_INVALID;
+  if (!sys) return fail(ctx, DEM_ERR_INVALID, "sys is NULL");
+  if (!(sys->time_step > 0.0f) || !isfinite(sys->time_step)) return fail(ctx, DEM_ERR_INVALID, "time_step must be > 0");
+  if ((uint64_t)sys->grid_size[0] * sys->grid_size[1] != sys->num_cells)
+    return fail(ctx, DEM_ERR_INVALID, "num_cells != grid_size[0]*grid_size[1]");
+  if (sys->flags & ~(DEM_GRID_CLAMP | DEM_OPEN_TOP)) return fail(ctx, DEM_ERR_INVALID, "unknown flags");
+  std::vector<Material> mat;
+  bool attract = false;
+  if (int r = convert_materials(ctx, types, num_types, 1u, mat, attract)) return r;
+  if (int r = ensure_device(ctx)) return r;
+  std::vector<Instance> inst(1, make_instance(sys->cube_dimension, sys->grid_extent, sys->grid_size, 0));
+  uint32_t total = 0;
+  if (int r = validate_instances(ctx, inst, sys->flags, &total)) return r;
+  ctx->h_mat = mat;
+  ctx->h_inst = inst;
+  ctx->c.dt = sys->time_step;
+  ctx->c.gx = sys->gravity[0];
+  ctx->c.gy = sys->gravity[1];
+  ctx->c.bkn = sys->boundary_normal_stiffness;
+  ctx->c.bkt = sys->boundary_shear_stiffness;
+  ctx->c.flags = sys->flags;
+  ctx->c.attract = attract ? 1u : 0u;
+  ctx->total_cells = total;
+  ctx->key_bits = bits_for(total);
+  if (int r = upload_tables(ctx)) return r;
+  ctx->params_set = true;
+  return DEM_OK;
+}
+
+int dem_set_materials(dem_context* ctx, const dem_particle_props* types, uint32_t num_types) {
+  if (int r = check_ctx(ctx)) return r;
+  std::vector<Material> mat;
+  bool attract = false;
+  if (int r = convert_materials(ctx, types, num_types, (uint32_t)ctx->h_inst.size(), mat, attract)) return r;
+  if (int r = ensure_device(ctx)) return r;
+  ctx->h_mat = mat;
+  ctx->c.attract = attract ? 1u : 0u;
+  return upload_tables(ctx);
+}
+
+int dem_set_instances(dem_context* ctx, const dem_instance* inst, uint32_t num_instances) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!inst || num_instances == 0) return fail(ctx, DEM_ERR_INVALID, "no instances");
+  if (int r = ensure_device(ctx)) return r;
+  std::vector<Instance> v(num_instances);
+  for (uint32_t k = 0; k < num_instances; ++k)
+    v[k] = make_instance(inst[k].cube_dimension, inst[k].grid_extent, inst[k].grid_size, inst[k].cell_base);
+  uint32_t total = 0;
+  if (int r = validate_instances(ctx, v, ctx->c.flags, &total)) return r;
+  for (const Material& m : ctx->h_mat)
+    if (m.inst >= num_instances) return fail(ctx, DEM_ERR_INVALID, "a material names a missing instance");
+  ctx->h_inst = v;
+  ctx->total_cells = total;
+  ctx->key_bits = bits_for(total);
+  return upload_tables(ctx);
+}
+
+int dem_set_gravity(dem_context* ctx, float gx, float gy) {
+  if (int r = check_ctx(ctx)) return r;
+  ctx->c.gx = gx;
+  ctx->c.gy = gy;
+  ctx->generation++;
+  return DEM_OK;
+}
+
+int dem_reserve(dem_context* ctx, uint32_t max_particles) {
+  if (!ctx) return DEM_ERR_INVALID;
+  if (int r = ensure_device(ctx)) return r;
+  return alloc_sort(ctx, ctx->sort, max_particles);
+}
+
+// ---- stages, reference signatures ------------------------------------------
+
+int dem_calc_hash(dem_context* ctx, const float* pos, uint32_t* gridParticleIndex,
+                  uint32_t* gridParticleHash, uint32_t numParticles) {
+  return dem_calc_hash_ex(ctx, pos, nullptr, gridParticleIndex, gridParticleHash, numParticles);
+}
+
+int dem_calc_hash_ex(dem_context* ctx, const float* pos, const uint32_t* type,
+                     uint32_t* gridParticleIndex, uint32_t* gridParticleHash, uint32_t numParticles) {
+  if (int r = check_ctx(ctx)) return r;
+  if (numParticles == 0) return DEM_OK;
+  if (!pos || !gridParticleIndex || !gridParticleHash) return fail(ctx, DEM_ERR_INVALID, "NULL array");
+  if (multi(ctx) && !type) return fail(ctx, DEM_ERR_INVALID, "an ensemble needs the type array to hash");
+  if (int r = ensure_device(ctx)) return r;
+  DEM_HIP(ctx, dem::launch_hash(ctx->c, ctx->stream, ctx->bs_hash, (const float2*)pos, type, gridParticleHash,
+                                gridParticleIndex, numParticles),
+          "hash kernel");
+  return DEM_OK;
+}
+
+int dem_sort_particles(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
+                       uint32_t numParticles) {
+  if (!ctx) return DEM_ERR_INVALID;
+  if (numParticles <= 1) return DEM_OK;
+  if (!gridParticleHash || !gridParticleIndex) return fail(ctx, DEM_ERR_INVALID, "NULL array");
+  if (int r = ensure_device(ctx)) return r;
+  if (int r = alloc_sort(ctx, ctx->sort, numParticles)) return r;
+  // Keys come from hashing, so they are < total_cells; without parameters
+  // the full 32 bits are sorted.
+  const uint32_t bits = ctx->params_set ? ctx->key_bits : 32u;
+  bool in_alt = false;
+  DEM_HIP(ctx, dem::launch_sort(ctx->stream, gridParticleHash, gridParticleIndex, numParticles, bits, ctx->sort, &in_alt),
+          "sort kernels");
+  if (in_alt) {
+    hipLaunchKernelGGL(k_copy2, dim3((numParticles + 255) / 256), dim3(256), 0, ctx->stream, ctx->sort.keys_alt,
+                       ctx->sort.vals_alt, gridParticleHash, gridParticleIndex, numParticles);
+    DEM_HIP(ctx, hipGetLastError(), "sort copy-back");
+  }
+  return DEM_OK;
+}
+
+int dem_reorder_data_and_find_cell_start(
+    dem_context* ctx, uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, float* sortedVel,
+    float* sortedTheta, float* sortedOmega, uint32_t* sortedID, uint32_t* sortedLoc,
+    uint32_t* sortedType, const uint32_t* gridParticleHash, const uint32_t* gridParticleIndex,
+    const float* oldPos, const float* oldVel, const float* oldTheta, const float* oldOmega,
+    const uint32_t* oldID, const uint32_t* oldType, uint32_t numParticles, uint32_t numCells) {
+  return dem_reorder_ex(ctx, cellStart, cellEnd, sortedPos, sortedVel, sortedTheta, sortedOmega, sortedID,
+                        sortedLoc, sortedType, nullptr, gridParticleHash, gridParticleIndex, oldPos, oldVel,
+                        oldTheta, oldOmega, oldID, oldType, nullptr, numParticles, numCells);
+}
+
+int dem_reorder_ex(dem_context* ctx, uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos,
+                   float* sortedVel, float* sortedTheta, float* sortedOmega, uint32_t* sortedID,
+                   uint32_t* sortedLoc, uint32_t* sortedType, float* sortedRadius,
+                   const uint32_t* gridParticleHash, const uint32_t* gridParticleIndex,
+                   const float* oldPos, const float* oldVel, const float* oldTheta,
+                   const float* oldOmega, const uint32_t* oldID, const uint32_t* oldType,
+                   const float* oldRadius, uint32_t numParticles, uint32_t numCells) {
+  if (!ctx) return DEM_ERR_INVALID;
+  if (!cellStart || !cellEnd) return fail(ctx, DEM_ERR_INVALID, "NULL cell array");
+  if ((sortedRadius == nullptr) != (oldRadius == nullptr))
+    return fail(ctx, DEM_ERR_INVALID, "sortedRadius and oldRadius must both be given or both NULL");
+  if (numParticles && (!sortedPos || !sortedVel || !sortedTheta || !sortedOmega || !sortedID || !sortedLoc ||
+                       !sortedType || !gridParticleHash || !gridParticleIndex || !oldPos || !oldVel || !oldTheta ||
+                       !oldOmega || !oldID || !oldType))
+    return fail(ctx, DEM_ERR_INVALID, "NULL array");
+  if (int r = ensure_device(ctx)) return r;
+  DEM_HIP(ctx, dem::launch_fill(ctx->stream, cellStart, dem::kEmpty, numCells), "cell fill kernel");
+  DEM_HIP(ctx, dem::launch_reorder(ctx->stream, ctx->bs_reorder, cellStart, cellEnd, (float2*)sortedPos,
+                                   (float2*)sortedVel, sortedTheta, sortedOmega, sortedID, sortedLoc, sortedType,
+                                   sortedRadius, gridParticleHash, gridParticleIndex, (const float2*)oldPos,
+                                   (const float2*)oldVel, oldTheta, oldOmega, oldID, oldType, oldRadius,
+                                   numParticles, numCells),
+          "reorder kernel");
+  return DEM_OK;
+}
+
+int dem_collide(dem_context* ctx, const float* pos, const float* vel, float* acc, const float* omega,
+                float* alpha, const uint32_t* type, const uint32_t* cellStart, const uint32_t* cellEnd,
+                uint32_t numParticles, uint32_t numCells, float* pressure) {
+  return dem_collide_ex(ctx, pos, vel, acc, omega, alpha, type, nullptr, cellStart, cellEnd, numParticles,
+                        numCells, pressure);
+}
+
+int dem_collide_ex(dem_context* ctx, const float* pos, const float* vel, float* acc, const float* omega,
+                   float* alpha, const uint32_t* type, const float* radius, const uint32_t* cellStart,
+                   const uint32_t* cellEnd, uint32_t numParticles, uint32_t numCells, float* pressure) {
+  if (int r = check_ctx(ctx)) return r;
+  if (int r = check_cells(ctx, numCells)) return r;
+  if (numParticles == 0) return DEM_OK;
+  if (!pos || !vel || !acc || !omega || !alpha || !type || !cellStart || !cellEnd || !pressure)
+    return fail(ctx, DEM_ERR_INVALID, "NULL array");
+  if (int r = ensure_device(ctx)) return r;
+  DEM_HIP(ctx, dem::launch_collide(ctx->c, ctx->stream, ctx->bs_collide, (const float2*)pos, (const float2*)vel,
+                                   (float2*)acc, omega, alpha, type, radius, cellStart, cellEnd, pressure,
+                                   numParticles),
+          "collide kernel");
+  return DEM_OK;
+}
+
+int dem_integrate_system(dem_context* ctx, float* pos, float* vel, const float* acc, float* theta, float* omega,
+                         const float* alpha, const uint32_t* type, float* pressure, uint32_t numParticles) {
+  return dem_integrate_ex(ctx, pos, vel, acc, theta, omega, alpha, type, nullptr, pressure, numParticles);
+}
+
+int dem_integrate_ex(dem_context* ctx, float* pos, float* vel, const float* acc, float* theta, float* omega,
+                     const float* alpha, const uint32_t* type, const float* radius, float* pressure,
+                     uint32_t numParticles) {
+  if (int r = check_ctx(ctx)) return r;
+  if (numParticles == 0) return DEM_OK;
+  if (!pos || !vel || !acc || !theta || !omega || !alpha || !type || !pressure)
+    return fail(ctx, DEM_ERR_INVALID, "NULL array");
+  if (int r = ensure_device(ctx)) return r;
+  DEM_HIP(ctx, dem::launch_integrate(ctx->c, ctx->stream, ctx->bs_integrate, (float2*)pos, (float2*)vel,
+                                     (const float2*)acc, theta, omega, alpha, type, radius, pressure,
+                                     numParticles),
+          "integrate kernel");
+  return DEM_OK;
+}
+
+}  // extern "C"
+
+// ---- owned simulation ------------------------------------------------------
+
+namespace {
+
+template <class T>
+int sim_alloc(dem_sim* m, T** p, size_t count) {
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1));
+  if (e != hipSuccess) return hip_fail(m->ctx, e, "hipMalloc(sim state)");
+  m->allocs.push_back((void*)*p);
+  return DEM_OK;
+}
+
+void sim_free(dem_sim* m) {
+  for (int k = 0; k < 2; ++k)
+    if (m->exec[k]) (void)hipGraphExecDestroy(m->exec[k]);
+  if (m->cap_stream) (void)hipStreamDestroy(m->cap_stream);
+  for (void* p : m->allocs) (void)hipFree(p);
+  free_sort(m->sort);
+  delete m;
+}
+
+// Stage sequence of one step (src/main.cu:431-507) on stream s; `ev` (8
+// events or NULL) brackets each stage for timing.
+hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev) {
+  dem_context* ctx = m->ctx;
+  const StepConst& c = ctx->c;
+  const int o = m->cur, t = o ^ 1;
+  const uint32_t n = m->n;
+  hipError_t e;
+  if (ev) (void)hipEventRecord(ev[0], s);
+  e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[1], s);
+  bool in_alt = false;
+  e = dem::launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt);
+  if (e != hipSuccess) return e;
+  m->sorted_key = in_alt ? m->sort.keys_alt : m->key;
+  m->sorted_idx = in_alt ? m->sort.vals_alt : m->idx;
+  if (ev) (void)hipEventRecord(ev[2], s);
+  e = dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[3], s);
+  e = dem::launch_reorder(s, ctx->bs_reorder, m->cell_start, m->cell_end, m->pos[t], m->vel[t], m->theta[t],
+                          m->omega[t], m->id[t], m->loc[t], m->type[t], m->rad ? m->radius[t] : nullptr,
+                          m->sorted_key, m->sorted_idx, m->pos[o], m->vel[o], m->theta[o], m->omega[o], m->id[o],
+                          m->type[o], m->rad ? m->radius[o] : nullptr, n, m->cells);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[4], s);
+  e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[t], m->vel[t], m->acc, m->omega[t], m->alpha, m->type[t],
+                          m->rad ? m->radius[t] : nullptr, m->cell_start, m->cell_end, m->pressure, n);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[5], s);
+  e = dem::launch_integrate(c, s, ctx->bs_integrate, m->pos[t], m->vel[t], m->acc, m->theta[t], m->omega[t],
+                            m->alpha, m->type[t], m->rad ? m->radius[t] : nullptr, m->pressure, n);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[6], s);
+  m->cur = t;
+  return hipSuccess;
+}
+
+// The sim's cell arrays were sized at creation; a later parameter change
+// must not make hashing produce keys beyond them.
+int sim_check(dem_sim* m) {
+  if (m->ctx->total_cells > m->cells)
+    return fail(m->ctx, DEM_ERR_STATE, "the grid grew beyond the sim's cell arrays; create a new sim");
+  return ensure_device(m->ctx);
+}
+
+int sim_capture(dem_sim* m) {
+  dem_context* ctx = m->ctx;
+  if (!m->cap_stream) DEM_HIP(ctx, hipStreamCreateWithFlags(&m->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (int k = 0; k < 2; ++k) {
+    if (m->exec[k]) {
+      (void)hipGraphExecDestroy(m->exec[k]);
+      m->exec[k] = nullptr;
+    }
+  }
+  const int saved = m->cur;
+  for (int start = 0; start < 2; ++start) {
+    hipGraph_t g = nullptr;
+    DEM_HIP(ctx, hipStreamBeginCapture(m->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    m->cur = start;
+    hipError_t e1 = sim_step_on(m, m->cap_stream, nullptr);
+    hipError_t e2 = e1 == hipSuccess ? sim_step_on(m, m->cap_stream, nullptr) : e1;
+    hipError_t e3 = hipStreamEndCapture(m->cap_stream, &g);
+    m->cur = saved;
+    if (e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(ctx, e2, "capturing the step");
+    }
+    if (e3 != hipSuccess) return hip_fail(ctx, e3, "hipStreamEndCapture");
+    hipError_t e4 = hipGraphInstantiate(&m->exec[start], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e4 != hipSuccess) return hip_fail(ctx, e4, "hipGraphInstantiate");
+  }
+  m->graph_gen = ctx->generation;
+  return DEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_radius, dem_sim** out) {
+  if (!out) return fail(ctx, DEM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (int r = check_ctx(ctx)) return r;
+  if (n == 0) return fail(ctx, DEM_ERR_INVALID, "n must be > 0");
+  if (int r = check_cells(ctx, num_cells)) return r;
+  if (int r = ensure_device(ctx)) return r;
+  auto* m = new (std::nothrow) dem_sim();
+  if (!m) return fail(ctx, DEM_ERR_ALLOC, "host allocation failed");
+  m->ctx = ctx;
+  m->n = n;
+  m->cells = num_cells;
+  m->rad = with_radius != 0;
+  int r = DEM_OK;
+  for (int b = 0; b < 2 && r == DEM_OK; ++b) {
+    if (!r) r = sim_alloc(m, &m->pos[b], n);
+    if (!r) r = sim_alloc(m, &m->vel[b], n);
+    if (!r) r = sim_alloc(m, &m->theta[b], n);
+    if (!r) r = sim_alloc(m, &m->omega[b], n);
+    if (!r) r = sim_alloc(m, &m->id[b], n);
+    if (!r) r = sim_alloc(m, &m->type[b], n);
+    if (!r) r = sim_alloc(m, &m->loc[b], n);
+    if (!r && m->rad) r = sim_alloc(m, &m->radius[b], n);
+  }
+  if (!r) r = sim_alloc(m, &m->acc, n);
+  if (!r) r = sim_alloc(m, &m->alpha, n);
+  if (!r) r = sim_alloc(m, &m->pressure, n);
+  if (!r) r = sim_alloc(m, &m->key, n);
+  if (!r) r = sim_alloc(m, &m->idx, n);
+  if (!r) r = sim_alloc(m, &m->cell_start, num_cells);
+  if (!r) r = sim_alloc(m, &m->cell_end, num_cells);
+  if (!r) r = alloc_sort(ctx, m->sort, n);
+  if (r) {
+    sim_free(m);
+    return r;
+  }
+  // cellEnd starts at 0 as in allocateVectors (src/functions.cu:140).
+  hipError_t e = dem::launch_fill(ctx->stream, m->cell_end, 0u, num_cells);
+  if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->cell_start, dem::kEmpty, num_cells);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    sim_free(m);
+    return hip_fail(ctx, e, "sim init");
+  }
+  m->sorted_key = m->key;
+  m->sorted_idx = m->idx;
+  *out = m;
+  return DEM_OK;
+}
+
+int dem_sim_destroy(dem_sim* m) {
+  if (!m) return DEM_ERR_INVALID;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  sim_free(m);
+  return DEM_OK;
+}
+
+int dem_sim_upload(dem_sim* m, const float* pos, const float* vel, const float* theta, const float* omega,
+                   const uint32_t* id, const uint32_t* type, const uint32_t* loc, const float* radius) {
+  if (!m) return DEM_ERR_INVALID;
+  dem_context* ctx = m->ctx;
+  if (!pos || !vel || !theta || !omega || !id || !type || !loc || (m->rad && !radius))
+    return fail(ctx, DEM_ERR_INVALID, "NULL host array");
+  if (int r = ensure_device(ctx)) return r;
+  const int c = m->cur;
+  const size_t n = m->n;
+  hipStream_t s = ctx->stream;
+  DEM_HIP(ctx, hipMemcpyAsync(m->pos[c], pos, 8 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->vel[c], vel, 8 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->theta[c], theta, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->omega[c], omega, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->id[c], id, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->type[c], type, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->loc[c], loc, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  if (m->rad) DEM_HIP(ctx, hipMemcpyAsync(m->radius[c], radius, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipStreamSynchronize(s), "upload sync");
+  return DEM_OK;
+}
+
+int dem_sim_download(dem_sim* m, float* pos, float* vel, float* theta, float* omega, uint32_t* id,
+                     uint32_t* type, uint32_t* loc, float* radius, float* acc, float* alpha, float* pressure,
+                     uint32_t* hash, uint32_t* index, uint32_t* cellStart, uint32_t* cellEnd) {
+  if (!m) return DEM_ERR_INVALID;
+  dem_context* ctx = m->ctx;
+  if (int r = ensure_device(ctx)) return r;
+  const int c = m->cur;
+  const size_t n = m->n;
+  hipStream_t s = ctx->stream;
+  struct Item { void* dst; const void* src; size_t bytes; } items[] = {
+      {pos, m->pos[c], 8 * n},          {vel, m->vel[c], 8 * n},           {theta, m->theta[c], 4 * n},
+      {omega, m->omega[c], 4 * n},      {id, m->id[c], 4 * n},             {type, m->type[c], 4 * n},
+      {loc, m->loc[c], 4 * n},          {radius, m->rad ? m->radius[c] : nullptr, 4 * n},
+      {acc, m->acc, 8 * n},             {alpha, m->alpha, 4 * n},          {pressure, m->pressure, 4 * n},
+      {hash, m->sorted_key, 4 * n},     {index, m->sorted_idx, 4 * n},
+      {cellStart, m->cell_start, 4 * (size_t)m->cells}, {cellEnd, m->cell_end, 4 * (size_t)m->cells}};
+  for (const Item& it : items)
+    if (it.dst && it.src) DEM_HIP(ctx, hipMemcpyAsync(it.dst, it.src, it.bytes, hipMemcpyDeviceToHost, s), "download");
+  DEM_HIP(ctx, hipStreamSynchronize(s), "download sync");
+  return DEM_OK;
+}
+
+int dem_sim_step(dem_sim* m, uint32_t steps) {
+  if (!m) return DEM_ERR_INVALID;
+  dem_context* ctx = m->ctx;
+  if (int r = sim_check(m)) return r;
+  for (uint32_t k = 0; k < steps; ++k) DEM_HIP(ctx, sim_step_on(m, ctx->stream, nullptr), "step");
+  return DEM_OK;
+}
+
+int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
+  if (!m) return DEM_ERR_INVALID;
+  dem_context* ctx = m->ctx;
+  if (int r = sim_check(m)) return r;
+  if (steps & 1u) {
+    DEM_HIP(ctx, sim_step_on(m, ctx->stream, nullptr), "step");
+    --steps;
+  }
+  if (steps == 0) return DEM_OK;
+  if (!m->exec[0] || m->graph_gen != ctx->generation)
+    if (int r = sim_capture(m)) return r;
+  for (uint32_t k = 0; k < steps; k += 2)
+    DEM_HIP(ctx, hipGraphLaunch(m->exec[m->cur], ctx->stream), "hipGraphLaunch");
+  // Each replay runs two steps, which leaves the state in the same buffer;
+  // the sorted key/index buffers are those of the captured steps.
+  return DEM_OK;
+}
+
+int dem_sim_stage_times(dem_sim* m, uint32_t reps, float* ms) {
+  if (!m || !ms || reps == 0) return m ? fail(m->ctx, DEM_ERR_INVALID, "bad arguments") : DEM_ERR_INVALID;
+  dem_context* ctx = m->ctx;
+  if (int r = sim_check(m)) return r;
+  hipEvent_t ev[7];
+  for (int k = 0; k < 7; ++k) DEM_HIP(ctx, hipEventCreate(&ev[k]), "hipEventCreate");
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  int rc = DEM_OK;
+  for (uint32_t r = 0; r < reps && rc == DEM_OK; ++r) {
+    hipError_t e = sim_step_on(m, ctx->stream, ev);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[6]);
+    if (e != hipSuccess) {
+      rc = hip_fail(ctx, e, "timed step");
+      break;
+    }
+    for (int k = 0; k < 6; ++k) {
+      float t = 0.0f;
+      (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+      acc[k] += t;
+    }
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, ev[0], ev[6]);
+    acc[6] += t;
+  }
+  for (int k = 0; k < 7; ++k) {
+    (void)hipEventDestroy(ev[k]);
+    ms[k] = (float)(acc[k] / reps);
+  }
+  return rc;
+}
+
+int dem_sim_sort_passes(dem_sim* m) {
+  if (!m) return DEM_ERR_INVALID;
+  return dem::sort_passes(m->ctx->key_bits);
+}
+
+int dem_sim_autotune(dem_sim* m, uint32_t* block) {
+  if (!m) return DEM_ERR_INVALID;
+  dem_context* ctx = m->ctx;
+  if (int r = sim_check(m)) return r;
+  hipStream_t s = ctx->stream;
+  const StepConst& c = ctx->c;
+  const int o = m->cur, t = o ^ 1;
+  const uint32_t n = m->n;
+  // Populate the scratch buffer t (the next step's sorted state) once; the
+  // current state in buffer o is never written.
+  bool in_alt = false;
+  DEM_HIP(ctx, dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n), "tune");
+  DEM_HIP(ctx, dem::launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt), "tune");
+  uint32_t* sk = in_alt ? m->sort.keys_alt : m->key;
+  uint32_t* si = in_alt ? m->sort.vals_alt : m->idx;
+  DEM_HIP(ctx, dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells), "tune");
+  auto reorder = [&](int bs) {
+    return dem::launch_reorder(s, bs, m->cell_start, m->cell_end, m->pos[t], m->vel[t], m->theta[t], m->omega[t],
+                               m->id[t], m->loc[t], m->type[t], m->rad ? m->radius[t] : nullptr, sk, si, m->pos[o],
+                               m->vel[o], m->theta[o], m->omega[o], m->id[o], m->type[o],
+                               m->rad ? m->radius[o] : nullptr, n, m->cells);
+  };
+  DEM_HIP(ctx, reorder(ctx->bs_reorder), "tune");
+  hipEvent_t e0, e1;
+  DEM_HIP(ctx, hipEventCreate(&e0), "hipEventCreate");
+  DEM_HIP(ctx, hipEventCreate(&e1), "hipEventCreate");
+  const int cand[] = {64, 128, 256};
+  auto time_it = [&](auto&& launch, int bs, float* out) -> hipError_t {
+    hipError_t e = launch(bs);  // warm
+    if (e != hipSuccess) return e;
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < 5 && e == hipSuccess; ++k) e = launch(bs);
+    (void)hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(out, e0, e1);
+    return e;
+  };
+  int best[4] = {ctx->bs_hash, ctx->bs_reorder, ctx->bs_collide, ctx->bs_integrate};
+  float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  hipError_t e = hipSuccess;
+  for (int bs : cand) {
+    float tm = 0;
+    e = time_it([&](int b) { return dem::launch_hash(c, s, b, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n); }, bs, &tm);
+    if (e != hipSuccess) break;
+    if (tm < best_t[0]) { best_t[0] = tm; best[0] = bs; }
+    e = time_it([&](int b) { return reorder(b); }, bs, &tm);
+    if (e != hipSuccess) break;
+    if (tm < best_t[1]) { best_t[1] = tm; best[1] = bs; }
+    e = time_it([&](int b) {
+          return dem::launch_collide(c, s, b, m->pos[t], m->vel[t], m->acc, m->omega[t], m->alpha, m->type[t],
+                                     m->rad ? m->radius[t] : nullptr, m->cell_start, m->cell_end, m->pressure, n);
+        }, bs, &tm);
+    if (e != hipSuccess) break;
+    if (tm < best_t[2]) { best_t[2] = tm; best[2] = bs; }
+    e = time_it([&](int b) {
+          return dem::launch_integrate(c, s, b, m->pos[t], m->vel[t], m->acc, m->theta[t], m->omega[t], m->alpha,
+                                       m->type[t], m->rad ? m->radius[t] : nullptr, m->pressure, n);
+        }, bs, &tm);
+    if (e != hipSuccess) break;
+    if (tm < best_t[3]) { best_t[3] = tm; best[3] = bs; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
+  ctx->bs_hash = best[0];
+  ctx->bs_reorder = best[1];
+  ctx->bs_collide = best[2];
+  ctx->bs_integrate = best[3];
+  ctx->generation++;
+  if (block) {
+    block[0] = (uint32_t)best[0];
+    block[1] = (uint32_t)best[1];
+    block[2] = (uint32_t)best[2];
+    block[3] = (uint32_t)best[3];
+    block[4] = 16;
+  }
+  return DEM_OK;
+}
+
+}  // extern "C"

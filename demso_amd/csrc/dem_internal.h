@@ -1,0 +1,82 @@
+// dem_internal.h -- types shared by the HIP kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/demso.h"
+
+namespace dem {
+
+constexpr uint32_t kEmpty = DEM_EMPTY_CELL;
+constexpr int kWave = 64;
+
+// Material row as the kernels read it (48 bytes, one row per type).  Fields
+// of ParticleProperties (src/main.cuh:39-54) that the step uses, plus the
+// areal density for per-particle radii and the owning instance.
+struct alignas(16) Material {
+  float radius, mass, inertia;
+  float kn, kt, cn;
+  float bdamp, mu, attract;
+  float areal;
+  uint32_t inst, pad;
+};
+
+// Instance geometry (32 bytes): wall box, hashing extent, grid, key base.
+struct alignas(16) Instance {
+  float box[2];
+  float extent[2];
+  uint32_t grid[2];
+  uint32_t cell_base, pad;
+};
+
+// Uniform step parameters, passed by value to every kernel (kernarg ->
+// scalar registers).  inst0 is instance 0 inlined for the single-system path.
+struct StepConst {
+  float dt, gx, gy, bkn, bkt;
+  uint32_t flags;       // DEM_GRID_CLAMP | DEM_OPEN_TOP
+  uint32_t attract;     // any material has a non-zero attraction coefficient
+  uint32_t num_types;
+  uint32_t num_instances;
+  uint32_t pad;
+  const Material* mat;
+  const Instance* inst;
+  Instance inst0;
+};
+
+// Launch helpers (defined in the .hip files).
+hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float2* pos,
+                       const uint32_t* type, uint32_t* hash, uint32_t* index, uint32_t n);
+hipError_t launch_fill(hipStream_t s, uint32_t* p, uint32_t value, uint32_t count);
+hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32_t* cell_end,
+                          float2* s_pos, float2* s_vel, float* s_theta, float* s_omega,
+                          uint32_t* s_id, uint32_t* s_loc, uint32_t* s_type, float* s_radius,
+                          const uint32_t* hash, const uint32_t* index, const float2* o_pos,
+                          const float2* o_vel, const float* o_theta, const float* o_omega,
+                          const uint32_t* o_id, const uint32_t* o_type, const float* o_radius,
+                          uint32_t n, uint32_t num_cells);
+hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
+                          const float2* vel, float2* acc, const float* omega, float* alpha,
+                          const uint32_t* type, const float* radius, const uint32_t* cell_start,
+                          const uint32_t* cell_end, float* pressure, uint32_t n);
+hipError_t launch_integrate(const StepConst& c, hipStream_t s, int block, float2* pos, float2* vel,
+                            const float2* acc, float* theta, float* omega, const float* alpha,
+                            const uint32_t* type, const float* radius, float* pressure, uint32_t n);
+
+// Radix sort of (key, value) pairs, stable, ascending, on the low key_bits.
+struct SortScratch {
+  uint32_t* keys_alt = nullptr;
+  uint32_t* vals_alt = nullptr;
+  uint32_t* block_counts = nullptr;  // 256 * max_tiles
+  uint32_t* row_totals = nullptr;    // 256
+  uint32_t capacity = 0;             // max keys
+};
+int sort_passes(uint32_t key_bits);
+uint32_t sort_tiles(uint32_t n);
+size_t sort_block_count_elems(uint32_t n);
+// Sorts keys/vals; the result lands in (keys, vals) if *result_in_alt is set
+// to false on return, else in (scratch.keys_alt, scratch.vals_alt).
+hipError_t launch_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits,
+                       const SortScratch& scratch, bool* result_in_alt);
+
+}  // namespace dem

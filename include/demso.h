@@ -1,0 +1,229 @@
+/*
+ * demso.h -- C ABI of the MI355X (gfx950) DemSo time step.
+ *
+ * The five stages of DemSo's step (uniform-grid hash, key/index sort,
+ * reorder + cellStart/cellEnd, collide, integrate) as hand-written HIP
+ * kernels behind plain `extern "C"` functions: device pointers and sizes in,
+ * an int status out.  Reference paths below are relative to the DemSo source
+ * tree (src/...).
+ *
+ * Conventions
+ *   - Every function returns DEM_OK (0) or a negative dem_status; the text of
+ *     the last failure is available from dem_last_error().  Nothing exits.
+ *   - Pointers named pos/vel/acc are float2 arrays passed as float* (x,y
+ *     interleaved, 8 bytes per particle), as in src/functions.cu:326.  All
+ *     other per-particle arrays are 4-byte float or uint32.  All device
+ *     pointers must be 4-byte aligned (8 for float2 arrays).
+ *   - All stage calls are asynchronous on the context's HIP stream and do no
+ *     host synchronisation and no allocation once the context has reserved
+ *     scratch for the sizes in use (dem_reserve), so a step can be captured
+ *     into a hipGraph.  The sort's scratch is grown on demand if a call needs
+ *     more than was reserved.
+ *   - A context is not thread-safe; use one context per host thread/stream.
+ */
+#ifndef DEMSO_H
+#define DEMSO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEMSO_ABI_VERSION 1
+#define DEM_EMPTY_CELL 0xFFFFFFFFu
+#define DEM_MAX_TYPES 65536u
+
+typedef enum dem_status {
+  DEM_OK = 0,
+  DEM_ERR_INVALID = -1,     /* bad argument or inconsistent sizes */
+  DEM_ERR_NO_DEVICE = -2,   /* no HIP device / wrong device index */
+  DEM_ERR_ALLOC = -3,       /* device or host allocation failed */
+  DEM_ERR_LAUNCH = -4,      /* kernel launch or HIP runtime call failed */
+  DEM_ERR_STATE = -5,       /* call order: parameters not set, etc. */
+  DEM_ERR_NONFINITE = -6    /* finite-state guard tripped */
+} dem_status;
+
+/* dem_system_props.flags */
+#define DEM_GRID_CLAMP 0x1u /* clamp-to-edge hashing; out-of-range neighbour
+                               cells are skipped.  Without it the grid wraps
+                               with `& (G-1)` as src/particles_kernel.cuh:127-128
+                               does, which needs power-of-two grid sizes. */
+#define DEM_OPEN_TOP 0x2u   /* no top wall (src/particles_kernel.cuh:611-613 skipped) */
+
+/* Global parameters: SystemProperties (src/main.cuh:108-122), field for
+ * field, followed by two additions. */
+typedef struct dem_system_props {
+  uint32_t num_particles;
+  float cube_dimension[2];   /* wall box [0,Lx] x [0,Ly] */
+  uint32_t grid_size[2];     /* cells per axis */
+  uint32_t num_cells;        /* grid_size[0]*grid_size[1] */
+  float time_step;
+  float gravity[2];
+  float boundary_normal_stiffness;
+  float boundary_shear_stiffness;
+  /* additions */
+  float grid_extent[2];      /* hashing domain: cell = floor(p*G/extent); {0,0} = cube_dimension */
+  uint32_t flags;            /* DEM_GRID_CLAMP | DEM_OPEN_TOP */
+} dem_system_props;
+
+/* Per-type material: ParticleProperties (src/main.cuh:39-54), field for
+ * field, followed by two additions. */
+typedef struct dem_particle_props {
+  float radius;
+  float mass;
+  float inertia;
+  float normal_stiffness;
+  float shear_stiffness;
+  float normal_damping;
+  float boundary_damping;
+  float friction_coefficient;
+  float attract_coefficient;
+  float color_r, color_g, color_b; /* carried, unused by the step */
+  /* additions */
+  float areal_density;  /* with a per-particle radius array: m = (areal*r)*r, I = 2*m*r*r/5 */
+  uint32_t instance;    /* ensemble instance this type belongs to (0 for a single system) */
+} dem_particle_props;
+
+/* One independent system of an ensemble: its own wall box and grid.  Its
+ * cells occupy keys [cell_base, cell_base + grid_size[0]*grid_size[1]). */
+typedef struct dem_instance {
+  float cube_dimension[2];
+  float grid_extent[2];      /* {0,0} = cube_dimension */
+  uint32_t grid_size[2];
+  uint32_t cell_base;
+  uint32_t reserved;
+} dem_instance;
+
+typedef struct dem_context dem_context;
+
+/* ---- context, parameters ------------------------------------------------ */
+
+/* Create a context on HIP device `device`, launching on `hip_stream`
+ * (a hipStream_t; NULL = the null stream). */
+int dem_create(dem_context** out, int device, void* hip_stream);
+int dem_destroy(dem_context* ctx);
+/* Last error text of ctx (ctx NULL: of the last failed dem_create). */
+const char* dem_last_error(const dem_context* ctx);
+int dem_abi_version(void);
+int dem_set_stream(dem_context* ctx, void* hip_stream);
+int dem_synchronize(dem_context* ctx);
+
+/* Replaces the constant-memory upload of allocateVectors
+ * (src/functions.cu:145-147): global parameters plus `num_types` material
+ * rows (the reference allows 10; here up to DEM_MAX_TYPES).  Defines a single
+ * instance from sys (box, grid, extent, cell_base 0). */
+int dem_set_params(dem_context* ctx, const dem_system_props* sys,
+                   const dem_particle_props* types, uint32_t num_types);
+/* Ensemble geometry; replaces the single instance set by dem_set_params.
+ * Material rows name their instance in dem_particle_props.instance. */
+int dem_set_instances(dem_context* ctx, const dem_instance* inst, uint32_t num_instances);
+/* Replace the material rows (same layout and limits as dem_set_params);
+ * each row's `instance` must name an existing instance.  Used to give every
+ * ensemble instance its own material after dem_set_instances. */
+int dem_set_materials(dem_context* ctx, const dem_particle_props* types, uint32_t num_types);
+/* set_gravity (src/functions.cu:663-668). */
+int dem_set_gravity(dem_context* ctx, float gx, float gy);
+/* Pre-allocate sort scratch for up to max_particles (call before capture). */
+int dem_reserve(dem_context* ctx, uint32_t max_particles);
+
+/* ---- the five stages, reference signatures ------------------------------ */
+
+/* calcHash (src/functions.cu:315-327; kernel src/particles_kernel.cuh:133-151):
+ * gridParticleHash[i] = cell key of pos[i], gridParticleIndex[i] = i. */
+int dem_calc_hash(dem_context* ctx, const float* pos, uint32_t* gridParticleIndex,
+                  uint32_t* gridParticleHash, uint32_t numParticles);
+
+/* sortParticles (src/functions.cu:336-341): ascending sort of the hash keys
+ * with the index array as payload, in place.  Stable: equal keys keep their
+ * input order (the reference's thrust::sort_by_key leaves ties unspecified). */
+int dem_sort_particles(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
+                       uint32_t numParticles);
+
+/* reorderDataAndFindCellStart (src/functions.cu:347-393; kernel
+ * src/particles_kernel.cuh:156-233): cellStart := EMPTY for all numCells,
+ * then cellStart/cellEnd of every occupied cell, the sorted copies of the
+ * state and sortedLoc[ID] = i.  cellEnd of empty cells is left untouched, as
+ * in the reference. */
+int dem_reorder_data_and_find_cell_start(
+    dem_context* ctx, uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, float* sortedVel,
+    float* sortedTheta, float* sortedOmega, uint32_t* sortedID, uint32_t* sortedLoc,
+    uint32_t* sortedType, const uint32_t* gridParticleHash, const uint32_t* gridParticleIndex,
+    const float* oldPos, const float* oldVel, const float* oldTheta, const float* oldOmega,
+    const uint32_t* oldID, const uint32_t* oldType, uint32_t numParticles, uint32_t numCells);
+
+/* collide (src/functions.cu:398-468; kernel src/particles_kernel.cuh:459-549
+ * without the controlled-particle term): pressure := 0, then for every
+ * particle the spring-dashpot + Coulomb-capped shear force of all contacts in
+ * the 3x3 cell neighbourhood; acc = F/m, alpha = M/I. */
+int dem_collide(dem_context* ctx, const float* pos, const float* vel, float* acc,
+                const float* omega, float* alpha, const uint32_t* type, const uint32_t* cellStart,
+                const uint32_t* cellEnd, uint32_t numParticles, uint32_t numCells, float* pressure);
+
+/* integrateSystem (src/functions.cu:529-551; kernel
+ * src/particles_kernel.cuh:580-627): symplectic Euler, theta wrap, the
+ * reflect-clamp side/top walls and the bottom-wall law, in place. */
+int dem_integrate_system(dem_context* ctx, float* pos, float* vel, const float* acc, float* theta,
+                         float* omega, const float* alpha, const uint32_t* type, float* pressure,
+                         uint32_t numParticles);
+
+/* ---- extended stages: per-particle radius and ensembles ----------------- */
+/* Same operations; `radius` arrays (may be NULL = radius from the type row)
+ * travel with the state; `type` in dem_calc_hash_ex selects the instance
+ * (may be NULL when there is one instance). */
+int dem_calc_hash_ex(dem_context* ctx, const float* pos, const uint32_t* type,
+                     uint32_t* gridParticleIndex, uint32_t* gridParticleHash, uint32_t numParticles);
+int dem_reorder_ex(dem_context* ctx, uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos,
+                   float* sortedVel, float* sortedTheta, float* sortedOmega, uint32_t* sortedID,
+                   uint32_t* sortedLoc, uint32_t* sortedType, float* sortedRadius,
+                   const uint32_t* gridParticleHash, const uint32_t* gridParticleIndex,
+                   const float* oldPos, const float* oldVel, const float* oldTheta,
+                   const float* oldOmega, const uint32_t* oldID, const uint32_t* oldType,
+                   const float* oldRadius, uint32_t numParticles, uint32_t numCells);
+int dem_collide_ex(dem_context* ctx, const float* pos, const float* vel, float* acc,
+                   const float* omega, float* alpha, const uint32_t* type, const float* radius,
+                   const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t numParticles,
+                   uint32_t numCells, float* pressure);
+int dem_integrate_ex(dem_context* ctx, float* pos, float* vel, const float* acc, float* theta,
+                     float* omega, const float* alpha, const uint32_t* type, const float* radius,
+                     float* pressure, uint32_t numParticles);
+
+/* ---- owned simulation: the step loop of src/main.cu:396-507 -------------- */
+
+typedef struct dem_sim dem_sim;
+
+/* Device state for n particles on a grid of num_cells total keys, double
+ * buffered as in the reference (src/functions.cu:68-99).  with_radius != 0
+ * adds a per-particle radius array.  Parameters come from ctx, which must
+ * outlive the sim. */
+int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_radius, dem_sim** out);
+int dem_sim_destroy(dem_sim* sim);
+/* Host arrays -> current device state (radius may be NULL without radius). */
+int dem_sim_upload(dem_sim* sim, const float* pos, const float* vel, const float* theta,
+                   const float* omega, const uint32_t* id, const uint32_t* type, const uint32_t* loc,
+                   const float* radius);
+/* Current device state -> host arrays; any pointer may be NULL. */
+int dem_sim_download(dem_sim* sim, float* pos, float* vel, float* theta, float* omega, uint32_t* id,
+                     uint32_t* type, uint32_t* loc, float* radius, float* acc, float* alpha,
+                     float* pressure, uint32_t* hash, uint32_t* index, uint32_t* cellStart,
+                     uint32_t* cellEnd);
+/* Run `steps` steps with eager launches on the context stream. */
+int dem_sim_step(dem_sim* sim, uint32_t steps);
+/* Run `steps` steps by replaying a captured hipGraph of the step sequence. */
+int dem_sim_step_graph(dem_sim* sim, uint32_t steps);
+/* Average device time per stage over `reps` eager steps, measured with HIP
+ * events on the context stream: ms[0]=hash ms[1]=sort ms[2]=cell fill
+ * ms[3]=reorder ms[4]=collide ms[5]=integrate ms[6]=whole step. */
+int dem_sim_stage_times(dem_sim* sim, uint32_t reps, float* ms);
+/* Choose block sizes per kernel by timing candidates on the current state
+ * (the state is restored afterwards).  Writes the choice to block[5]
+ * (hash, reorder, collide, integrate, sort items-per-thread) if non-NULL. */
+int dem_sim_autotune(dem_sim* sim, uint32_t* block);
+/* Number of radix passes the sort uses for this sim's key range. */
+int dem_sim_sort_passes(dem_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEMSO_H */

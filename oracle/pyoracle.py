@@ -1,0 +1,238 @@
+"""ctypes wrapper of liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The CPU restatement of the reference kernels (dem_oracle.hpp).  Imported by
+tests, by the smoke check and by the benchmark's CPU-baseline leg, never by
+the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from ctypes import POINTER, c_double, c_float, c_int, c_uint32, c_void_p
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+class OrcMaterial(ctypes.Structure):
+    _fields_ = [(f, c_float) for f in ("radius", "mass", "inertia", "kn", "kt", "cn", "bdamp", "mu", "attract",
+                                       "areal")] + [("inst", c_uint32), ("pad", c_uint32)]
+
+
+class OrcInstance(ctypes.Structure):
+    _fields_ = [("box", c_float * 2), ("extent", c_float * 2), ("grid", c_uint32 * 2), ("cell_base", c_uint32),
+                ("pad", c_uint32)]
+
+
+class OrcSystem(ctypes.Structure):
+    _fields_ = [("dt", c_float), ("g", c_float * 2), ("bkn", c_float), ("bkt", c_float), ("flags", c_uint32),
+                ("num_types", c_uint32), ("num_instances", c_uint32), ("mat", c_void_p), ("inst", c_void_p)]
+
+
+P = c_void_p
+_lib = None
+
+
+def build() -> Path:
+    res = subprocess.run(["make", "-C", str(HERE), "liboracle.so"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    lib = ctypes.CDLL(str(LIB_PATH))
+    proto = {
+        "orc_abi_version": (c_int, []),
+        "orc_set_threads": (c_int, [c_int]),
+        "orc_sizeof_system": (c_int, []),
+        "orc_sizeof_material": (c_int, []),
+        "orc_sizeof_instance": (c_int, []),
+        "orc_calc_hash": (c_int, [POINTER(OrcSystem), P, P, P, P, c_uint32]),
+        "orc_sort_particles": (c_int, [P, P, c_uint32]),
+        "orc_reorder": (c_int, [P] * 19 + [c_uint32, c_uint32]),
+        "orc_collide": (c_int, [POINTER(OrcSystem)] + [P] * 9 + [c_uint32, P]),
+        "orc_integrate": (c_int, [POINTER(OrcSystem)] + [P] * 9 + [c_uint32]),
+        "orc_sim_create_f32": (c_void_p, [POINTER(OrcSystem), c_uint32, c_uint32, c_uint32, c_uint32, c_int]),
+        "orc_sim_create_f64": (c_void_p, [POINTER(OrcSystem), c_uint32, c_uint32, c_uint32, c_uint32, c_int]),
+        "orc_sim_destroy_f32": (None, [c_void_p]),
+        "orc_sim_destroy_f64": (None, [c_void_p]),
+        "orc_sim_set_state_f32": (c_int, [c_void_p] + [P] * 8),
+        "orc_sim_set_state_f64": (c_int, [c_void_p] + [P] * 8),
+        "orc_sim_step_f32": (c_double, [c_void_p, c_uint32]),
+        "orc_sim_step_f64": (c_double, [c_void_p, c_uint32]),
+        "orc_sim_get_state_f32": (c_int, [c_void_p] + [P] * 15),
+        "orc_sim_get_state_f64": (c_int, [c_void_p] + [P] * 15),
+    }
+    for name, (res, args) in proto.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    assert lib.orc_sizeof_system() == ctypes.sizeof(OrcSystem)
+    assert lib.orc_sizeof_material() == ctypes.sizeof(OrcMaterial)
+    assert lib.orc_sizeof_instance() == ctypes.sizeof(OrcInstance)
+    _lib = lib
+    return lib
+
+
+def set_threads(t: int) -> int:
+    return load().orc_set_threads(t)
+
+
+def _p(a: Optional[np.ndarray]) -> Optional[int]:
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class System:
+    """orc::System built from a demso_amd.scene.Scene (keeps its tables alive)."""
+
+    def __init__(self, scene):
+        self.mat = (OrcMaterial * len(scene.materials))()
+        for k, m in enumerate(scene.materials):
+            r = self.mat[k]
+            r.radius, r.mass, r.inertia, r.kn, r.kt, r.cn = m.radius, m.mass, m.inertia, m.kn, m.kt, m.cn
+            r.bdamp, r.mu, r.attract, r.areal, r.inst = m.bdamp, m.mu, m.attract, m.areal, m.instance
+        self.inst = (OrcInstance * len(scene.instances))()
+        for k, g in enumerate(scene.instances):
+            self.inst[k].box[:] = g.box
+            self.inst[k].extent[:] = g.extent
+            self.inst[k].grid[:] = g.grid
+            self.inst[k].cell_base = g.cell_base
+        s = OrcSystem()
+        s.dt = scene.dt
+        s.g[:] = scene.gravity
+        s.bkn, s.bkt, s.flags = scene.bkn, scene.bkt, scene.flags
+        s.num_types, s.num_instances = len(scene.materials), len(scene.instances)
+        s.mat = ctypes.cast(self.mat, c_void_p).value
+        s.inst = ctypes.cast(self.inst, c_void_p).value
+        self.sys = s
+        self.total_cells = scene.total_cells
+        self.multi = len(scene.instances) > 1
+
+
+# ---- single stages (FP32) -------------------------------------------------------
+
+def calc_hash(system: System, pos: np.ndarray, type: Optional[np.ndarray] = None):
+    lib = load()
+    n = pos.shape[0]
+    h = np.empty(n, np.uint32)
+    i = np.empty(n, np.uint32)
+    lib.orc_calc_hash(ctypes.byref(system.sys), _p(pos), _p(type), _p(h), _p(i), n)
+    return h, i
+
+
+def sort_particles(hash_: np.ndarray, index: np.ndarray):
+    h, i = hash_.copy(), index.copy()
+    load().orc_sort_particles(_p(h), _p(i), h.shape[0])
+    return h, i
+
+
+def reorder(hash_, index, old: dict, num_cells: int, cell_end_init: Optional[np.ndarray] = None):
+    """Returns (cellStart, cellEnd, sorted) with sorted a dict of arrays."""
+    lib = load()
+    n = hash_.shape[0]
+    cs = np.empty(num_cells, np.uint32)
+    ce = np.zeros(num_cells, np.uint32) if cell_end_init is None else cell_end_init.copy()
+    rad = old.get("radius")
+    s = {
+        "pos": np.empty((n, 2), np.float32), "vel": np.empty((n, 2), np.float32),
+        "theta": np.empty(n, np.float32), "omega": np.empty(n, np.float32), "id": np.empty(n, np.uint32),
+        "loc": np.zeros(n, np.uint32), "type": np.empty(n, np.uint32),
+        "radius": None if rad is None else np.empty(n, np.float32),
+    }
+    lib.orc_reorder(_p(cs), _p(ce), _p(s["pos"]), _p(s["vel"]), _p(s["theta"]), _p(s["omega"]), _p(s["id"]),
+                    _p(s["loc"]), _p(s["type"]), _p(s["radius"]), _p(hash_), _p(index), _p(old["pos"]),
+                    _p(old["vel"]), _p(old["theta"]), _p(old["omega"]), _p(old["id"]), _p(old["type"]), _p(rad),
+                    n, num_cells)
+    return cs, ce, s
+
+
+def collide(system: System, st: dict, cell_start, cell_end):
+    lib = load()
+    n = st["pos"].shape[0]
+    acc = np.empty((n, 2), np.float32)
+    alpha = np.empty(n, np.float32)
+    pres = np.empty(n, np.float32)
+    lib.orc_collide(ctypes.byref(system.sys), _p(st["pos"]), _p(st["vel"]), _p(acc), _p(st["omega"]), _p(alpha),
+                    _p(st["type"]), _p(st.get("radius")), _p(cell_start), _p(cell_end), n, _p(pres))
+    return acc, alpha, pres
+
+
+def integrate(system: System, st: dict, acc, alpha, pressure):
+    """Returns a new state dict (pos, vel, theta, omega) and the updated pressure."""
+    lib = load()
+    n = st["pos"].shape[0]
+    out = {k: st[k].copy() for k in ("pos", "vel", "theta", "omega")}
+    pres = pressure.copy()
+    lib.orc_integrate(ctypes.byref(system.sys), _p(out["pos"]), _p(out["vel"]), _p(acc), _p(out["theta"]),
+                      _p(out["omega"]), _p(alpha), _p(st["type"]), _p(st.get("radius")), _p(pres), n)
+    return out, pres
+
+
+# ---- owned simulation ----------------------------------------------------------------
+
+class Sim:
+    """Whole steps on the CPU (precision 'f32' or 'f64')."""
+
+    def __init__(self, scene, precision: str = "f32"):
+        assert precision in ("f32", "f64")
+        self.lib = load()
+        self.prec = precision
+        self.system = System(scene)
+        self.n = scene.n
+        self.cells = scene.total_cells
+        self.with_radius = scene.radius is not None
+        create = getattr(self.lib, f"orc_sim_create_{precision}")
+        self.h = create(ctypes.byref(self.system.sys), len(scene.materials), len(scene.instances), self.n,
+                        self.cells, int(self.with_radius))
+        if not self.h:
+            raise MemoryError("oracle sim allocation failed")
+        rc = getattr(self.lib, f"orc_sim_set_state_{precision}")(
+            self.h, _p(scene.pos), _p(scene.vel), _p(scene.theta), _p(scene.omega), _p(scene.id), _p(scene.type),
+            _p(scene.loc), _p(scene.radius))
+        if rc != 0:
+            raise ValueError("oracle sim: radius array missing")
+
+    def close(self):
+        if self.h:
+            getattr(self.lib, f"orc_sim_destroy_{self.prec}")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, steps: int = 1) -> float:
+        """Runs `steps` steps; returns the wall time in seconds."""
+        return float(getattr(self.lib, f"orc_sim_step_{self.prec}")(self.h, steps))
+
+    def state(self) -> dict:
+        n, c = self.n, self.cells
+        f = np.float32 if self.prec == "f32" else np.float64
+        out = {
+            "pos": np.empty((n, 2), f), "vel": np.empty((n, 2), f), "theta": np.empty(n, f),
+            "omega": np.empty(n, f), "id": np.empty(n, np.uint32), "type": np.empty(n, np.uint32),
+            "loc": np.empty(n, np.uint32), "radius": np.empty(n, f) if self.with_radius else None,
+            "acc": np.empty((n, 2), f), "alpha": np.empty(n, f), "pressure": np.empty(n, f),
+            "hash": np.empty(n, np.uint32), "index": np.empty(n, np.uint32), "cellStart": np.empty(c, np.uint32),
+            "cellEnd": np.empty(c, np.uint32),
+        }
+        g = lambda k: _p(out.get(k))
+        getattr(self.lib, f"orc_sim_get_state_{self.prec}")(
+            self.h, g("pos"), g("vel"), g("theta"), g("omega"), g("id"), g("type"), g("loc"), g("radius"),
+            g("acc"), g("alpha"), g("pressure"), g("hash"), g("index"), g("cellStart"), g("cellEnd"))
+        return out
