@@ -16,10 +16,10 @@ namespace dem {
 namespace {
 
 __device__ __forceinline__ int to_int_sat(float g) {
-  if (!(g == g)) return 0;
-  if (g >= 2147483648.0f) return 2147483647;
-  if (g < -2147483648.0f) return -2147483647 - 1;
-  return (int)g;
+  // NaN -> 0, saturation at the int range, plain truncation inside it
+  int r = (int)fminf(fmaxf(g, -2147483648.0f), 2147483520.0f);
+  r = (g >= 2147483648.0f) ? 2147483647 : r;
+  return (g == g) ? r : 0;
 }
 
 __device__ __forceinline__ uint32_t clamp_idx(uint32_t t, uint32_t count) {
@@ -100,14 +100,14 @@ __global__ __launch_bounds__(256) void k_hash(StepConst c, const float2* __restr
 // -------------------------------------------------------------- cell fill --
 // cudaMemset(cellStart, 0xff) of src/functions.cu:371 as a kernel (a memset
 // node in a captured graph is avoided on purpose).
-__global__ __launch_bounds__(256) void k_fill4(uint4* __restrict__ p, uint32_t v, uint32_t n4) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n4) p[i] = make_uint4(v, v, v, v);
-}
-
-__global__ __launch_bounds__(256) void k_fill1(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+__global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint32_t v, uint32_t n, int vec) {
+  const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) * 4u;
+  if (i >= n) return;
+  if (vec && i + 3u < n) {
+    *reinterpret_cast<uint4*>(p + i) = make_uint4(v, v, v, v);
+  } else {
+    for (uint32_t k = i; k < n && k < i + 4u; ++k) p[k] = v;
+  }
 }
 
 // --------------------------------------------------------------- reorder --
@@ -155,11 +155,119 @@ __global__ __launch_bounds__(256) void k_reorder(
 
 // --------------------------------------------------------------- collide --
 // collideD (src/particles_kernel.cuh:459-549) with collideCell (:345-385),
-// collideSpheres (:235-307) and attractSpheres (:310-342) inlined.  The
-// neighbour's velocity, spin and material are only loaded for actual
-// contacts (the reference loads them for every candidate; the values used
-// are the same).  Per-cell partial sums are kept, as collideCell returns one
-// float2 per cell that collideD adds up.
+// collideSpheres (:235-307) and attractSpheres (:310-342) inlined.
+//
+// The kernel is VALU-bound (it runs ~10^3 vector instructions per wave), so
+// the candidate loop is kept lean:
+//  * the three cells of a row are consecutive keys, so their candidates are
+//    one contiguous range of the sorted arrays; collideCell returns one
+//    partial float2 per cell that collideD adds up, so the partial sum is
+//    flushed when the walk crosses a cell end (flushing a zero sum changes
+//    nothing: the sums start at +0 and never become -0).  Where a wrapping
+//    grid makes the row non-contiguous (edge columns), the row's three cells
+//    are walked one by one in the reference order;
+//  * most candidates are not in contact: d2 = rx*rx+ry*ry is the float the
+//    reference takes sqrtf of, and if d2 > fl(R*R)*(1+2^-20) then
+//    sqrtf(d2) >= R for certain (fl(R*R)*(1+2^-20) exceeds the square of
+//    the midpoint between R and its predecessor while R*R is a normal float,
+//    i.e. R >= 2^-63; smaller R always takes the exact path), so the correctly rounded
+//    sqrt and the contact law run only for candidates that may touch; the
+//    contact decision itself is the reference's dist < R;
+//  * the neighbour's velocity, spin and material are loaded only for actual
+//    contacts (the reference loads them for every candidate; the values used
+//    are the same).
+constexpr uint32_t kNoBoundary = 0xFFFFFFFFu;
+
+struct PairCtx {
+  float2 p, v;
+  float w, rA, dt;
+  double den;
+  Material mA;
+};
+
+template <bool kRadius>
+__device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j,
+                                          const float2* __restrict__ pos, const float2* __restrict__ vel,
+                                          const float* __restrict__ omega, const uint32_t* __restrict__ type,
+                                          const float* __restrict__ radius, float& cfx, float& cfy,
+                                          float& moment, float& pres) {
+  const float2 pB = pos[j];
+  uint32_t tB = 0;
+  float rB;
+  if (kRadius) {
+    rB = radius[j];
+  } else {
+    tB = clamp_idx(type[j], c.num_types);
+    rB = c.mat[tB].radius;
+  }
+  const float rx = pB.x - q.p.x, ry = pB.y - q.p.y;
+  const float d2 = rx * rx + ry * ry;
+  const float R = q.rA + rB;
+  const bool maybe = !(d2 > (R * R) * 1.00000095367431640625f) || R < 1.0842022e-19f;
+  if (!(maybe || c.attract)) return;
+  const float dist = sqrtf(d2);
+  if (kRadius) tB = clamp_idx(type[j], c.num_types);
+  if (dist < R) {
+    const float kBn = c.mat[tB].kn, kBt = c.mat[tB].kt, cBn = c.mat[tB].cn, muB = c.mat[tB].mu;
+    const float2 vB = vel[j];
+    const float wB = omega[j];
+    const float nx = rx / dist, ny = ry / dist;
+    const float tx = -ny, ty = nx;
+    const float rvx = vB.x - q.v.x, rvy = vB.y - q.v.y;
+    const float vn = rvx * nx + rvy * ny;
+    const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
+    const float kn = series(q.mA.kn, kBn);
+    const float kt = series(q.mA.kt, kBt);
+    const float cn = series(q.mA.cn, cBn);
+    const float mu = (q.mA.mu + muB) / 2;
+    const float s = -kn * (R - dist);
+    float ffx = s * nx, ffy = s * ny;
+    if (vn < 0.0f) {
+      const float s2 = cn * vn;
+      ffx += s2 * nx;
+      ffy += s2 * ny;
+    }
+    const float flen = sqrtf(ffx * ffx + ffy * ffy);
+    pres = add_pressure(pres, flen, q.den);
+    if (mu != 0.0f) {
+      const float sr = q.rA * q.w + rB * wB;
+      const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
+      float ftx = (kt * cvx) * q.dt, fty = (kt * cvy) * q.dt;
+      const float ftmax = mu * flen;
+      if (!(sqrtf(ftx * ftx + fty * fty) < ftmax)) {
+        const float cl = sqrtf(cvx * cvx + cvy * cvy);
+        ftx = (ftmax * cvx) / cl;
+        fty = (ftmax * cvy) / cl;
+      }
+      ffx += ftx;
+      ffy += fty;
+      moment += q.rA * (ftx * tx + fty * ty);
+    }
+    cfx += ffx;
+    cfy += ffy;
+  }
+  if (c.attract) {
+    const float aA = q.mA.attract, aB = c.mat[tB].attract;
+    const float adA = (float)((double)(q.rA * 2.0f) * 1.2);
+    const float adB = (float)((double)(rB * 2.0f) * 1.2);
+    const float nx = rx / dist, ny = ry / dist;
+    float afx = 0.0f, afy = 0.0f;
+    if (dist < adA) {
+      const float s = aA * (adA - dist);
+      afx += s * nx;
+      afy += s * ny;
+    }
+    if (dist < adB) {
+      const float s = aB * (adB - dist);
+      afx += s * nx;
+      afy += s * ny;
+    }
+    pres = add_pressure(pres, sqrtf(afx * afx + afy * afy), q.den);
+    cfx += afx;
+    cfy += afy;
+  }
+}
+
 template <bool kRadius, bool kMulti>
 __global__ __launch_bounds__(256) void k_collide(
     StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,
@@ -170,119 +278,83 @@ __global__ __launch_bounds__(256) void k_collide(
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
 
+  PairCtx q;
   const uint32_t tA = clamp_idx(type[i], c.num_types);
-  const Material mA = c.mat[tA];
-  const Instance in = kMulti ? c.inst[clamp_idx(mA.inst, c.num_instances)] : c.inst0;
-  float rA, massA, inertiaA;
-  body<kRadius>(mA, radius, i, rA, massA, inertiaA);
-  const float2 p = pos[i];
-  const float2 v = vel[i];
-  const float w = omega[i];
-  const double den = pressure_den(rA);
-  const float dt = c.dt;
+  q.mA = c.mat[tA];
+  const Instance in = kMulti ? c.inst[clamp_idx(q.mA.inst, c.num_instances)] : c.inst0;
+  float massA, inertiaA;
+  body<kRadius>(q.mA, radius, i, q.rA, massA, inertiaA);
+  q.p = pos[i];
+  q.v = vel[i];
+  q.w = omega[i];
+  q.den = pressure_den(q.rA);
+  q.dt = c.dt;
   const bool clamp = c.flags & DEM_GRID_CLAMP;
+  const int G0 = (int)in.grid[0], G1 = (int)in.grid[1];
 
   int gx, gy;
-  grid_pos(in, p, gx, gy);
+  grid_pos(in, q.p, gx, gy);
   if (clamp) clamp_cell(in, gx, gy);
+  // x columns of a row: clamp -> out-of-range skipped; wrap -> & (G-1)
+  const bool vx0 = !clamp || gx > 0, vx2 = !clamp || gx < G0 - 1;
+  const uint32_t x0 = ((uint32_t)gx - 1u) & (clamp ? 0xFFFFFFFFu : (uint32_t)G0 - 1u);
+  const uint32_t x1 = clamp ? (uint32_t)gx : ((uint32_t)gx & ((uint32_t)G0 - 1u));
+  const uint32_t x2 = ((uint32_t)gx + 1u) & (clamp ? 0xFFFFFFFFu : (uint32_t)G0 - 1u);
+  // a wrap that moves a column to the other edge breaks key contiguity
+  const bool contiguous = clamp || (x0 + 1u == x1 && x1 + 1u == x2);
 
   float fx = 0.0f, fy = 0.0f, moment = 0.0f, pres = 0.0f;
   for (int y = -1; y <= 1; ++y) {
-    for (int x = -1; x <= 1; ++x) {
-      uint32_t cx, cy;
-      if (clamp) {
-        const int ix = gx + x, iy = gy + y;
-        if (ix < 0 || iy < 0 || ix >= (int)in.grid[0] || iy >= (int)in.grid[1]) continue;
-        cx = (uint32_t)ix;
-        cy = (uint32_t)iy;
+    uint32_t cy;
+    if (clamp) {
+      const int iy = gy + y;
+      if (iy < 0 || iy >= G1) continue;
+      cy = (uint32_t)iy;
+    } else {
+      cy = ((uint32_t)gy + (uint32_t)y) & ((uint32_t)G1 - 1u);
+    }
+    const uint32_t row = in.cell_base + cy * (uint32_t)G0;
+    uint32_t s0 = vx0 ? cell_start[row + x0] : kEmpty;
+    uint32_t s1 = cell_start[row + x1];
+    uint32_t s2 = vx2 ? cell_start[row + x2] : kEmpty;
+    const bool n0 = s0 != kEmpty, n1 = s1 != kEmpty, n2 = s2 != kEmpty;
+    const uint32_t e0 = n0 ? min(cell_end[row + x0], n) : 0u;
+    const uint32_t e1 = n1 ? min(cell_end[row + x1], n) : 0u;
+    const uint32_t e2 = n2 ? min(cell_end[row + x2], n) : 0u;
+    if (!n0) s0 = 0;
+    if (!n1) s1 = 0;
+    if (!n2) s2 = 0;
+    // contiguous row: one range [rs, re) with cell ends b0, b1 inside it;
+    // otherwise the three cells one by one, in the reference order
+    const int nseg = contiguous ? 1 : 3;
+    for (int sg = 0; sg < nseg; ++sg) {
+      uint32_t rs, re, b0, b1;
+      if (contiguous) {
+        rs = n0 ? s0 : (n1 ? s1 : s2);
+        re = n2 ? e2 : (n1 ? e1 : e0);
+        b0 = n0 ? e0 : kNoBoundary;
+        b1 = n1 ? e1 : kNoBoundary;
       } else {
-        cx = ((uint32_t)gx + (uint32_t)x) & (in.grid[0] - 1u);
-        cy = ((uint32_t)gy + (uint32_t)y) & (in.grid[1] - 1u);
+        rs = sg == 0 ? s0 : (sg == 1 ? s1 : s2);
+        re = sg == 0 ? e0 : (sg == 1 ? e1 : e2);
+        b0 = kNoBoundary;
+        b1 = kNoBoundary;
       }
-      const uint32_t key = in.cell_base + cy * in.grid[0] + cx;
-      const uint32_t st = cell_start[key];
-      if (st == kEmpty) continue;
-      const uint32_t en = min(cell_end[key], n);
       float cfx = 0.0f, cfy = 0.0f;
-      for (uint32_t j = st; j < en; ++j) {
-        if (j == i) continue;
-        const float2 pB = pos[j];
-        uint32_t tB = 0;
-        float rB;
-        if (kRadius) {
-          rB = radius[j];
-        } else {
-          tB = clamp_idx(type[j], c.num_types);
-          rB = c.mat[tB].radius;
+      for (uint32_t j = rs; j < re; ++j) {
+        if (j == b0 || j == b1) {
+          fx += cfx;
+          fy += cfy;
+          cfx = 0.0f;
+          cfy = 0.0f;
         }
-        const float rx = pB.x - p.x, ry = pB.y - p.y;
-        const float dist = sqrtf(rx * rx + ry * ry);
-        const float R = rA + rB;
-        if (kRadius && (dist < R || c.attract)) tB = clamp_idx(type[j], c.num_types);
-        if (dist < R) {
-          const float kBn = c.mat[tB].kn, kBt = c.mat[tB].kt, cBn = c.mat[tB].cn, muB = c.mat[tB].mu;
-          const float2 vB = vel[j];
-          const float wB = omega[j];
-          const float nx = rx / dist, ny = ry / dist;
-          const float tx = -ny, ty = nx;
-          const float rvx = vB.x - v.x, rvy = vB.y - v.y;
-          const float vn = rvx * nx + rvy * ny;
-          const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
-          const float kn = series(mA.kn, kBn);
-          const float kt = series(mA.kt, kBt);
-          const float cn = series(mA.cn, cBn);
-          const float mu = (mA.mu + muB) / 2;
-          const float s = -kn * (R - dist);
-          float ffx = s * nx, ffy = s * ny;
-          if (vn < 0.0f) {
-            const float s2 = cn * vn;
-            ffx += s2 * nx;
-            ffy += s2 * ny;
-          }
-          const float flen = sqrtf(ffx * ffx + ffy * ffy);
-          pres = add_pressure(pres, flen, den);
-          if (mu != 0.0f) {
-            const float sr = rA * w + rB * wB;
-            const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
-            float ftx = (kt * cvx) * dt, fty = (kt * cvy) * dt;
-            const float ftmax = mu * flen;
-            if (!(sqrtf(ftx * ftx + fty * fty) < ftmax)) {
-              const float cl = sqrtf(cvx * cvx + cvy * cvy);
-              ftx = (ftmax * cvx) / cl;
-              fty = (ftmax * cvy) / cl;
-            }
-            ffx += ftx;
-            ffy += fty;
-            moment += rA * (ftx * tx + fty * ty);
-          }
-          cfx += ffx;
-          cfy += ffy;
-        }
-        if (c.attract) {
-          const float aA = mA.attract, aB = c.mat[tB].attract;
-          const float adA = (float)((double)(rA * 2.0f) * 1.2);
-          const float adB = (float)((double)(rB * 2.0f) * 1.2);
-          const float nx = rx / dist, ny = ry / dist;
-          float afx = 0.0f, afy = 0.0f;
-          if (dist < adA) {
-            const float s = aA * (adA - dist);
-            afx += s * nx;
-            afy += s * ny;
-          }
-          if (dist < adB) {
-            const float s = aB * (adB - dist);
-            afx += s * nx;
-            afy += s * ny;
-          }
-          pres = add_pressure(pres, sqrtf(afx * afx + afy * afy), den);
-          cfx += afx;
-          cfy += afy;
-        }
+        if (j != i) candidate<kRadius>(c, q, j, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
       }
       fx += cfx;
       fy += cfy;
     }
   }
+
   acc[i] = make_float2(fx / massA, fy / massA);
   alpha[i] = moment / inertiaA;
   pressure[i] = pres;
@@ -410,14 +482,9 @@ hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float
 
 hipError_t launch_fill(hipStream_t s, uint32_t* p, uint32_t value, uint32_t count) {
   if (count == 0) return hipSuccess;
-  uint32_t head = 0;
-  if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
-    const uint32_t n4 = count / 4u;
-    if (n4) hipLaunchKernelGGL(k_fill4, grid_for(n4, 256), dim3(256), 0, s, reinterpret_cast<uint4*>(p), value, n4);
-    head = n4 * 4u;
-  }
-  if (count > head)
-    hipLaunchKernelGGL(k_fill1, grid_for(count - head, 256), dim3(256), 0, s, p + head, value, count - head);
+  const int vec = (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+  const uint32_t threads = (count + 3u) / 4u;
+  hipLaunchKernelGGL(k_fill, grid_for(threads, 256), dim3(256), 0, s, p, value, count, vec);
   return hipGetLastError();
 }
 
