@@ -179,11 +179,16 @@ def main() -> int:
     sim.close()
     ctx.close()
     if args.calibration_copy_mb > 0:
+        # a plain copy of known size whose source is not cache-resident: the
+        # source is written, then 1 GiB of other data evicts the 256 MB MALL
         nbytes = args.calibration_copy_mb << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        evict = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
+        torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
+        del evict
     return 0
 
 

@@ -67,3 +67,37 @@ def test_create_without_device_reports_error():
     assert b"device" in lib.dem_last_error(None)
     assert lib.dem_destroy(None) == -1
     assert lib.dem_sim_step(None, 1) == -1
+
+
+def test_reference_shim_signatures(tmp_path):
+    """The C++ shim exposes the reference's wrapper signatures
+    (src/functions.cuh:50-72) and links against libdemso.so."""
+    src = tmp_path / "shim.cpp"
+    src.write_text(r'''
+#include "demso_reference_shim.hpp"
+struct V2f { float x, y; };
+struct V2u { unsigned x, y; };
+struct Sys { unsigned numParticles; V2f cubeDimension; V2u gridSize; unsigned numCells; float timeStep;
+             V2f gravity; float boundaryNormalStiffness, boundaryShearStiffness; };
+struct Part { float radius, mass, inertia, normalStiffness, shearStiffness, normalDamping, boundaryDamping,
+              frictionCoefficient, attractCoefficient, colorR, colorG, colorB; };
+void (*f1)(float*, unsigned*, unsigned*, unsigned) = calcHash;
+void (*f2)(unsigned*, unsigned*, unsigned) = sortParticles;
+void (*f3)(unsigned*, unsigned*, float*, float*, float*, float*, unsigned*, unsigned*, unsigned*, unsigned*,
+           unsigned*, float*, float*, float*, float*, unsigned*, unsigned*, unsigned, unsigned) = reorderDataAndFindCellStart;
+void (*f4)(float*, float*, float*, float*, float*, unsigned*, unsigned*, unsigned*, unsigned, unsigned, float*) = collide;
+void (*f5)(float*, float*, float*, float*, float*, float*, unsigned*, float*, unsigned) = integrateSystem;
+int main() {
+  Sys s{}; Part p[1]{};
+  s.numParticles = 1; s.cubeDimension = {1, 1}; s.gridSize = {4, 4}; s.numCells = 16; s.timeStep = 1e-3f;
+  int rc = demso_shim::init(s, p, 1);
+  return (rc == DEM_OK || rc == DEM_ERR_NO_DEVICE) ? 0 : 1;
+}
+''')
+    exe = tmp_path / "shim"
+    lib_dir = ROOT / "demso_amd"
+    res = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+                          f"-L{lib_dir}", "-ldemso", f"-Wl,-rpath,{lib_dir}"], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert run.returncode == 0, run.stderr
