@@ -63,6 +63,21 @@ def make_scene(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def one_by_one(ctx, scene, steps: int) -> dict:
+    """The ensemble's instances run one after another, each as its own
+    single-system simulation with the plain step (graph replay)."""
+    from demso_amd import api, tuning
+
+    total = 0.0
+    for k in range(len(scene.instances)):
+        sim = api.Simulation(ctx, tuning.instance_scene(scene, k))
+        sim.step(1)
+        total += tuning.time_graph_steps(sim, steps) * steps
+        sim.close()
+    return {"value": scene.n * steps / total, "unit": "particle-steps/s", "seconds": total,
+            "instances": len(scene.instances)}
+
+
 def cpu_baseline(scene, steps: int, threads: int) -> dict:
     """The oracle (FP32 port of the reference kernels) on this host."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -105,11 +120,19 @@ def main() -> int:
 
     scene, workload = make_scene(args.workload)
     ctx = api.Context(0)
+    cell_tuning = None
+    if args.workload == "ensemble" and not args.no_autotune:
+        # the ensemble workload tunes the cell-size multiplier as well
+        from demso_amd import tuning
+
+        cell_tuning = tuning.autotune(ctx, scene, multipliers=(1.0, 1.25, 1.5, 2.0), steps=10)
+        scene = tuning.with_cell_multiplier(scene, cell_tuning["multiplier"])
     sim = api.Simulation(ctx, scene)
     n, cells = scene.n, scene.total_cells
 
     sim.step(1)
     tuned = None if args.no_autotune else sim.autotune()
+    passes = sim.sort_passes()
     sim.step_graph(max(2, args.warmup))
     torch.cuda.synchronize()
     ctx.synchronize()
@@ -142,6 +165,13 @@ def main() -> int:
     if args.cpu_steps > 0:
         cpu = cpu_baseline(scene, args.cpu_steps, args.cpu_threads)
 
+    ensemble = None
+    if args.workload == "ensemble":
+        sim.close()
+        single = one_by_one(ctx, scene, max(10, args.steps // 4))
+        ensemble = {"one_by_one": single, "speedup_vs_one_by_one": value / single["value"],
+                    "cell_tuning": cell_tuning}
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -155,7 +185,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": workload, "particles": n, "cells": cells, "sort_passes": sim.sort_passes(),
+        "config": {"workload": workload, "particles": n, "cells": cells, "sort_passes": passes,
                    "block_sizes": tuned},
         "roofline": {
             "bound": "hbm",
@@ -175,6 +205,8 @@ def main() -> int:
         "stage_ms": stages,
         "cpu_baseline": cpu,
     }
+    if ensemble is not None:
+        out["ensemble"] = ensemble
     print(json.dumps(out), flush=True)
     sim.close()
     ctx.close()

@@ -256,7 +256,8 @@ def ensemble(seed: int = 2, instances: int = 256, nmin: int = 2000, nmax: int = 
              steps: int = 2000, counts: Optional[Sequence[int]] = None) -> Scene:
     """Batched ensemble: independent column-collapse instances, each with its
     own particle count, box (config-0 box scaled by the lattice side / 64),
-    grid and material (kn log-uniform, kt = 0.2 kn, mu and restitution
+    grid (cells of 0.011 m covering the box and the lattice above its open
+    top) and material (kn log-uniform, kt = 0.2 kn, mu and restitution
     uniform).  Instance k's particles are the contiguous index range
     segments[k]..segments[k+1]-1 and its type is k."""
     if counts is None:
@@ -272,7 +273,9 @@ def ensemble(seed: int = 2, instances: int = 256, nmin: int = 2000, nmax: int = 
         rows = int(math.ceil(nk / side))
         scale = side / 64.0
         box = (1.0 * scale, 0.5 * scale)
-        grid, extent = _grid_for(box, cell)
+        # the open-top lattice stands above the box: the hashing grid covers it
+        # (cells, not walls) so the upper rows do not crowd the top cell row
+        grid, extent = _grid_for((box[0], max(box[1], rows * pitch + jitter)), cell)
         kn = math.exp(math.log(kn_range[0]) + uk[k] * (math.log(kn_range[1]) - math.log(kn_range[0])))
         mu = mu_range[0] + umu[k] * (mu_range[1] - mu_range[0])
         e = e_range[0] + ue[k] * (e_range[1] - e_range[0])
@@ -284,7 +287,9 @@ def ensemble(seed: int = 2, instances: int = 256, nmin: int = 2000, nmax: int = 
         type_parts.append(np.full(nk, k, np.uint32))
     st = _state(np.concatenate(pos_parts), np.concatenate(rad_parts), np.concatenate(type_parts))
     seg = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-    return Scene(name="ensemble", materials=mats, instances=insts, dt=dt, bkn=0.0, bkt=0.0,
+    # one global wall stiffness for all instances (SystemProperties holds one):
+    # stiff enough that the series combination with each instance's kt is ~kt
+    return Scene(name="ensemble", materials=mats, instances=insts, dt=dt, bkn=1e6, bkt=1e6,
                  flags=DEM_GRID_CLAMP | DEM_OPEN_TOP, steps=steps, segments=seg, **st)
 
 
