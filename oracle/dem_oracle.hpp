@@ -201,9 +201,10 @@ void reorder_and_find_cell_start(uint32_t* cell_start, uint32_t* cell_end, const
   }
 }
 
-// seriesAssociation (src/particles_kernel.cuh:36-41).
-inline float series(float a, float b) {
-  if (a == 0.0f || b == 0.0f) return 0.0f;
+// seriesAssociation (src/particles_kernel.cuh:36-41), in the run's precision
+// (float for the FP32 run, as the reference; double for the FP64 run).
+template <class T> inline T series(T a, T b) {
+  if (a == T(0) || b == T(0)) return T(0);
   return (a * b) / (a + b);
 }
 
@@ -247,10 +248,10 @@ V2<T> collide_spheres(const System& s, V2<T> posA, V2<T> posB, V2<T> velA, V2<T>
     V2<T> relVel = velB - velA;
     T relVel_n = dot(relVel, norm);
     V2<T> relVel_t = relVel - relVel_n * norm;
-    T kn = T(series(A.kn, B.kn));
-    T kt = T(series(A.kt, B.kt));
-    T cn = T(series(A.cn, B.cn));
-    T mu = T((A.mu + B.mu) / 2);
+    T kn = series<T>(A.kn, B.kn);
+    T kt = series<T>(A.kt, B.kt);
+    T cn = series<T>(A.cn, B.cn);
+    T mu = (T(A.mu) + T(B.mu)) / T(2);
     force = (-kn * (collideDist - dist)) * norm;
     if (relVel_n < T(0)) force += (cn * relVel_n) * norm;
     add_pressure(pressure, length(force), rA);
@@ -351,7 +352,7 @@ V2<T> collide_boundary(const System& s, V2<T> pos, V2<T> vel, T omega, uint32_t 
     V2<T> relVel = mk(-vel.x, -vel.y);
     T relVel_n = dot(relVel, norm);
     V2<T> relVel_t = relVel - relVel_n * norm;
-    T kt = T(series(M.kt, s.bkt));
+    T kt = series<T>(M.kt, s.bkt);
     T cn = T(M.cn);
     force = (-b.m) * mk(T(s.g[0]), T(s.g[1]));
     if (relVel_n < T(0)) force += (cn * relVel_n) * norm;

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY (see dem_oracle.hpp).  Loaded by the tests, the
 // smoke check and the benchmark's CPU-baseline leg; never by the product.
-#include "dem_oracle.hpp"
+#include "adjoint_oracle.hpp"
 
 #include <chrono>
 #include <new>
@@ -222,6 +222,63 @@ int orc_sim_get_state_f64(void* p, double* pos, double* vel, double* theta, doub
                           uint32_t* cell_start, uint32_t* cell_end) {
   return sim_get_state(static_cast<OwnedSim<double>*>(p), pos, vel, theta, omega, id, type, loc,
                        radius, acc, alpha, pressure, hash, index, cell_start, cell_end);
+}
+
+// ---- FP64 adjoint -------------------------------------------------------------
+
+void* orc_adj_create(const System* s, uint32_t nt, uint32_t ni, uint32_t n, uint32_t cells, int rad) {
+  auto* a = new (std::nothrow) orc::AdjointOracle();
+  if (!a) return nullptr;
+  a->init(*s, nt, ni, n, cells, rad != 0);
+  return a;
+}
+void orc_adj_destroy(void* p) { delete static_cast<orc::AdjointOracle*>(p); }
+
+int orc_adj_set_state(void* p, const float* pos, const float* vel, const float* theta, const float* omega,
+                      const uint32_t* id, const uint32_t* type, const uint32_t* loc, const float* radius) {
+  auto* a = static_cast<orc::AdjointOracle*>(p);
+  auto& m = a->sim;
+  const int c = m.cur;
+  for (uint32_t i = 0; i < m.n; ++i) {
+    m.pos[c][i] = V2<double>{pos[2 * i], pos[2 * i + 1]};
+    m.vel[c][i] = V2<double>{vel[2 * i], vel[2 * i + 1]};
+    m.theta[c][i] = theta[i];
+    m.omega[c][i] = omega[i];
+    m.id[c][i] = id[i];
+    m.type[c][i] = type[i];
+    m.loc[c][i] = loc[i];
+    if (m.has_radius) m.radius[c][i] = radius[i];
+  }
+  return 0;
+}
+
+int orc_adj_forward(void* p, uint32_t steps) {
+  static_cast<orc::AdjointOracle*>(p)->forward(steps);
+  return 0;
+}
+
+// Final positions/velocities by particle id.
+int orc_adj_final_state(void* p, double* pos_by_id, double* vel_by_id) {
+  auto* a = static_cast<orc::AdjointOracle*>(p);
+  auto& m = a->sim;
+  const int c = m.cur;
+  for (uint32_t i = 0; i < m.n; ++i) {
+    const uint32_t d = m.id[c][i];
+    if (d >= m.n) return -1;
+    pos_by_id[2 * d] = m.pos[c][i].x;
+    pos_by_id[2 * d + 1] = m.pos[c][i].y;
+    if (vel_by_id) {
+      vel_by_id[2 * d] = m.vel[c][i].x;
+      vel_by_id[2 * d + 1] = m.vel[c][i].y;
+    }
+  }
+  return 0;
+}
+
+int orc_adj_backward(void* p, const double* gpos, const double* gvel, double* gpos0, double* gvel0,
+                     double* gparams) {
+  static_cast<orc::AdjointOracle*>(p)->backward(gpos, gvel, gpos0, gvel0, gparams);
+  return 0;
 }
 
 }  // extern "C"

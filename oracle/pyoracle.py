@@ -72,6 +72,12 @@ def load():
         "orc_sim_step_f64": (c_double, [c_void_p, c_uint32]),
         "orc_sim_get_state_f32": (c_int, [c_void_p] + [P] * 15),
         "orc_sim_get_state_f64": (c_int, [c_void_p] + [P] * 15),
+        "orc_adj_create": (c_void_p, [POINTER(OrcSystem), c_uint32, c_uint32, c_uint32, c_uint32, c_int]),
+        "orc_adj_destroy": (None, [c_void_p]),
+        "orc_adj_set_state": (c_int, [c_void_p] + [P] * 8),
+        "orc_adj_forward": (c_int, [c_void_p, c_uint32]),
+        "orc_adj_final_state": (c_int, [c_void_p, P, P]),
+        "orc_adj_backward": (c_int, [c_void_p] + [P] * 5),
     }
     for name, (res, args) in proto.items():
         fn = getattr(lib, name)
@@ -236,3 +242,46 @@ class Sim:
             self.h, g("pos"), g("vel"), g("theta"), g("omega"), g("id"), g("type"), g("loc"), g("radius"),
             g("acc"), g("alpha"), g("pressure"), g("hash"), g("index"), g("cellStart"), g("cellEnd"))
         return out
+
+
+class AdjointOracle:
+    """FP64 forward + hand-written adjoint on the CPU (the reference for the
+    HIP adjoint).  Gradients are by particle id; params per material in the
+    order (kn, kt, cn, mu)."""
+
+    def __init__(self, scene):
+        self.lib = load()
+        self.system = System(scene)
+        self.n = scene.n
+        self.nt = len(scene.materials)
+        self.h = self.lib.orc_adj_create(ctypes.byref(self.system.sys), self.nt, len(scene.instances), self.n,
+                                         scene.total_cells, int(scene.radius is not None))
+        self.lib.orc_adj_set_state(self.h, _p(scene.pos), _p(scene.vel), _p(scene.theta), _p(scene.omega),
+                                   _p(scene.id), _p(scene.type), _p(scene.loc), _p(scene.radius))
+
+    def close(self):
+        if self.h:
+            self.lib.orc_adj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, steps: int):
+        self.lib.orc_adj_forward(self.h, steps)
+        pos = np.empty((self.n, 2), np.float64)
+        vel = np.empty((self.n, 2), np.float64)
+        self.lib.orc_adj_final_state(self.h, _p(pos), _p(vel))
+        return pos, vel
+
+    def backward(self, gpos: np.ndarray, gvel: Optional[np.ndarray] = None):
+        gpos = np.ascontiguousarray(gpos, np.float64)
+        gv = None if gvel is None else np.ascontiguousarray(gvel, np.float64)
+        gp0 = np.empty((self.n, 2), np.float64)
+        gv0 = np.empty((self.n, 2), np.float64)
+        gpar = np.empty((self.nt, 4), np.float64)
+        self.lib.orc_adj_backward(self.h, _p(gpos), _p(gv), _p(gp0), _p(gv0), _p(gpar))
+        return gp0, gv0, gpar
