@@ -112,6 +112,13 @@ PROTOTYPES = {
     "dem_sim_stage_times": (c_int, [c_void_p, c_uint32, POINTER(c_float)]),
     "dem_sim_autotune": (c_int, [c_void_p, POINTER(c_uint32)]),
     "dem_sim_sort_passes": (c_int, [c_void_p]),
+    "dem_sim_get_by_id": (c_int, [c_void_p, P, P]),
+    "dem_sim_set_by_id": (c_int, [c_void_p, P, P]),
+    "dem_set_contact_params": (c_int, [c_void_p, POINTER(ctypes.c_double), c_uint32]),
+    "dem_adj_create": (c_int, [c_void_p, c_uint32, POINTER(c_void_p)]),
+    "dem_adj_destroy": (c_int, [c_void_p]),
+    "dem_adj_forward": (c_int, [c_void_p, c_uint32]),
+    "dem_adj_backward": (c_int, [c_void_p, P, P, P, P, P]),
 }
 
 _lib: ctypes.CDLL | None = None

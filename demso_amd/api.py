@@ -142,6 +142,12 @@ class Context:
     def set_gravity(self, gx: float, gy: float) -> None:
         self._check(self.lib.dem_set_gravity(self.handle, gx, gy))
 
+    def set_contact_params(self, params) -> None:
+        """params: (num_types, 4) array of (kn, kt, cn, mu) per material."""
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1, 4))
+        self._check(self.lib.dem_set_contact_params(
+            self.handle, p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), p.shape[0]))
+
     def reserve(self, n: int) -> None:
         self._check(self.lib.dem_reserve(self.handle, n))
 
@@ -333,5 +339,57 @@ class Simulation:
     def sort_passes(self) -> int:
         return int(self.lib.dem_sim_sort_passes(self.handle))
 
+    def get_by_id(self):
+        """Current positions and velocities as (n, 2) GPU tensors indexed by particle id."""
+        import torch
+
+        pos = torch.empty((self.n, 2), dtype=torch.float32, device="cuda")
+        vel = torch.empty((self.n, 2), dtype=torch.float32, device="cuda")
+        self.ctx._check(self.lib.dem_sim_get_by_id(self.handle, _ptr(pos), _ptr(vel)))
+        return pos, vel
+
+    def set_by_id(self, pos=None, vel=None) -> None:
+        """Overwrite positions / velocities from (n, 2) GPU tensors indexed by id."""
+        self.ctx._check(self.lib.dem_sim_set_by_id(self.handle, _ptr(pos), _ptr(vel)))
+
     def synchronize(self) -> None:
         self.ctx.synchronize()
+
+
+class Adjoint:
+    """dem_adjoint: reverse mode of collide+integrate over a run of steps,
+    with checkpoints every `checkpoint_every` steps."""
+
+    def __init__(self, sim: Simulation, checkpoint_every: int = 100):
+        self.sim = sim
+        self.lib = sim.lib
+        h = c_void_p()
+        sim.ctx._check(self.lib.dem_adj_create(sim.handle, checkpoint_every, byref(h)))
+        self.handle = h.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.dem_adj_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, steps: int) -> None:
+        self.sim.ctx._check(self.lib.dem_adj_forward(self.handle, steps))
+
+    def backward(self, grad_pos=None, grad_vel=None, num_types: int = 1):
+        """grad_pos/grad_vel: (n, 2) float32 GPU tensors by particle id.
+        Returns (dL/dpos0, dL/dvel0, dL/dparams (num_types, 4) float64)."""
+        import torch
+
+        n = self.sim.n
+        gp0 = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+        gv0 = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+        gpar = torch.empty((num_types, 4), dtype=torch.float64, device="cuda")
+        self.sim.ctx._check(self.lib.dem_adj_backward(self.handle, _ptr(grad_pos), _ptr(grad_vel), _ptr(gp0),
+                                                      _ptr(gv0), _ptr(gpar)))
+        return gp0, gv0, gpar

@@ -1,0 +1,706 @@
+// adjoint.hip -- reverse mode of the DEM step (collide + integrate) on gfx950.
+//
+// The forward keeps a checkpoint of the state every K steps.  The backward
+// walks the checkpoints in reverse; for each segment it re-runs the K
+// forward steps, storing the sorted pre-collide state and the permutation of
+// each step (the neighbour lists are not stored: cells are rebuilt from the
+// stored sorted positions), then per step in reverse:
+//   cells   -> forces (the forward collide kernel, bit-identical acc/alpha)
+//   k_adj_integrate : reverse of integrateSystemD (src/particles_kernel.cuh:
+//                     580-627) incl. the walls and the bottom-wall law
+//                     (:388-456), replaying the FP32 forward for its
+//                     switch decisions
+//   k_adj_collide   : reverse of the pair law (:235-307) over the 3x3
+//                     neighbourhood, one lane per particle gathering the
+//                     neighbours' force/moment adjoints (no atomics on state)
+//   k_adj_permute   : adjoint of the sort permutation (scatter)
+// Switches (contact, approach-only damping, stick/slip, wall hits) are
+// frozen at the forward's decisions.  In exact arithmetic the pair law is
+// F = s n + f_t t, tau = f_t with s = -kn (R - dist) + [vn<0] cn vn,
+// c_t = (vB - vA).t - (rA wA + rB wB), f_t = kt dt c_t (stick) or
+// mu |s| sgn(c_t) (slip); force on B = -F, moments rA f_t and rB f_t.  The
+// adjoint state and the vector-Jacobian products are FP64; parameter
+// gradients (kn, kt, cn, mu per material, through the series combination)
+// are accumulated in FP64.  Attraction is not differentiated (rejected).
+#include "capi_internal.h"
+
+#include <math.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace dem {
+namespace {
+
+constexpr int kParams = 4;  // kn, kt, cn, mu
+
+__device__ __forceinline__ uint32_t clampi(uint32_t t, uint32_t count) { return t < count ? t : count - 1u; }
+
+__device__ __forceinline__ float series_f(float a, float b) {
+  if (a == 0.0f || b == 0.0f) return 0.0f;
+  return (a * b) / (a + b);
+}
+
+__device__ __forceinline__ void series_grad(double a, double b, double& da, double& db) {
+  if (a == 0.0 || b == 0.0) {
+    da = db = 0.0;
+    return;
+  }
+  const double s = (a + b) * (a + b);
+  da = b * b / s;
+  db = a * a / s;
+}
+
+__device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+// Same conversion as the forward kernels: NaN -> 0, saturation, truncation.
+__device__ __forceinline__ int to_int_sat(float g) {
+  int r = (int)fminf(fmaxf(g, -2147483648.0f), 2147483520.0f);
+  r = (g >= 2147483648.0f) ? 2147483647 : r;
+  return (g == g) ? r : 0;
+}
+
+template <bool kRadius>
+__device__ __forceinline__ void body_of(const Material& m, const float* __restrict__ radius, uint32_t i, float& r,
+                                        float& mass, float& inertia) {
+  if (kRadius) {
+    r = radius[i];
+    mass = (m.areal * r) * r;
+    inertia = (((2.0f * mass) * r) * r) / 5.0f;
+  } else {
+    r = m.radius;
+    mass = m.mass;
+    inertia = m.inertia;
+  }
+}
+
+// Adds a lane's gradient contributions for material t: one atomic per
+// parameter per wave when the whole wave shares the material, else per lane.
+__device__ __forceinline__ void flush_grads(double g[kParams], uint32_t t, bool active, double* __restrict__ grad) {
+  const uint32_t t0 = __shfl(t, 0);
+  const bool uniform = __all(!active || t == t0);
+  if (uniform) {
+#pragma unroll
+    for (int p = 0; p < kParams; ++p) {
+      double v = active ? g[p] : 0.0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      g[p] = v;
+    }
+    if (__lane_id() == 0) {
+#pragma unroll
+      for (int p = 0; p < kParams; ++p)
+        if (g[p] != 0.0) atomicAdd(&grad[kParams * t0 + p], g[p]);
+    }
+  } else if (active) {
+#pragma unroll
+    for (int p = 0; p < kParams; ++p)
+      if (g[p] != 0.0) atomicAdd(&grad[kParams * t + p], g[p]);
+  }
+}
+
+struct AdjIn {
+  const float2* pos;
+  const float2* vel;
+  const float* omega;
+  const float* radius;
+  const uint32_t* type;
+};
+
+// ---------------------------------------------------------------------------
+__global__ void k_adj_init(const uint32_t* __restrict__ id, const float* __restrict__ gpos,
+                           const float* __restrict__ gvel, double2* __restrict__ lx, double2* __restrict__ lv,
+                           double* __restrict__ lw, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = min(id[i], n - 1u);
+  lx[i] = gpos ? make_double2(gpos[2 * d], gpos[2 * d + 1]) : make_double2(0.0, 0.0);
+  lv[i] = gvel ? make_double2(gvel[2 * d], gvel[2 * d + 1]) : make_double2(0.0, 0.0);
+  lw[i] = 0.0;
+}
+
+__global__ void k_adj_out(const uint32_t* __restrict__ id, const double2* __restrict__ lx,
+                          const double2* __restrict__ lv, float* __restrict__ gpos0, float* __restrict__ gvel0,
+                          uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = id[i];
+  if (d >= n) return;
+  if (gpos0) {
+    gpos0[2 * d] = (float)lx[i].x;
+    gpos0[2 * d + 1] = (float)lx[i].y;
+  }
+  if (gvel0) {
+    gvel0[2 * d] = (float)lv[i].x;
+    gvel0[2 * d + 1] = (float)lv[i].y;
+  }
+}
+
+// sorted-state snapshot of one step: pos, vel, omega, radius, type, index
+__global__ void k_store_slot(const float2* __restrict__ pos, const float2* __restrict__ vel,
+                             const float* __restrict__ omega, const float* __restrict__ radius,
+                             const uint32_t* __restrict__ type, const uint32_t* __restrict__ index,
+                             float2* __restrict__ spos, float2* __restrict__ svel, float* __restrict__ somega,
+                             float* __restrict__ sradius, uint32_t* __restrict__ stype, uint32_t* __restrict__ sindex,
+                             uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  spos[i] = pos[i];
+  svel[i] = vel[i];
+  somega[i] = omega[i];
+  if (radius) sradius[i] = radius[i];
+  stype[i] = type[i];
+  sindex[i] = index[i];
+}
+
+// cellStart/cellEnd of sorted keys (cellStart pre-filled with EMPTY)
+__global__ void k_cell_bounds(const uint32_t* __restrict__ key, uint32_t* __restrict__ cs, uint32_t* __restrict__ ce,
+                              uint32_t n, uint32_t num_cells) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t h = key[i];
+  if (i == 0 || h != key[i - 1]) {
+    if (h < num_cells) cs[h] = i;
+    if (i > 0 && key[i - 1] < num_cells) ce[key[i - 1]] = i;
+  }
+  if (i == n - 1 && h < num_cells) ce[h] = n;
+}
+
+__global__ void k_adj_permute(const uint32_t* __restrict__ index, const double2* __restrict__ lxp,
+                              const double2* __restrict__ lvp, const double* __restrict__ lwp,
+                              double2* __restrict__ lx, double2* __restrict__ lv, double* __restrict__ lw,
+                              uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t src = index[i];
+  if (src >= n) return;
+  lx[src] = lxp[i];
+  lv[src] = lvp[i];
+  lw[src] = lwp[i];
+}
+
+// ---------------------------------------------------------------------------
+// Reverse of integrateSystemD for one particle.
+template <bool kRadius, bool kMulti>
+__global__ __launch_bounds__(256) void k_adj_integrate(StepConst c, AdjIn S, const float2* __restrict__ acc,
+                                                       const float* __restrict__ alpha,
+                                                       const double2* __restrict__ lx, const double2* __restrict__ lv,
+                                                       const double* __restrict__ lw, double2* __restrict__ lxp,
+                                                       double2* __restrict__ lvp, double* __restrict__ lwp,
+                                                       double2* __restrict__ lF, double* __restrict__ lM,
+                                                       double* __restrict__ grad, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < n;
+  const uint32_t ii = active ? i : 0u;
+  const uint32_t t = clampi(S.type[ii], c.num_types);
+  double g[kParams] = {0.0, 0.0, 0.0, 0.0};
+  if (active) {
+    const Material m = c.mat[t];
+    const Instance in = kMulti ? c.inst[clampi(m.inst, c.num_instances)] : c.inst0;
+    float r, mass, inertia;
+    body_of<kRadius>(m, S.radius, i, r, mass, inertia);
+    const float dt = c.dt;
+    // FP32 forward replay (same operations as k_integrate) for the switches
+    const float2 a = acc[i];
+    float2 v1 = S.vel[i];
+    v1.x = v1.x + (c.gx + a.x) * dt;
+    v1.y = v1.y + (c.gy + a.y) * dt;
+    float2 p1 = S.pos[i];
+    p1.x = p1.x + v1.x * dt;
+    p1.y = p1.y + v1.y * dt;
+    float w1 = S.omega[i] + alpha[i] * dt;
+    const float bd = -m.bdamp;
+    bool fr = false, fl = false, ft = false, fb = false;
+    if (p1.x > in.box[0] - r) { p1.x = in.box[0] - r; v1.x = v1.x * bd; fr = true; }
+    if (p1.x < r) { p1.x = r; v1.x = v1.x * bd; fl = true; }
+    if (!(c.flags & DEM_OPEN_TOP) && p1.y > in.box[1] - r) { p1.y = in.box[1] - r; v1.y = v1.y * bd; ft = true; }
+    bool contact = false, damp = false, fric = false, stick = false;
+    float sy = 0.0f, cx = 0.0f, P = 0.0f, ktw = 0.0f, f0x = 0.0f, f0y = 0.0f;
+    if (p1.y < r) {
+      fb = true;
+      const float rx = p1.x - p1.x, ry = 0.0f - p1.y;
+      const float dist = sqrtf(rx * rx + ry * ry);
+      if (dist < r) {
+        contact = true;
+        const float nx = rx / dist, ny = ry / dist;
+        const float tx = -ny, ty = nx;
+        sy = tx;  // t = (sy, 0)
+        const float rvx = -v1.x, rvy = -v1.y;
+        const float vn = rvx * nx + rvy * ny;
+        const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
+        ktw = series_f(m.kt, c.bkt);
+        const float s = -mass;
+        f0x = s * c.gx;
+        f0y = s * c.gy;
+        if (vn < 0.0f) {
+          damp = true;
+          const float s2 = m.cn * vn;
+          f0x += s2 * nx;
+          f0y += s2 * ny;
+        }
+        P = sqrtf(f0x * f0x + f0y * f0y);
+        if (m.mu != 0.0f) {
+          fric = true;
+          const float sr = r * w1;
+          const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
+          const float ftx = (ktw * cvx) * dt, fty = (ktw * cvy) * dt;
+          stick = sqrtf(ftx * ftx + fty * fty) < m.mu * P;
+          cx = cvx;
+        }
+      }
+    }
+    // reverse, in double
+    const double ddt = dt, dbd = bd, dm = mass, dI = inertia, dr = r;
+    double2 lp1 = lx[i], lv1 = lv[i];
+    double lw1 = lw[i];
+    if (fb) {
+      lv1.y *= dbd;
+      lp1.y = 0.0;
+      const double lFwx = (lv1.x * ddt + lp1.x * (ddt * ddt)) / dm;
+      const double lFwy = (lv1.y * ddt + lp1.y * (ddt * ddt)) / dm;
+      const double law = lw1 * ddt;
+      if (contact) {
+        if (fric) {
+          const double lFt = lFwx + law * dr * (double)sy / dI;
+          if (stick) {
+            const double lc = lFt * (double)ktw * ddt;
+            double da, db;
+            series_grad(m.kt, c.bkt, da, db);
+            g[1] += lFt * ddt * (double)cx * da;
+            lv1.x += -lc;
+            lw1 += -(double)sy * dr * lc;
+          } else {
+            const double lP = lFt * (double)m.mu * sgn(cx);
+            g[3] += lFt * (double)P * sgn(cx);
+            if (damp && P > 0.0f) {
+              const double lf0y = lP * (double)f0y / (double)P;
+              lv1.y += lf0y * (-(double)m.cn);
+              g[2] += lf0y * (-(double)v1.y);
+            }
+          }
+        } else if (damp) {
+          lv1.y += lFwy * (-(double)m.cn);
+          g[2] += lFwy * (-(double)v1.y);
+        }
+      }
+    }
+    if (ft) { lv1.y *= dbd; lp1.y = 0.0; }
+    if (fl) { lv1.x *= dbd; lp1.x = 0.0; }
+    if (fr) { lv1.x *= dbd; lp1.x = 0.0; }
+    lv1.x += lp1.x * ddt;
+    lv1.y += lp1.y * ddt;
+    lxp[i] = lp1;
+    lvp[i] = lv1;
+    lwp[i] = lw1;
+    lF[i] = make_double2(lv1.x * (ddt / dm), lv1.y * (ddt / dm));
+    lM[i] = lw1 * ddt / dI;
+  }
+  flush_grads(g, t, active, grad);
+}
+
+// ---------------------------------------------------------------------------
+// Reverse of the pair law for particle i over its 3x3 neighbourhood.
+template <bool kRadius, bool kMulti>
+__global__ __launch_bounds__(256) void k_adj_collide(StepConst c, AdjIn S, const uint32_t* __restrict__ cell_start,
+                                                     const uint32_t* __restrict__ cell_end,
+                                                     const double2* __restrict__ lF, const double* __restrict__ lM,
+                                                     double2* __restrict__ lxp, double2* __restrict__ lvp,
+                                                     double* __restrict__ lwp, double* __restrict__ grad, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < n;
+  const uint32_t ii = active ? i : 0u;
+  const uint32_t tA = clampi(S.type[ii], c.num_types);
+  double g[kParams] = {0.0, 0.0, 0.0, 0.0};
+  if (active) {
+    const Material A = c.mat[tA];
+    const Instance in = kMulti ? c.inst[clampi(A.inst, c.num_instances)] : c.inst0;
+    float rA, massA, inertiaA;
+    body_of<kRadius>(A, S.radius, i, rA, massA, inertiaA);
+    const float2 pA = S.pos[i], vA = S.vel[i];
+    const float wA = S.omega[i];
+    const float dt = c.dt;
+    const bool clamp = c.flags & DEM_GRID_CLAMP;
+    int gx = to_int_sat(floorf(pA.x * (float)in.grid[0] / in.extent[0]));
+    int gy = to_int_sat(floorf(pA.y * (float)in.grid[1] / in.extent[1]));
+    if (clamp) {
+      gx = min(max(gx, 0), (int)in.grid[0] - 1);
+      gy = min(max(gy, 0), (int)in.grid[1] - 1);
+    }
+    const double2 lFA = lF[i];
+    const double lMA = lM[i];
+    double2 dx_acc = make_double2(0.0, 0.0), drv_acc = make_double2(0.0, 0.0);
+    double dw_acc = 0.0;
+    for (int y = -1; y <= 1; ++y) {
+      for (int x = -1; x <= 1; ++x) {
+        uint32_t cxi, cyi;
+        if (clamp) {
+          const int ix = gx + x, iy = gy + y;
+          if (ix < 0 || iy < 0 || ix >= (int)in.grid[0] || iy >= (int)in.grid[1]) continue;
+          cxi = (uint32_t)ix;
+          cyi = (uint32_t)iy;
+        } else {
+          cxi = ((uint32_t)gx + (uint32_t)x) & (in.grid[0] - 1u);
+          cyi = ((uint32_t)gy + (uint32_t)y) & (in.grid[1] - 1u);
+        }
+        const uint32_t key = in.cell_base + cyi * in.grid[0] + cxi;
+        const uint32_t st = cell_start[key];
+        if (st == kEmpty) continue;
+        const uint32_t en = min(cell_end[key], n);
+        for (uint32_t j = st; j < en; ++j) {
+          if (j == i) continue;
+          const uint32_t tB = clampi(S.type[j], c.num_types);
+          const Material B = c.mat[tB];
+          float rB, mB, IB;
+          body_of<kRadius>(B, S.radius, j, rB, mB, IB);
+          const float2 pB = S.pos[j];
+          // FP32 forward replay of collideSpheres' decisions
+          const float rx = pB.x - pA.x, ry = pB.y - pA.y;
+          const float dist = sqrtf(rx * rx + ry * ry);
+          const float R = rA + rB;
+          if (!(dist < R)) continue;
+          const float2 vB = S.vel[j];
+          const float wB = S.omega[j];
+          const float nx = rx / dist, ny = ry / dist;
+          const float tx = -ny, ty = nx;
+          const float rvx = vB.x - vA.x, rvy = vB.y - vA.y;
+          const float vn = rvx * nx + rvy * ny;
+          const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
+          const float kn = series_f(A.kn, B.kn), kt = series_f(A.kt, B.kt), cn = series_f(A.cn, B.cn);
+          const float mu = (A.mu + B.mu) / 2;
+          const bool damp = vn < 0.0f;
+          bool stick = false;
+          const bool fric = mu != 0.0f;
+          if (fric) {
+            const float s = -kn * (R - dist);
+            float ffx = s * nx, ffy = s * ny;
+            if (damp) {
+              const float s2 = cn * vn;
+              ffx += s2 * nx;
+              ffy += s2 * ny;
+            }
+            const float flen = sqrtf(ffx * ffx + ffy * ffy);
+            const float sr = rA * wA + rB * wB;
+            const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
+            const float ftx = (kt * cvx) * dt, fty = (kt * cvy) * dt;
+            stick = sqrtf(ftx * ftx + fty * fty) < mu * flen;
+          }
+          // closed-form derivatives in double
+          const double dn_x = nx, dn_y = ny, dt_x = tx, dt_y = ty, ddist = dist;
+          const double dvn = (double)rvx * dn_x + (double)rvy * dn_y;
+          const double dvt = (double)rvx * dt_x + (double)rvy * dt_y;
+          const double dkn = kn, dkt = kt, dcn = cn, dmu = mu, ddtm = dt;
+          const double delta = (double)R - ddist;
+          const double s = -dkn * delta + (damp ? dcn * dvn : 0.0);
+          const double P = fabs(s);
+          double ct = 0.0, ft = 0.0;
+          if (fric) {
+            ct = dvt - ((double)rA * wA + (double)rB * wB);
+            ft = stick ? dkt * ddtm * ct : dmu * P * sgn(ct);
+          }
+          const double2 lFB = lF[j];
+          const double gFx = lFA.x - lFB.x, gFy = lFA.y - lFB.y;
+          const double gtau = (double)rA * lMA + (double)rB * lM[j];
+          const double a = gFx * dn_x + gFy * dn_y;
+          const double gFt = gFx * dt_x + gFy * dt_y;
+          const double bb = gFt + gtau;
+          const double cdamp = damp ? dcn : 0.0;
+          const double dsx = dn_x * dkn + dt_x * (cdamp * dvt / ddist);
+          const double dsy = dn_y * dkn + dt_y * (cdamp * dvt / ddist);
+          const double dsrx = dn_x * cdamp, dsry = dn_y * cdamp;
+          double dftx = 0.0, dfty = 0.0, dftrx = 0.0, dftry = 0.0, dftw = 0.0, slip_k = 0.0;
+          if (fric) {
+            if (stick) {
+              dftx = dt_x * (-dkt * ddtm * dvn / ddist);
+              dfty = dt_y * (-dkt * ddtm * dvn / ddist);
+              dftrx = dt_x * (dkt * ddtm);
+              dftry = dt_y * (dkt * ddtm);
+              dftw = -dkt * ddtm;
+            } else {
+              slip_k = dmu * sgn(s) * sgn(ct);
+              dftx = dsx * slip_k;
+              dfty = dsy * slip_k;
+              dftrx = dsrx * slip_k;
+              dftry = dsry * slip_k;
+            }
+          }
+          dx_acc.x += dsx * a + dt_x * (s * gFt / ddist) + dftx * bb - dt_x * (ft * a / ddist);
+          dx_acc.y += dsy * a + dt_y * (s * gFt / ddist) + dfty * bb - dt_y * (ft * a / ddist);
+          drv_acc.x += dsrx * a + dftrx * bb;
+          drv_acc.y += dsry * a + dftry * bb;
+          dw_acc += dftw * bb;
+          if (i < j) {
+            const double gkn = a * (-delta) + bb * slip_k * (-delta);
+            const double gcn = damp ? (a * dvn + bb * slip_k * dvn) : 0.0;
+            const double gkt = (fric && stick) ? bb * ddtm * ct : 0.0;
+            const double gmu = (fric && !stick) ? bb * P * sgn(ct) : 0.0;
+            double ka, kb, ta, tb, ca, cb;
+            series_grad(A.kn, B.kn, ka, kb);
+            series_grad(A.kt, B.kt, ta, tb);
+            series_grad(A.cn, B.cn, ca, cb);
+            g[0] += gkn * ka;
+            g[1] += gkt * ta;
+            g[2] += gcn * ca;
+            g[3] += gmu * 0.5;
+            if (tB == tA) {
+              g[0] += gkn * kb;
+              g[1] += gkt * tb;
+              g[2] += gcn * cb;
+              g[3] += gmu * 0.5;
+            } else {
+              if (gkn * kb != 0.0) atomicAdd(&grad[kParams * tB + 0], gkn * kb);
+              if (gkt * tb != 0.0) atomicAdd(&grad[kParams * tB + 1], gkt * tb);
+              if (gcn * cb != 0.0) atomicAdd(&grad[kParams * tB + 2], gcn * cb);
+              if (gmu != 0.0) atomicAdd(&grad[kParams * tB + 3], gmu * 0.5);
+            }
+          }
+        }
+      }
+    }
+    double2 lxi = lxp[i], lvi = lvp[i];
+    lxi.x -= dx_acc.x;
+    lxi.y -= dx_acc.y;
+    lvi.x -= drv_acc.x;
+    lvi.y -= drv_acc.y;
+    lxp[i] = lxi;
+    lvp[i] = lvi;
+    lwp[i] = lwp[i] + (double)rA * dw_acc;
+  }
+  flush_grads(g, tA, active, grad);
+}
+
+inline dim3 grid_of(uint32_t n) { return dim3((n + 255u) / 256u); }
+
+}  // namespace
+}  // namespace dem
+
+using namespace dem;
+
+// One device copy of the current state buffer of a sim.
+struct AdjCheckpoint {
+  float2 *pos = nullptr, *vel = nullptr;
+  float *theta = nullptr, *omega = nullptr, *radius = nullptr;
+  uint32_t *id = nullptr, *type = nullptr, *loc = nullptr;
+  int cur = 0;
+};
+
+struct dem_adjoint {
+  dem_sim* sim = nullptr;
+  uint32_t K = 100;
+  uint32_t steps = 0;
+  std::vector<AdjCheckpoint> ckpt;
+  AdjCheckpoint final_state;
+  // segment slots: K x n
+  float2 *spos = nullptr, *svel = nullptr;
+  float *somega = nullptr, *sradius = nullptr;
+  uint32_t *stype = nullptr, *sindex = nullptr;
+  // adjoint state (FP64)
+  double2 *lx = nullptr, *lv = nullptr, *lxp = nullptr, *lvp = nullptr, *lF = nullptr;
+  double *lw = nullptr, *lwp = nullptr, *lM = nullptr, *grad = nullptr;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+template <class T>
+int adj_alloc(dem_adjoint* a, T** p, size_t count) {
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1));
+  if (e != hipSuccess) return hip_fail(a->sim->ctx, e, "hipMalloc(adjoint)");
+  a->allocs.push_back((void*)*p);
+  return DEM_OK;
+}
+
+int alloc_checkpoint(dem_adjoint* a, AdjCheckpoint& c) {
+  const uint32_t n = a->sim->n;
+  int r = DEM_OK;
+  if (!r) r = adj_alloc(a, &c.pos, n);
+  if (!r) r = adj_alloc(a, &c.vel, n);
+  if (!r) r = adj_alloc(a, &c.theta, n);
+  if (!r) r = adj_alloc(a, &c.omega, n);
+  if (!r) r = adj_alloc(a, &c.id, n);
+  if (!r) r = adj_alloc(a, &c.type, n);
+  if (!r) r = adj_alloc(a, &c.loc, n);
+  if (!r && a->sim->rad) r = adj_alloc(a, &c.radius, n);
+  return r;
+}
+
+hipError_t copy_state(dem_sim* m, AdjCheckpoint& c, bool to_checkpoint, hipStream_t s) {
+  const size_t n = m->n;
+  const int b = to_checkpoint ? m->cur : c.cur;
+  struct Item { void* ck; void* st; size_t bytes; } items[] = {
+      {c.pos, m->pos[b], 8 * n},   {c.vel, m->vel[b], 8 * n},   {c.theta, m->theta[b], 4 * n},
+      {c.omega, m->omega[b], 4 * n}, {c.id, m->id[b], 4 * n},    {c.type, m->type[b], 4 * n},
+      {c.loc, m->loc[b], 4 * n},   {c.radius, m->rad ? m->radius[b] : nullptr, 4 * n}};
+  for (const Item& it : items) {
+    if (!it.ck || !it.st) continue;
+    hipError_t e = to_checkpoint ? hipMemcpyAsync(it.ck, it.st, it.bytes, hipMemcpyDeviceToDevice, s)
+                                 : hipMemcpyAsync(it.st, it.ck, it.bytes, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+  }
+  if (to_checkpoint) c.cur = m->cur;
+  else m->cur = c.cur;
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dem_adj_create(dem_sim* sim, uint32_t checkpoint_every, dem_adjoint** out) {
+  if (!sim || !out) return DEM_ERR_INVALID;
+  *out = nullptr;
+  dem_context* ctx = sim->ctx;
+  if (checkpoint_every == 0) return fail(ctx, DEM_ERR_INVALID, "checkpoint_every must be >= 1");
+  if (ctx->c.attract) return fail(ctx, DEM_ERR_INVALID, "the adjoint does not cover attraction");
+  if (int r = ensure_device(ctx)) return r;
+  auto* a = new (std::nothrow) dem_adjoint();
+  if (!a) return fail(ctx, DEM_ERR_ALLOC, "host allocation failed");
+  a->sim = sim;
+  a->K = checkpoint_every;
+  const size_t n = sim->n, slots = (size_t)checkpoint_every * n;
+  int r = DEM_OK;
+  if (!r) r = adj_alloc(a, &a->spos, slots);
+  if (!r) r = adj_alloc(a, &a->svel, slots);
+  if (!r) r = adj_alloc(a, &a->somega, slots);
+  if (!r && sim->rad) r = adj_alloc(a, &a->sradius, slots);
+  if (!r) r = adj_alloc(a, &a->stype, slots);
+  if (!r) r = adj_alloc(a, &a->sindex, slots);
+  if (!r) r = adj_alloc(a, &a->lx, n);
+  if (!r) r = adj_alloc(a, &a->lv, n);
+  if (!r) r = adj_alloc(a, &a->lw, n);
+  if (!r) r = adj_alloc(a, &a->lxp, n);
+  if (!r) r = adj_alloc(a, &a->lvp, n);
+  if (!r) r = adj_alloc(a, &a->lwp, n);
+  if (!r) r = adj_alloc(a, &a->lF, n);
+  if (!r) r = adj_alloc(a, &a->lM, n);
+  if (!r) r = adj_alloc(a, &a->grad, (size_t)kParams * DEM_MAX_TYPES);
+  if (!r) r = alloc_checkpoint(a, a->final_state);
+  if (r) {
+    dem_adj_destroy(a);
+    return r;
+  }
+  *out = a;
+  return DEM_OK;
+}
+
+int dem_adj_destroy(dem_adjoint* a) {
+  if (!a) return DEM_ERR_INVALID;
+  (void)hipSetDevice(a->sim->ctx->device);
+  (void)hipStreamSynchronize(a->sim->ctx->stream);
+  for (void* p : a->allocs) (void)hipFree(p);
+  delete a;
+  return DEM_OK;
+}
+
+int dem_adj_forward(dem_adjoint* a, uint32_t steps) {
+  if (!a) return DEM_ERR_INVALID;
+  dem_sim* m = a->sim;
+  dem_context* ctx = m->ctx;
+  if (int r = sim_check(m)) return r;
+  if (ctx->c.attract) return fail(ctx, DEM_ERR_INVALID, "the adjoint does not cover attraction");
+  const uint32_t nck = steps / a->K + 1;
+  while (a->ckpt.size() < nck) {
+    a->ckpt.emplace_back();
+    if (int r = alloc_checkpoint(a, a->ckpt.back())) {
+      a->ckpt.pop_back();
+      return r;
+    }
+  }
+  hipStream_t s = ctx->stream;
+  for (uint32_t k = 0; k < steps; ++k) {
+    if (k % a->K == 0) DEM_HIP(ctx, copy_state(m, a->ckpt[k / a->K], true, s), "checkpoint");
+    DEM_HIP(ctx, sim_step_on(m, s, nullptr), "forward step");
+  }
+  DEM_HIP(ctx, copy_state(m, a->final_state, true, s), "final state");
+  a->steps = steps;
+  return DEM_OK;
+}
+
+int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_vel, float* grad_pos0,
+                     float* grad_vel0, double* grad_params) {
+  if (!a) return DEM_ERR_INVALID;
+  dem_sim* m = a->sim;
+  dem_context* ctx = m->ctx;
+  if (int r = sim_check(m)) return r;
+  if (a->steps == 0) return fail(ctx, DEM_ERR_STATE, "dem_adj_forward has not run");
+  hipStream_t s = ctx->stream;
+  const uint32_t n = m->n;
+  const StepConst& c = ctx->c;
+  const bool mul = multi(ctx);
+  // adjoint of the final state, from gradients given by particle id
+  hipLaunchKernelGGL(k_adj_init, grid_of(n), dim3(256), 0, s, a->final_state.id, grad_pos, grad_vel, a->lx, a->lv,
+                     a->lw, n);
+  DEM_HIP(ctx, hipMemsetAsync(a->grad, 0, sizeof(double) * kParams * c.num_types, s), "grad reset");
+  const uint32_t nseg = (a->steps + a->K - 1) / a->K;
+  for (int seg = (int)nseg - 1; seg >= 0; --seg) {
+    const uint32_t k0 = (uint32_t)seg * a->K, len = std::min(a->K, a->steps - k0);
+    // re-run the segment, storing the sorted pre-collide state of each step
+    DEM_HIP(ctx, copy_state(m, a->ckpt[seg], false, s), "restore checkpoint");
+    for (uint32_t k = 0; k < len; ++k) {
+      const int o = m->cur, t = o ^ 1;
+      DEM_HIP(ctx, launch_hash(c, s, ctx->bs_hash, m->pos[o], mul ? m->type[o] : nullptr, m->key, m->idx, n), "hash");
+      bool in_alt = false;
+      DEM_HIP(ctx, launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt), "sort");
+      uint32_t* sk = in_alt ? m->sort.keys_alt : m->key;
+      uint32_t* si = in_alt ? m->sort.vals_alt : m->idx;
+      DEM_HIP(ctx, launch_fill(s, m->cell_start, kEmpty, m->cells), "fill");
+      DEM_HIP(ctx, launch_reorder(s, ctx->bs_reorder, m->cell_start, m->cell_end, m->pos[t], m->vel[t], m->theta[t],
+                                  m->omega[t], m->id[t], m->loc[t], m->type[t], m->rad ? m->radius[t] : nullptr, sk, si,
+                                  m->pos[o], m->vel[o], m->theta[o], m->omega[o], m->id[o], m->type[o],
+                                  m->rad ? m->radius[o] : nullptr, n, m->cells),
+              "reorder");
+      const size_t off = (size_t)k * n;
+      hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[t], m->vel[t], m->omega[t],
+                         m->rad ? m->radius[t] : nullptr, m->type[t], si, a->spos + off, a->svel + off,
+                         a->somega + off, a->sradius ? a->sradius + off : nullptr, a->stype + off, a->sindex + off, n);
+      DEM_HIP(ctx, launch_collide(c, s, ctx->bs_collide, m->pos[t], m->vel[t], m->acc, m->omega[t], m->alpha,
+                                  m->type[t], m->rad ? m->radius[t] : nullptr, m->cell_start, m->cell_end, m->pressure, n),
+              "collide");
+      DEM_HIP(ctx, launch_integrate(c, s, ctx->bs_integrate, m->pos[t], m->vel[t], m->acc, m->theta[t], m->omega[t],
+                                    m->alpha, m->type[t], m->rad ? m->radius[t] : nullptr, m->pressure, n),
+              "integrate");
+      m->cur = t;
+    }
+    // reverse through the segment
+    for (int k = (int)len - 1; k >= 0; --k) {
+      const size_t off = (size_t)k * n;
+      AdjIn S{a->spos + off, a->svel + off, a->somega + off, a->sradius ? a->sradius + off : nullptr, a->stype + off};
+      DEM_HIP(ctx, launch_hash(c, s, ctx->bs_hash, S.pos, mul ? S.type : nullptr, m->key, m->idx, n), "adj hash");
+      DEM_HIP(ctx, launch_fill(s, m->cell_start, kEmpty, m->cells), "adj fill");
+      hipLaunchKernelGGL(k_cell_bounds, grid_of(n), dim3(256), 0, s, m->key, m->cell_start, m->cell_end, n, m->cells);
+      DEM_HIP(ctx, launch_collide(c, s, ctx->bs_collide, S.pos, S.vel, m->acc, S.omega, m->alpha, S.type, S.radius,
+                                  m->cell_start, m->cell_end, m->pressure, n),
+              "adj collide forward");
+#define DEM_ADJ(K, ...)                                                                                 \
+  do {                                                                                                  \
+    if (m->rad) {                                                                                       \
+      if (mul) hipLaunchKernelGGL((K<true, true>), grid_of(n), dim3(256), 0, s, __VA_ARGS__);             \
+      else hipLaunchKernelGGL((K<true, false>), grid_of(n), dim3(256), 0, s, __VA_ARGS__);               \
+    } else {                                                                                            \
+      if (mul) hipLaunchKernelGGL((K<false, true>), grid_of(n), dim3(256), 0, s, __VA_ARGS__);            \
+      else hipLaunchKernelGGL((K<false, false>), grid_of(n), dim3(256), 0, s, __VA_ARGS__);              \
+    }                                                                                                   \
+  } while (0)
+      DEM_ADJ(k_adj_integrate, c, S, (const float2*)m->acc, (const float*)m->alpha, (const double2*)a->lx,
+              (const double2*)a->lv, (const double*)a->lw, a->lxp, a->lvp, a->lwp, a->lF, a->lM, a->grad, n);
+      DEM_ADJ(k_adj_collide, c, S, (const uint32_t*)m->cell_start, (const uint32_t*)m->cell_end,
+              (const double2*)a->lF, (const double*)a->lM, a->lxp, a->lvp, a->lwp, a->grad, n);
+#undef DEM_ADJ
+      hipLaunchKernelGGL(k_adj_permute, grid_of(n), dim3(256), 0, s, a->sindex + off, a->lxp, a->lvp, a->lwp, a->lx,
+                         a->lv, a->lw, n);
+      DEM_HIP(ctx, hipGetLastError(), "adjoint kernels");
+    }
+  }
+  // lx/lv now follow the order of the first checkpoint
+  hipLaunchKernelGGL(k_adj_out, grid_of(n), dim3(256), 0, s, a->ckpt[0].id, a->lx, a->lv, grad_pos0, grad_vel0, n);
+  if (grad_params)
+    DEM_HIP(ctx, hipMemcpyAsync(grad_params, a->grad, sizeof(double) * kParams * c.num_types, hipMemcpyDeviceToDevice, s),
+            "grad copy");
+  DEM_HIP(ctx, copy_state(m, a->final_state, false, s), "restore final state");
+  DEM_HIP(ctx, hipStreamSynchronize(s), "adjoint sync");
+  return DEM_OK;
+}
+
+}  // extern "C"

@@ -5,7 +5,7 @@
 // device memory instead of __constant__ symbols, scratch management, the
 // owned double-buffered simulation, hipGraph capture/replay, HIP-event stage
 // timing and block-size autotuning.  No call here exits the process.
-#include "dem_internal.h"
+#include "capi_internal.h"
 
 #include <math.h>
 #include <string.h>
@@ -20,49 +20,11 @@ using dem::Material;
 using dem::SortScratch;
 using dem::StepConst;
 
-struct dem_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  bool params_set = false;
-  StepConst c{};
-  Material* d_mat = nullptr;
-  uint32_t mat_cap = 0;
-  Instance* d_inst = nullptr;
-  uint32_t inst_cap = 0;
-  std::vector<Material> h_mat;
-  std::vector<Instance> h_inst;
-  uint32_t total_cells = 0;  // every key produced by hashing is < total_cells
-  uint32_t key_bits = 32;
-  SortScratch sort;
-  uint32_t* d_copy_tmp = nullptr;
-  int bs_hash = 256, bs_reorder = 256, bs_collide = 128, bs_integrate = 256;
-  uint64_t generation = 1;  // bumped by every parameter change
-};
-
-struct dem_sim {
-  dem_context* ctx = nullptr;
-  uint32_t n = 0, cells = 0;
-  bool rad = false;
-  float2 *pos[2] = {}, *vel[2] = {};
-  float *theta[2] = {}, *omega[2] = {}, *radius[2] = {};
-  uint32_t *id[2] = {}, *type[2] = {}, *loc[2] = {};
-  float2* acc = nullptr;
-  float *alpha = nullptr, *pressure = nullptr;
-  uint32_t *key = nullptr, *idx = nullptr;
-  uint32_t *sorted_key = nullptr, *sorted_idx = nullptr;
-  uint32_t *cell_start = nullptr, *cell_end = nullptr;
-  SortScratch sort;
-  int cur = 0;
-  hipStream_t cap_stream = nullptr;
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  uint64_t graph_gen = 0;
-  std::vector<void*> allocs;
-};
-
 namespace {
-
 thread_local std::string g_create_error;
+}  // namespace
+
+namespace dem {
 
 int fail(dem_context* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
@@ -74,12 +36,6 @@ int hip_fail(dem_context* ctx, hipError_t e, const char* what) {
   return fail(ctx, e == hipErrorOutOfMemory ? DEM_ERR_ALLOC : DEM_ERR_LAUNCH,
               std::string(what) + ": " + hipGetErrorString(e));
 }
-
-#define DEM_HIP(ctx, call, what)                     \
-  do {                                               \
-    hipError_t e_ = (call);                          \
-    if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
-  } while (0)
 
 bool is_pow2(uint32_t v) { return v && !(v & (v - 1u)); }
 
@@ -186,6 +142,12 @@ int check_cells(dem_context* ctx, uint32_t num_cells) {
 }
 
 bool multi(const dem_context* ctx) { return ctx->c.num_instances > 1; }
+
+}  // namespace dem
+
+using namespace dem;
+
+namespace {
 
 __global__ void k_copy2(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
                         uint32_t* __restrict__ da, uint32_t* __restrict__ db, uint32_t n) {
@@ -469,7 +431,7 @@ int dem_integrate_ex(dem_context* ctx, float* pos, float* vel, const float* acc,
 
 // ---- owned simulation ------------------------------------------------------
 
-namespace {
+namespace dem {
 
 template <class T>
 int sim_alloc(dem_sim* m, T** p, size_t count) {
@@ -566,7 +528,7 @@ int sim_capture(dem_sim* m) {
   return DEM_OK;
 }
 
-}  // namespace
+}  // namespace dem
 
 extern "C" {
 
@@ -727,6 +689,66 @@ int dem_sim_stage_times(dem_sim* m, uint32_t reps, float* ms) {
     ms[k] = (float)(acc[k] / reps);
   }
   return rc;
+}
+
+namespace {
+__global__ void k_get_by_id(const uint32_t* __restrict__ id, const float2* __restrict__ pos,
+                            const float2* __restrict__ vel, float2* __restrict__ pos_out, float2* __restrict__ vel_out,
+                            uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = id[i];
+  if (d >= n) return;
+  if (pos_out) pos_out[d] = pos[i];
+  if (vel_out) vel_out[d] = vel[i];
+}
+
+__global__ void k_set_by_id(const uint32_t* __restrict__ id, const float2* __restrict__ pos_in,
+                            const float2* __restrict__ vel_in, float2* __restrict__ pos, float2* __restrict__ vel,
+                            uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = id[i];
+  if (d >= n) return;
+  if (pos_in) pos[i] = pos_in[d];
+  if (vel_in) vel[i] = vel_in[d];
+}
+}  // namespace
+
+int dem_sim_get_by_id(dem_sim* m, float* pos_by_id, float* vel_by_id) {
+  if (!m) return DEM_ERR_INVALID;
+  dem_context* ctx = m->ctx;
+  if (int r = ensure_device(ctx)) return r;
+  const int c = m->cur;
+  hipLaunchKernelGGL(k_get_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c], m->pos[c], m->vel[c],
+                     (float2*)pos_by_id, (float2*)vel_by_id, m->n);
+  DEM_HIP(ctx, hipGetLastError(), "get by id");
+  return DEM_OK;
+}
+
+int dem_sim_set_by_id(dem_sim* m, const float* pos_by_id, const float* vel_by_id) {
+  if (!m) return DEM_ERR_INVALID;
+  dem_context* ctx = m->ctx;
+  if (int r = ensure_device(ctx)) return r;
+  const int c = m->cur;
+  hipLaunchKernelGGL(k_set_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c],
+                     (const float2*)pos_by_id, (const float2*)vel_by_id, m->pos[c], m->vel[c], m->n);
+  DEM_HIP(ctx, hipGetLastError(), "set by id");
+  return DEM_OK;
+}
+
+int dem_set_contact_params(dem_context* ctx, const double* p, uint32_t num_types) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!p || num_types != ctx->h_mat.size())
+    return fail(ctx, DEM_ERR_INVALID, "need one (kn, kt, cn, mu) row per material");
+  if (int r = ensure_device(ctx)) return r;
+  for (uint32_t t = 0; t < num_types; ++t) {
+    ctx->h_mat[t].kn = (float)p[4 * t + 0];
+    ctx->h_mat[t].kt = (float)p[4 * t + 1];
+    ctx->h_mat[t].cn = (float)p[4 * t + 2];
+    ctx->h_mat[t].mu = (float)p[4 * t + 3];
+  }
+  return upload_tables(ctx);
 }
 
 int dem_sim_sort_passes(dem_sim* m) {

@@ -1,0 +1,67 @@
+// capi_internal.h -- host-side structures behind the C ABI (context, owned
+// simulation) and the helpers shared by capi.hip and adjoint.hip.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "dem_internal.h"
+
+struct dem_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool params_set = false;
+  dem::StepConst c{};
+  dem::Material* d_mat = nullptr;
+  uint32_t mat_cap = 0;
+  dem::Instance* d_inst = nullptr;
+  uint32_t inst_cap = 0;
+  std::vector<dem::Material> h_mat;
+  std::vector<dem::Instance> h_inst;
+  uint32_t total_cells = 0;  // every key produced by hashing is < total_cells
+  uint32_t key_bits = 32;
+  dem::SortScratch sort;
+  uint32_t* d_copy_tmp = nullptr;
+  int bs_hash = 256, bs_reorder = 256, bs_collide = 128, bs_integrate = 256;
+  uint64_t generation = 1;  // bumped by every parameter change
+};
+
+struct dem_sim {
+  dem_context* ctx = nullptr;
+  uint32_t n = 0, cells = 0;
+  bool rad = false;
+  float2 *pos[2] = {}, *vel[2] = {};
+  float *theta[2] = {}, *omega[2] = {}, *radius[2] = {};
+  uint32_t *id[2] = {}, *type[2] = {}, *loc[2] = {};
+  float2* acc = nullptr;
+  float *alpha = nullptr, *pressure = nullptr;
+  uint32_t *key = nullptr, *idx = nullptr;
+  uint32_t *sorted_key = nullptr, *sorted_idx = nullptr;
+  uint32_t *cell_start = nullptr, *cell_end = nullptr;
+  dem::SortScratch sort;
+  int cur = 0;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  uint64_t graph_gen = 0;
+  std::vector<void*> allocs;
+};
+
+namespace dem {
+
+int fail(dem_context* ctx, int code, const std::string& msg);
+int hip_fail(dem_context* ctx, hipError_t e, const char* what);
+int check_ctx(dem_context* ctx);
+int ensure_device(dem_context* ctx);
+bool multi(const dem_context* ctx);
+int sim_check(dem_sim* m);
+// One step (src/main.cu:431-507) on stream s; ev (7 events) or NULL.
+hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev);
+
+}  // namespace dem
+
+#define DEM_HIP(ctx, call, what)                              \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return dem::hip_fail(ctx, e_, what); \
+  } while (0)

@@ -221,6 +221,37 @@ int dem_sim_stage_times(dem_sim* sim, uint32_t reps, float* ms);
 int dem_sim_autotune(dem_sim* sim, uint32_t* block);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);
+/* Current positions / velocities gathered by particle ID into device arrays
+ * (n float2 each; either may be NULL). */
+int dem_sim_get_by_id(dem_sim* sim, float* pos_by_id, float* vel_by_id);
+/* Overwrite current positions / velocities from device arrays indexed by
+ * particle ID (either may be NULL). */
+int dem_sim_set_by_id(dem_sim* sim, const float* pos_by_id, const float* vel_by_id);
+
+/* Update the contact parameters of every material from a host array of
+ * num_types rows (kn, kt, cn, mu), leaving the other fields unchanged. */
+int dem_set_contact_params(dem_context* ctx, const double* kn_kt_cn_mu, uint32_t num_types);
+
+/* ---- adjoint of collide + integrate ---------------------------------------
+ * No counterpart in the reference.  Reverse mode through `steps` steps with
+ * the contact set, approach-only damping and stick/slip frozen at the
+ * forward's decisions (hashing and sorting are not differentiated: the
+ * permutation is a constant), checkpoints every `checkpoint_every` steps and
+ * recomputation of each segment (its neighbour lists are rebuilt, not
+ * stored).  FP32 forward, FP64 adjoint state and parameter gradients.
+ * Attraction is not covered (dem_adj_create fails if a material attracts). */
+typedef struct dem_adjoint dem_adjoint;
+int dem_adj_create(dem_sim* sim, uint32_t checkpoint_every, dem_adjoint** out);
+int dem_adj_destroy(dem_adjoint* adj);
+/* Run `steps` steps from the sim's current state, keeping checkpoints. */
+int dem_adj_forward(dem_adjoint* adj, uint32_t steps);
+/* Given dL/dpos and dL/dvel of the final state (device, n float2 each,
+ * indexed by particle ID; either may be NULL), write dL/dpos0, dL/dvel0 of
+ * the state dem_adj_forward started from (device, by ID; either may be
+ * NULL) and dL/d(kn, kt, cn, mu) of every material (device, num_types x 4
+ * doubles; may be NULL).  The sim is left in the final state. */
+int dem_adj_backward(dem_adjoint* adj, const float* grad_pos, const float* grad_vel, float* grad_pos0,
+                     float* grad_vel0, double* grad_params);
 
 #ifdef __cplusplus
 }

@@ -101,3 +101,11 @@ int main() {
     assert res.returncode == 0, res.stderr
     run = subprocess.run([str(exe)], capture_output=True, text=True)
     assert run.returncode == 0, run.stderr
+
+
+def test_torch_extension_loads():
+    """The libtorch extension is built in-tree and imports without a GPU."""
+    from demso_amd import torch_ext
+
+    mod = torch_ext.load()
+    assert hasattr(mod, "trajectory")
