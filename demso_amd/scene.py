@@ -226,6 +226,28 @@ def column_collapse(seed: int = 0, cols: int = 64, rows: int = 64, n: Optional[i
                  flags=DEM_GRID_CLAMP | DEM_OPEN_TOP, steps=steps, **st)
 
 
+def calibration_column(seed: int = 3, cols: int = 256, rows: int = 256, kn: float = 1e5, kt: float = 2e4,
+                       e: float = 0.5, mu: float = 0.5, dt: float = 1e-5, steps: int = 2000,
+                       wall_restitution: float = 0.5) -> Scene:
+    """Column collapse with config-0 distributions at any lattice size (the
+    adjoint calibration workload: 256 x 256 = 65,536 discs): the config-0 box
+    scaled by cols/64, cells of 0.011 m covering the box and the lattice
+    above its open top, one material (kt, cn from e), boundary damping
+    fixed so it is not confounded with the calibrated restitution."""
+    pitch, jitter = 0.0115, 0.0005
+    n = cols * rows
+    box = (1.0 * cols / 64.0, 0.5 * cols / 64.0)
+    pos = _lattice(seed, 0, cols, rows, n, pitch, jitter, (0.0, 0.0))
+    radius = 0.0045 + 0.001 * uniform(seed, 2, n)
+    grid, extent = _grid_for((box[0], max(box[1], rows * pitch + jitter)), 0.011)
+    mat = disc_material(kn, kt, e, mu, 0.005)
+    mat.bdamp = wall_restitution
+    st = _state(pos, radius, np.zeros(n, np.uint32))
+    return Scene(name="calibration_column", materials=[mat], dt=dt, bkn=kn, bkt=kt,
+                 instances=[Instance(box=box, extent=extent, grid=grid)],
+                 flags=DEM_GRID_CLAMP | DEM_OPEN_TOP, steps=steps, notes={"restitution": [e]}, **st)
+
+
 def settling(seed: int = 1, side: int = 2048, r0: float = 0.005, sigma: float = 0.2,
              jitter: float = 0.0005, kn: float = 1e5, kt: float = 2e4, e: float = 0.5,
              mu: float = 0.5, dt: float = 1e-5, steps: int = 1000) -> Scene:

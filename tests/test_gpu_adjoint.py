@@ -111,3 +111,80 @@ def test_torch_autograd_function(gpu_ctx, weights):
     assert np.allclose(pos0.grad.cpu().numpy(), gp0, rtol=1e-6, atol=1e-9)
     assert np.allclose(params.grad.numpy(), gpar, rtol=1e-9, atol=0)
     traj.close()
+
+
+def fd_subcase(gpu_ctx):
+    """Config-3 finite-difference sub-case: 512 discs (32 x 16 lattice with
+    config-0 distributions) settled for 2000 steps, then 200 steps with
+    loss = mean squared final position error against a target run
+    (kn = 2e5, mu = 0.35, e = 0.6)."""
+    from demso_amd import calibrate
+
+    s = calibrate.settle(gpu_ctx, sc.calibration_column(seed=3, cols=32, rows=16), 2000)
+    tgt = calibrate.set_params(s, [2e5], [0.35], [0.6])
+    return s, tgt
+
+
+def test_config3_fd_subcase(gpu_ctx, oracle):
+    """dL/d(kn, kt, cn, mu) from the GPU adjoint (FP32 forward, FP64
+    accumulation) and from the FP64 host adjoint against central finite
+    differences of the FP64 oracle's loss (relative step 1e-4; the loss is
+    smooth over that step, as the frozen contact set and stick/slip states
+    assume).  Checked on every parameter whose log-sensitivity |p dL/dp| is
+    at least 1e-3 of the largest (friction hardly acts in a settled column:
+    nearly every contact sticks)."""
+    from demso_amd import calibrate
+
+    s, tgt = fd_subcase(gpu_ctx)
+    steps, rel = 200, 1e-4
+    a = oracle.AdjointOracle(tgt)
+    target, _ = a.forward(steps)
+    a.close()
+
+    def loss(scene):
+        o = oracle.AdjointOracle(scene)
+        p, _ = o.forward(steps)
+        o.close()
+        return float(((p - target) ** 2).sum()) / scene.n
+
+    run = calibrate.Runner(gpu_ctx, s, steps, checkpoint_every=50)
+    res = run.run(s, target)
+    run.close()
+    host = oracle.AdjointOracle(s)
+    pf, _ = host.forward(steps)
+    _, _, ghost = host.backward(2.0 * (pf - target) / s.n)
+    host.close()
+    assert res.loss == pytest.approx(loss(s), rel=1e-3)
+    names = ("kn", "kt", "cn", "mu")
+    base = [getattr(s.materials[0], k) for k in names]
+    sens = [abs(ghost[0, i] * base[i]) for i in range(4)]
+    checked = 0
+    for idx, name in enumerate(names):
+        vals = []
+        for sign in (1, -1):
+            t = s.copy()
+            setattr(t.materials[0], name, base[idx] * (1 + sign * rel))
+            vals.append(loss(t))
+        hp = float(np.float32(base[idx] * (1 + rel))) - float(np.float32(base[idx]))
+        hm = float(np.float32(base[idx])) - float(np.float32(base[idx] * (1 - rel)))
+        fd = (vals[0] - vals[1]) / (hp + hm)
+        gpu, cpu = res.grad_params[0, idx], ghost[0, idx]
+        print(f"{name}: gpu {gpu:.6e} host {cpu:.6e} fd {fd:.6e} "
+              f"gpu/fd-1 {gpu / fd - 1 if fd else float('nan'):.2e} host/fd-1 {cpu / fd - 1 if fd else float('nan'):.2e}")
+        if sens[idx] < 1e-3 * max(sens):
+            continue
+        assert cpu == pytest.approx(fd, rel=1e-3)
+        assert gpu == pytest.approx(fd, rel=1e-3)
+        checked += 1
+    assert checked >= 2
+
+
+def test_calibration_reduces_loss(gpu_ctx):
+    from demso_amd import calibrate
+
+    s = calibrate.settle(gpu_ctx, sc.calibration_column(seed=3, cols=24, rows=24), 2000)
+    tgt = calibrate.set_params(s, [2e5], [0.35], [0.6])
+    out = calibrate.calibrate(gpu_ctx, s, tgt, steps=150, iters=6, lr=0.1, checkpoint_every=50)
+    losses = [h["loss"] for h in out["history"]]
+    print("losses", losses)
+    assert losses[-1] < losses[0]
