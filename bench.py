@@ -60,6 +60,12 @@ def make_scene(name: str):
     if name == "ensemble":
         s = sc.ensemble()
         return s, f"ensemble: 256 instances, {s.n} discs, {s.total_cells} cells, FP32"
+    if name == "adjoint":
+        s = sc.calibration_column(seed=3, cols=256, rows=256)
+        return s, "adjoint: 65,536-disc column (256x256 lattice, settled 5000 steps), 2000 steps, checkpoints every 100"
+    if name == "calibration":
+        s = sc.ensemble(seed=4, instances=64, nmin=16384, nmax=131072)
+        return s, f"calibration: 64 instances, {s.n} discs, 1000 steps fwd+adjoint per Adam iteration"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -76,6 +82,69 @@ def one_by_one(ctx, scene, steps: int) -> dict:
         sim.close()
     return {"value": scene.n * steps / total, "unit": "particle-steps/s", "seconds": total,
             "instances": len(scene.instances)}
+
+
+def adjoint_cost(ctx, scene, steps: int, K: int, settle_steps: int) -> dict:
+    """Plain forward, forward with checkpoints and backward of one run of
+    `steps` steps from a settled packing (wall times, after an untimed
+    warm-up of each); loss = mean squared final position error against a
+    target run with kn = 2e5, mu = 0.35, e = 0.6."""
+    from demso_amd import api, calibrate
+
+    s = calibrate.settle(ctx, scene, settle_steps)
+    tgt = calibrate.set_params(s, [2e5], [0.35], [0.6])
+    run = calibrate.Runner(ctx, s, steps, checkpoint_every=K)
+    target = run.final_positions(tgt).astype("float64")
+    run.run(s, target)  # warm-up
+    res = run.run(s, target)
+    sim = run.sim
+    ctx.set_contact_params(calibrate.contact_rows(s))
+    sim.upload(s)
+    sim.step_graph(steps)  # warm-up
+    sim.upload(s)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sim.step_graph(steps)
+    ctx.synchronize()
+    plain = time.perf_counter() - t0
+    run.close()
+    return {
+        "steps": steps, "checkpoint_every": K, "particles": s.n,
+        "plain_forward_s": plain, "forward_with_checkpoints_s": res.seconds_forward,
+        "backward_s": res.seconds_backward,
+        "backward_over_forward": res.seconds_backward / plain,
+        "forward_plus_backward_over_forward": (res.seconds_forward + res.seconds_backward) / plain,
+        "loss": res.loss, "grad_params": res.grad_params.tolist(),
+    }
+
+
+def calibration_run(ctx, scene, steps: int, iters: int, K: int, settle_steps: int) -> dict:
+    """Batched calibration: every instance starts from (kn, mu, e) =
+    (1e5, 0.5, 0.5) and Adam recovers its own drawn parameters from the
+    final positions of a target run with them (all instances in one launch
+    per stage)."""
+    from demso_amd import calibrate
+
+    s = calibrate.settle(ctx, scene, settle_steps)
+    T = len(s.materials)
+    truth = {"kn": [m.kn for m in s.materials], "mu": [m.mu for m in s.materials],
+             "e": list(s.notes.get("restitution", [])) or None}
+    tgt = s.copy()
+    start = calibrate.set_params(s, [1e5] * T, [0.5] * T, [0.5] * T)
+    t0 = time.perf_counter()
+    out = calibrate.calibrate(ctx, start, tgt, steps=steps, iters=iters, lr=0.1, checkpoint_every=K)
+    secs = time.perf_counter() - t0
+    hist = out["history"]
+    kn_err = [abs(a / b - 1) for a, b in zip(out["kn"], truth["kn"])]
+    mu_err = [abs(a - b) for a, b in zip(out["mu"], truth["mu"])]
+    return {
+        "instances": T, "particles": s.n, "steps": steps, "iterations": iters, "checkpoint_every": K,
+        "seconds": secs, "seconds_per_iteration": secs / iters,
+        "forward_s_mean": sum(h["t_fwd"] for h in hist) / iters,
+        "backward_s_mean": sum(h["t_bwd"] for h in hist) / iters,
+        "loss_first": hist[0]["loss"], "loss_last": hist[-1]["loss"],
+        "kn_rel_err_median": sorted(kn_err)[len(kn_err) // 2], "mu_abs_err_median": sorted(mu_err)[len(mu_err) // 2],
+    }
 
 
 def cpu_baseline(scene, steps: int, threads: int) -> dict:
@@ -102,7 +171,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="settling", choices=["settling", "column_collapse", "ensemble"])
+    ap.add_argument("--workload", default="settling", choices=["settling", "column_collapse", "ensemble", "adjoint", "calibration"])
+    ap.add_argument("--adj-steps", type=int, default=0, help="steps of the adjoint / calibration run (0 = workload default)")
+    ap.add_argument("--iters", type=int, default=5, help="Adam iterations of the calibration workload")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-autotune", action="store_true")
@@ -165,6 +236,14 @@ def main() -> int:
     if args.cpu_steps > 0:
         cpu = cpu_baseline(scene, args.cpu_steps, args.cpu_threads)
 
+    extra = {}
+    if args.workload == "adjoint":
+        sim.close()
+        extra["adjoint"] = adjoint_cost(ctx, scene, args.adj_steps or 2000, 100, 5000)
+    if args.workload == "calibration":
+        sim.close()
+        extra["calibration"] = calibration_run(ctx, scene, args.adj_steps or 1000, args.iters, 100, 2000)
+
     ensemble = None
     if args.workload == "ensemble":
         sim.close()
@@ -207,6 +286,7 @@ def main() -> int:
     }
     if ensemble is not None:
         out["ensemble"] = ensemble
+    out.update(extra)
     print(json.dumps(out), flush=True)
     sim.close()
     ctx.close()

@@ -114,6 +114,8 @@ PROTOTYPES = {
     "dem_sim_sort_passes": (c_int, [c_void_p]),
     "dem_sim_get_by_id": (c_int, [c_void_p, P, P]),
     "dem_sim_set_by_id": (c_int, [c_void_p, P, P]),
+    "dem_gather_rows": (c_int, [c_void_p, P, c_uint32] + [P] * 10 + [c_uint32, P]),
+    "dem_sim_gather_rows": (c_int, [c_void_p, P, c_uint32, P]),
     "dem_set_contact_params": (c_int, [c_void_p, POINTER(ctypes.c_double), c_uint32]),
     "dem_adj_create": (c_int, [c_void_p, c_uint32, POINTER(c_void_p)]),
     "dem_adj_destroy": (c_int, [c_void_p]),

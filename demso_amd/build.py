@@ -1,4 +1,5 @@
-"""Build the gfx950 HIP library (libdemso.so) in-tree.
+"""Build the gfx950 HIP library (libdemso.so) and the headless driver
+(demso_run) in-tree.
 
 Every HIP source is compiled with:
   -ffp-contract=off                       no FMA contraction: the FP32 arithmetic
@@ -62,6 +63,27 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+DRIVER = PKG / "demso_run"
+DRIVER_SOURCES = ["xml.cpp", "dsml.cpp", "prepare.cpp", "demso_run.cpp"]
+
+
+def build_driver(verbose: bool = False) -> Path:
+    """Compile the headless .dsml driver (host C++ over libdemso.so and the
+    HIP runtime) into demso_amd/demso_run."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    tmp = DRIVER.with_name("demso_run.tmp")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include",
+           f"-I{ROOT / 'include'}", *[str(PKG / "driver" / f) for f in DRIVER_SOURCES],
+           f"-L{PKG}", "-ldemso", f"-L{rocm}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed ({res.returncode}):\n{res.stderr}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     odir = ROOT / "oracle"
@@ -75,4 +97,5 @@ def build_oracle(verbose: bool = False) -> Path:
 
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_driver(verbose=True))
     print(build_oracle(verbose=True))

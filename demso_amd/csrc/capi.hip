@@ -737,6 +737,65 @@ int dem_sim_set_by_id(dem_sim* m, const float* pos_by_id, const float* vel_by_id
   return DEM_OK;
 }
 
+namespace {
+// One output row per followed particle, 12 words: id, type, pos.x, pos.y,
+// vel.x, vel.y, acc.x, acc.y, theta, omega, alpha, pressure (floats as their
+// bit patterns).  A followed ID outside [0, n) gives a row of 0xFFFFFFFF.
+__global__ void k_gather_rows(const uint32_t* __restrict__ follow, uint32_t count, const uint32_t* __restrict__ loc,
+                              const float2* __restrict__ pos, const float2* __restrict__ vel,
+                              const float2* __restrict__ acc, const float* __restrict__ theta,
+                              const float* __restrict__ omega, const float* __restrict__ alpha,
+                              const uint32_t* __restrict__ id, const uint32_t* __restrict__ type,
+                              const float* __restrict__ pressure, uint32_t n, uint32_t* __restrict__ rows) {
+  const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= count) return;
+  uint32_t* r = rows + (size_t)kRowWords * f;
+  const uint32_t d = follow[f];
+  const uint32_t i = d < n ? loc[d] : n;
+  if (i >= n) {
+    for (int k = 0; k < kRowWords; ++k) r[k] = 0xFFFFFFFFu;
+    return;
+  }
+  const float2 p = pos[i], v = vel[i], a = acc[i];
+  r[0] = id[i];
+  r[1] = type[i];
+  r[2] = __float_as_uint(p.x);
+  r[3] = __float_as_uint(p.y);
+  r[4] = __float_as_uint(v.x);
+  r[5] = __float_as_uint(v.y);
+  r[6] = __float_as_uint(a.x);
+  r[7] = __float_as_uint(a.y);
+  r[8] = __float_as_uint(theta[i]);
+  r[9] = __float_as_uint(omega[i]);
+  r[10] = __float_as_uint(alpha[i]);
+  r[11] = __float_as_uint(pressure[i]);
+}
+}  // namespace
+
+int dem_gather_rows(dem_context* ctx, const uint32_t* follow_ids, uint32_t count, const uint32_t* loc,
+                    const float* pos, const float* vel, const float* acc, const float* theta, const float* omega,
+                    const float* alpha, const uint32_t* id, const uint32_t* type, const float* pressure,
+                    uint32_t numParticles, uint32_t* rows) {
+  if (int r = check_ctx(ctx)) return r;
+  if (count == 0) return DEM_OK;
+  if (!follow_ids || !loc || !pos || !vel || !acc || !theta || !omega || !alpha || !id || !type || !pressure || !rows)
+    return fail(ctx, DEM_ERR_INVALID, "dem_gather_rows: null pointer");
+  if (int r = ensure_device(ctx)) return r;
+  hipLaunchKernelGGL(k_gather_rows, dim3((count + 63) / 64), dim3(64), 0, ctx->stream, follow_ids, count, loc,
+                     (const float2*)pos, (const float2*)vel, (const float2*)acc, theta, omega, alpha, id, type,
+                     pressure, numParticles, rows);
+  DEM_HIP(ctx, hipGetLastError(), "gather rows");
+  return DEM_OK;
+}
+
+int dem_sim_gather_rows(dem_sim* m, const uint32_t* follow_ids, uint32_t count, uint32_t* rows) {
+  if (!m) return DEM_ERR_INVALID;
+  const int c = m->cur;
+  return dem_gather_rows(m->ctx, follow_ids, count, m->loc[c], (const float*)m->pos[c], (const float*)m->vel[c],
+                         (const float*)m->acc, m->theta[c], m->omega[c], m->alpha, m->id[c], m->type[c], m->pressure,
+                         m->n, rows);
+}
+
 int dem_set_contact_params(dem_context* ctx, const double* p, uint32_t num_types) {
   if (int r = check_ctx(ctx)) return r;
   if (!p || num_types != ctx->h_mat.size())

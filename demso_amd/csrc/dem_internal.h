@@ -10,6 +10,7 @@ namespace dem {
 
 constexpr uint32_t kEmpty = DEM_EMPTY_CELL;
 constexpr int kWave = 64;
+constexpr int kRowWords = DEM_ROW_WORDS;
 
 // Material row as the kernels read it (48 bytes, one row per type).  Fields
 // of ParticleProperties (src/main.cuh:39-54) that the step uses, plus the

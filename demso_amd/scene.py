@@ -288,7 +288,7 @@ def ensemble(seed: int = 2, instances: int = 256, nmin: int = 2000, nmax: int = 
     counts = np.asarray(counts, dtype=np.int64)
     uk, umu, ue = uniform(seed, 11, instances), uniform(seed, 12, instances), uniform(seed, 13, instances)
     r0 = 0.5 * (rmin + rmax)
-    pos_parts, rad_parts, type_parts, mats, insts = [], [], [], [], []
+    pos_parts, rad_parts, type_parts, mats, insts, es = [], [], [], [], [], []
     base = 0
     for k, nk in enumerate(counts.tolist()):
         side = int(math.ceil(math.sqrt(nk)))
@@ -302,6 +302,7 @@ def ensemble(seed: int = 2, instances: int = 256, nmin: int = 2000, nmax: int = 
         mu = mu_range[0] + umu[k] * (mu_range[1] - mu_range[0])
         e = e_range[0] + ue[k] * (e_range[1] - e_range[0])
         mats.append(disc_material(kn, 0.2 * kn, e, mu, r0, instance=k))
+        es.append(e)
         insts.append(Instance(box=box, extent=extent, grid=grid, cell_base=base))
         base += grid[0] * grid[1]
         pos_parts.append(_lattice(seed, 100 + 4 * k, side, rows, nk, pitch, jitter, (0.0, 0.0)))
@@ -312,7 +313,8 @@ def ensemble(seed: int = 2, instances: int = 256, nmin: int = 2000, nmax: int = 
     # one global wall stiffness for all instances (SystemProperties holds one):
     # stiff enough that the series combination with each instance's kt is ~kt
     return Scene(name="ensemble", materials=mats, instances=insts, dt=dt, bkn=1e6, bkt=1e6,
-                 flags=DEM_GRID_CLAMP | DEM_OPEN_TOP, steps=steps, segments=seg, **st)
+                 flags=DEM_GRID_CLAMP | DEM_OPEN_TOP, steps=steps, segments=seg,
+                 notes={"restitution": es}, **st)
 
 
 def reference_grid_size(box: Tuple[float, float], max_radius: float) -> Tuple[int, int]:

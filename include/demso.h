@@ -228,6 +228,20 @@ int dem_sim_get_by_id(dem_sim* sim, float* pos_by_id, float* vel_by_id);
  * particle ID (either may be NULL). */
 int dem_sim_set_by_id(dem_sim* sim, const float* pos_by_id, const float* vel_by_id);
 
+/* Output rows of followed particles, replacing the per-value copies of
+ * writeOutputFile (src/functions.cu:608-661): for each of `count` particle
+ * IDs (device array follow_ids), row f of `rows` (device, count x
+ * DEM_ROW_WORDS uint32) receives id, type, pos.x, pos.y, vel.x, vel.y,
+ * acc.x, acc.y, theta, omega, alpha, pressure of the particle at sorted
+ * index loc[ID] (floats as their bit patterns; an ID >= numParticles gives
+ * a row of 0xFFFFFFFF).  Asynchronous on the context stream. */
+#define DEM_ROW_WORDS 12
+int dem_gather_rows(dem_context* ctx, const uint32_t* follow_ids, uint32_t count, const uint32_t* loc,
+                    const float* pos, const float* vel, const float* acc, const float* theta, const float* omega,
+                    const float* alpha, const uint32_t* id, const uint32_t* type, const float* pressure,
+                    uint32_t numParticles, uint32_t* rows);
+/* The same rows from the sim's current state. */
+int dem_sim_gather_rows(dem_sim* sim, const uint32_t* follow_ids, uint32_t count, uint32_t* rows);
 /* Update the contact parameters of every material from a host array of
  * num_types rows (kn, kt, cn, mu), leaving the other fields unchanged. */
 int dem_set_contact_params(dem_context* ctx, const double* kn_kt_cn_mu, uint32_t num_types);

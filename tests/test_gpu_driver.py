@@ -1,0 +1,89 @@
+"""The headless driver on the GPU: a .dsml scene stepped through the
+reference's loop (the five wrappers with their reference signatures) and
+through graph replays, against each other and against the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from dsml_scenes import MIXED, MIXED_BOX, python_scene, run_driver, write
+from demso_amd import dumps
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 100
+
+
+def run_loop(tmp_path, loop: str):
+    scene = write(tmp_path, "mixed.dsml", MIXED)
+    out = tmp_path / f"out_{loop}.csv"
+    prefix = tmp_path / f"{loop}_"
+    print(f"driver: --loop {loop} --steps {STEPS}", flush=True)
+    r = run_driver("--loop", loop, "--steps", STEPS, "--output", out, "--dump-every", 50, "--dump-prefix", prefix,
+                   "--seed", 3, scene, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return out, prefix
+
+
+@pytest.fixture(scope="module")
+def runs(tmp_path_factory):
+    tmp = tmp_path_factory.mktemp("driver")
+    scene = write(tmp, "mixed.dsml", MIXED)
+    r = run_driver("--seed", 3, "--dump-initial", tmp / "initial.bin", scene)
+    assert r.returncode == 0, r.stderr
+    return tmp, {loop: run_loop(tmp, loop) for loop in ("stages", "graph")}
+
+
+def test_stage_loop_equals_graph_loop(runs):
+    tmp, res = runs
+    for step in (50, 100):
+        a = dumps.read_dump(f"{res['stages'][1]}{step}.bin")
+        b = dumps.read_dump(f"{res['graph'][1]}{step}.bin")
+        assert a["step"] == b["step"] == step
+        for k in ("pos", "vel", "theta", "omega", "type", "acc", "alpha", "pressure"):
+            assert np.array_equal(a[k], b[k]), k
+    assert res["stages"][0].read_text() == res["graph"][0].read_text()
+
+
+def test_driver_matches_oracle(runs, oracle):
+    """The driver's state after 50 and 100 steps against the oracle run from
+    the driver's own initial state (wrapping grid, attraction on one type)."""
+    tmp, res = runs
+    init = dumps.read_dump(tmp / "initial.bin")
+    scene = python_scene(init, ["a", "b", "big"], MIXED_BOX, dt=1e-3, bkn=1000.0, bkt=100.0)
+    sim = oracle.Sim(scene, "f32")
+    done = 0
+    for step in (50, 100):
+        sim.step(step - done)
+        done = step
+        o = sim.state()
+        ref = {}
+        order = np.argsort(o["id"])
+        for k in ("pos", "vel", "theta", "omega", "acc", "alpha", "pressure"):
+            ref[k] = o[k][order]
+        g = dumps.read_dump(f"{res['stages'][1]}{step}.bin")
+        for k, v in ref.items():
+            scale = max(float(np.abs(v).max()), 1e-30)
+            err = float(np.abs(g[k].astype(np.float64) - v).max()) / scale
+            print(f"step {step} {k}: max rel err {err:.2e}")
+            assert err <= 1e-5, k
+    sim.close()
+
+
+def test_output_file(runs):
+    """One line per step: the elapsed time (timeStep * step in float), then
+    the followed particles (the two follow="true" singles, then <follow>
+    IDs 5 and 40) with the state of the step's end."""
+    tmp, res = runs
+    times, rows = dumps.read_output(res["stages"][0])
+    assert rows.shape == (STEPS, 4, 12)
+    assert np.allclose(times, np.float32(1e-3) * np.arange(1, STEPS + 1, dtype=np.float32), atol=1e-6)
+    assert rows[:, :, 0].astype(int).tolist() == [[0, 1, 5, 40]] * STEPS
+    final = dumps.read_dump(f"{res['stages'][1]}{STEPS}.bin")
+    for j, pid in enumerate((0, 1, 5, 40)):
+        r = rows[-1, j]
+        assert int(r[1]) == int(final["type"][pid])
+        exp = np.concatenate([final["pos"][pid], final["vel"][pid], final["acc"][pid],
+                              [final["theta"][pid], final["omega"][pid], final["alpha"][pid],
+                               final["pressure"][pid]]])
+        assert np.allclose(r[2:], exp, rtol=0, atol=5e-7 + 1e-6 * np.abs(exp)), (pid, r[2:], exp)
