@@ -339,6 +339,17 @@ class Simulation:
     def sort_passes(self) -> int:
         return int(self.lib.dem_sim_sort_passes(self.handle))
 
+    def set_sort_mode(self, mode: int) -> None:
+        """0 auto, 1 radix sort every step, 2 re-sort whenever possible."""
+        self.ctx._check(self.lib.dem_sim_set_sort_mode(self.handle, mode))
+
+    def mover_count(self) -> int:
+        """Particles that changed cell in the last re-sorted step."""
+        self.ctx.synchronize()
+        v = c_uint32()
+        self.ctx._check(self.lib.dem_sim_mover_count(self.handle, byref(v)))
+        return int(v.value)
+
     def get_by_id(self):
         """Current positions and velocities as (n, 2) GPU tensors indexed by particle id."""
         import torch

@@ -611,7 +611,8 @@ int dem_adj_forward(dem_adjoint* a, uint32_t steps) {
   hipStream_t s = ctx->stream;
   for (uint32_t k = 0; k < steps; ++k) {
     if (k % a->K == 0) DEM_HIP(ctx, copy_state(m, a->ckpt[k / a->K], true, s), "checkpoint");
-    DEM_HIP(ctx, sim_step_on(m, s, nullptr), "forward step");
+    DEM_HIP(ctx, sim_step_on(m, s, nullptr, sim_use_resort(m)), "forward step");
+    m->resort_ok = true;
   }
   DEM_HIP(ctx, copy_state(m, a->final_state, true, s), "final state");
   a->steps = steps;
@@ -699,6 +700,7 @@ int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_ve
     DEM_HIP(ctx, hipMemcpyAsync(grad_params, a->grad, sizeof(double) * kParams * c.num_types, hipMemcpyDeviceToDevice, s),
             "grad copy");
   DEM_HIP(ctx, copy_state(m, a->final_state, false, s), "restore final state");
+  m->resort_ok = false;  // the recomputation rewrote the cell arrays
   DEM_HIP(ctx, hipStreamSynchronize(s), "adjoint sync");
   return DEM_OK;
 }

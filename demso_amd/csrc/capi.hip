@@ -442,8 +442,11 @@ int sim_alloc(dem_sim* m, T** p, size_t count) {
 }
 
 void sim_free(dem_sim* m) {
-  for (int k = 0; k < 2; ++k)
+  for (int k = 0; k < 2; ++k) {
     if (m->exec[k]) (void)hipGraphExecDestroy(m->exec[k]);
+    if (m->exec_inc[k]) (void)hipGraphExecDestroy(m->exec_inc[k]);
+  }
+  if (m->h_count) (void)hipHostFree(m->h_count);
   if (m->cap_stream) (void)hipStreamDestroy(m->cap_stream);
   for (void* p : m->allocs) (void)hipFree(p);
   free_sort(m->sort);
@@ -451,22 +454,40 @@ void sim_free(dem_sim* m) {
 }
 
 // Stage sequence of one step (src/main.cu:431-507) on stream s; `ev` (8
-// events or NULL) brackets each stage for timing.
-hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev) {
+// events or NULL) brackets each stage for timing.  The sorted keys go to
+// skey[t] and the sorted index to sidx with either sort, so the next step
+// can re-sort from them.
+hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort) {
   dem_context* ctx = m->ctx;
   const StepConst& c = ctx->c;
   const int o = m->cur, t = o ^ 1;
   const uint32_t n = m->n;
+  const uint32_t* type_for_hash = multi(ctx) ? m->type[o] : nullptr;
   hipError_t e;
   if (ev) (void)hipEventRecord(ev[0], s);
-  e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n);
-  if (e != hipSuccess) return e;
-  if (ev) (void)hipEventRecord(ev[1], s);
-  bool in_alt = false;
-  e = dem::launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt);
-  if (e != hipSuccess) return e;
-  m->sorted_key = in_alt ? m->sort.keys_alt : m->key;
-  m->sorted_idx = in_alt ? m->sort.vals_alt : m->idx;
+  if (resort) {
+    e = dem::launch_hash_movers(c, s, m->pos[o], type_for_hash, m->skey[o], m->key, m->resort.bits,
+                                m->resort.tile_count, n);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], s);
+    e = dem::launch_resort(s, m->key, m->skey[o], m->cell_start, m->cell_end, m->cells, true, ctx->key_bits,
+                           m->skey[t], m->sidx, m->resort, n);
+    if (e != hipSuccess) return e;
+  } else {
+    e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], type_for_hash, m->key, m->idx, n);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], s);
+    bool in_alt = false;
+    e = dem::launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt);
+    if (e != hipSuccess) return e;
+    const uint32_t* rk = in_alt ? m->sort.keys_alt : m->key;
+    const uint32_t* ri = in_alt ? m->sort.vals_alt : m->idx;
+    hipLaunchKernelGGL(k_copy2, dim3((n + 255) / 256), dim3(256), 0, s, rk, ri, m->skey[t], m->sidx, n);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  m->sorted_key = m->skey[t];
+  m->sorted_idx = m->sidx;
   if (ev) (void)hipEventRecord(ev[2], s);
   e = dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells);
   if (e != hipSuccess) return e;
@@ -489,6 +510,15 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev) {
   return hipSuccess;
 }
 
+bool sim_use_resort(const dem_sim* m) {
+  if (!m->resort_ok || m->sort_mode == 1) return false;
+  if (m->sort_mode == 2) return true;
+  // The re-sort's movers step is one workgroup: fast up to a few thousand
+  // movers, slower than the radix sort far beyond.
+  const uint32_t last = *(volatile const uint32_t*)m->h_count;
+  return last <= std::max(4096u, m->n / 64u);
+}
+
 // The sim's cell arrays were sized at creation; a later parameter change
 // must not make hashing produce keys beyond them.
 int sim_check(dem_sim* m) {
@@ -501,28 +531,34 @@ int sim_capture(dem_sim* m) {
   dem_context* ctx = m->ctx;
   if (!m->cap_stream) DEM_HIP(ctx, hipStreamCreateWithFlags(&m->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
   for (int k = 0; k < 2; ++k) {
-    if (m->exec[k]) {
-      (void)hipGraphExecDestroy(m->exec[k]);
-      m->exec[k] = nullptr;
-    }
+    if (m->exec[k]) (void)hipGraphExecDestroy(m->exec[k]);
+    if (m->exec_inc[k]) (void)hipGraphExecDestroy(m->exec_inc[k]);
+    m->exec[k] = m->exec_inc[k] = nullptr;
   }
   const int saved = m->cur;
-  for (int start = 0; start < 2; ++start) {
-    hipGraph_t g = nullptr;
-    DEM_HIP(ctx, hipStreamBeginCapture(m->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    m->cur = start;
-    hipError_t e1 = sim_step_on(m, m->cap_stream, nullptr);
-    hipError_t e2 = e1 == hipSuccess ? sim_step_on(m, m->cap_stream, nullptr) : e1;
-    hipError_t e3 = hipStreamEndCapture(m->cap_stream, &g);
-    m->cur = saved;
-    if (e2 != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return hip_fail(ctx, e2, "capturing the step");
+  uint32_t* const saved_key = m->sorted_key;
+  uint32_t* const saved_idx = m->sorted_idx;
+  for (int variant = 0; variant < 2; ++variant) {
+    for (int start = 0; start < 2; ++start) {
+      hipGraph_t g = nullptr;
+      DEM_HIP(ctx, hipStreamBeginCapture(m->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      m->cur = start;
+      hipError_t e1 = sim_step_on(m, m->cap_stream, nullptr, variant == 1);
+      hipError_t e2 = e1 == hipSuccess ? sim_step_on(m, m->cap_stream, nullptr, variant == 1) : e1;
+      hipError_t e3 = hipStreamEndCapture(m->cap_stream, &g);
+      m->cur = saved;
+      m->sorted_key = saved_key;
+      m->sorted_idx = saved_idx;
+      if (e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(ctx, e2, "capturing the step");
+      }
+      if (e3 != hipSuccess) return hip_fail(ctx, e3, "hipStreamEndCapture");
+      hipGraphExec_t* slot = variant == 1 ? &m->exec_inc[start] : &m->exec[start];
+      hipError_t e4 = hipGraphInstantiate(slot, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e4 != hipSuccess) return hip_fail(ctx, e4, "hipGraphInstantiate");
     }
-    if (e3 != hipSuccess) return hip_fail(ctx, e3, "hipStreamEndCapture");
-    hipError_t e4 = hipGraphInstantiate(&m->exec[start], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e4 != hipSuccess) return hip_fail(ctx, e4, "hipGraphInstantiate");
   }
   m->graph_gen = ctx->generation;
   return DEM_OK;
@@ -564,6 +600,29 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   if (!r) r = sim_alloc(m, &m->cell_start, num_cells);
   if (!r) r = sim_alloc(m, &m->cell_end, num_cells);
   if (!r) r = alloc_sort(ctx, m->sort, n);
+  const uint32_t words = (n + 31u) / 32u, tiles = (n + dem::kResortTile - 1u) / dem::kResortTile;
+  for (int b = 0; b < 2 && r == DEM_OK; ++b) {
+    if (!r) r = sim_alloc(m, &m->skey[b], n);
+    if (!r) r = sim_alloc(m, &m->resort.mkey[b], n);
+    if (!r) r = sim_alloc(m, &m->resort.mpos[b], n);
+  }
+  if (!r) r = sim_alloc(m, &m->sidx, n);
+  if (!r) r = sim_alloc(m, &m->resort.bits, words);
+  if (!r) r = sim_alloc(m, &m->resort.word_prefix, words);
+  if (!r) r = sim_alloc(m, &m->resort.tile_count, tiles);
+  if (!r) r = sim_alloc(m, &m->resort.info, 2);
+  if (!r) {
+    if (hipHostMalloc((void**)&m->h_count, sizeof(uint32_t), hipHostMallocMapped) != hipSuccess) {
+      m->h_count = nullptr;
+      r = fail(ctx, DEM_ERR_ALLOC, "hipHostMalloc(mover count)");
+    } else {
+      *m->h_count = 0;
+      void* dp = nullptr;
+      if (hipHostGetDevicePointer(&dp, m->h_count, 0) != hipSuccess) r = fail(ctx, DEM_ERR_ALLOC, "hipHostGetDevicePointer");
+      m->resort.host_count = (uint32_t*)dp;
+    }
+  }
+  m->resort.capacity = n;
   if (r) {
     sim_free(m);
     return r;
@@ -609,6 +668,8 @@ int dem_sim_upload(dem_sim* m, const float* pos, const float* vel, const float* 
   DEM_HIP(ctx, hipMemcpyAsync(m->loc[c], loc, 4 * n, hipMemcpyHostToDevice, s), "upload");
   if (m->rad) DEM_HIP(ctx, hipMemcpyAsync(m->radius[c], radius, 4 * n, hipMemcpyHostToDevice, s), "upload");
   DEM_HIP(ctx, hipStreamSynchronize(s), "upload sync");
+  m->resort_ok = false;  // an uploaded state is in no known sorted order
+  *m->h_count = 0;
   return DEM_OK;
 }
 
@@ -634,11 +695,19 @@ int dem_sim_download(dem_sim* m, float* pos, float* vel, float* theta, float* om
   return DEM_OK;
 }
 
+// One eager step with the sort method chosen for it.
+int sim_step_auto(dem_sim* m) {
+  const bool resort = sim_use_resort(m);
+  DEM_HIP(m->ctx, sim_step_on(m, m->ctx->stream, nullptr, resort), "step");
+  m->resort_ok = true;
+  return DEM_OK;
+}
+
 int dem_sim_step(dem_sim* m, uint32_t steps) {
   if (!m) return DEM_ERR_INVALID;
-  dem_context* ctx = m->ctx;
   if (int r = sim_check(m)) return r;
-  for (uint32_t k = 0; k < steps; ++k) DEM_HIP(ctx, sim_step_on(m, ctx->stream, nullptr), "step");
+  for (uint32_t k = 0; k < steps; ++k)
+    if (int r = sim_step_auto(m)) return r;
   return DEM_OK;
 }
 
@@ -647,16 +716,34 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
   dem_context* ctx = m->ctx;
   if (int r = sim_check(m)) return r;
   if (steps & 1u) {
-    DEM_HIP(ctx, sim_step_on(m, ctx->stream, nullptr), "step");
+    if (int r = sim_step_auto(m)) return r;
     --steps;
   }
   if (steps == 0) return DEM_OK;
   if (!m->exec[0] || m->graph_gen != ctx->generation)
     if (int r = sim_capture(m)) return r;
-  for (uint32_t k = 0; k < steps; k += 2)
-    DEM_HIP(ctx, hipGraphLaunch(m->exec[m->cur], ctx->stream), "hipGraphLaunch");
+  for (uint32_t k = 0; k < steps; k += 2) {
+    const bool resort = sim_use_resort(m);
+    DEM_HIP(ctx, hipGraphLaunch(resort ? m->exec_inc[m->cur] : m->exec[m->cur], ctx->stream), "hipGraphLaunch");
+    m->resort_ok = true;
+  }
   // Each replay runs two steps, which leaves the state in the same buffer;
-  // the sorted key/index buffers are those of the captured steps.
+  // the sorted keys are in skey[cur] and the sorted index in sidx.
+  m->sorted_key = m->skey[m->cur];
+  m->sorted_idx = m->sidx;
+  return DEM_OK;
+}
+
+int dem_sim_set_sort_mode(dem_sim* m, int mode) {
+  if (!m) return DEM_ERR_INVALID;
+  if (mode < 0 || mode > 2) return fail(m->ctx, DEM_ERR_INVALID, "sort mode must be 0, 1 or 2");
+  m->sort_mode = mode;
+  return DEM_OK;
+}
+
+int dem_sim_mover_count(dem_sim* m, uint32_t* count) {
+  if (!m || !count) return m ? fail(m->ctx, DEM_ERR_INVALID, "count is NULL") : DEM_ERR_INVALID;
+  *count = *(volatile const uint32_t*)m->h_count;
   return DEM_OK;
 }
 
@@ -669,7 +756,8 @@ int dem_sim_stage_times(dem_sim* m, uint32_t reps, float* ms) {
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   int rc = DEM_OK;
   for (uint32_t r = 0; r < reps && rc == DEM_OK; ++r) {
-    hipError_t e = sim_step_on(m, ctx->stream, ev);
+    hipError_t e = sim_step_on(m, ctx->stream, ev, sim_use_resort(m));
+    if (e == hipSuccess) m->resort_ok = true;
     if (e == hipSuccess) e = hipEventSynchronize(ev[6]);
     if (e != hipSuccess) {
       rc = hip_fail(ctx, e, "timed step");
@@ -734,6 +822,9 @@ int dem_sim_set_by_id(dem_sim* m, const float* pos_by_id, const float* vel_by_id
   hipLaunchKernelGGL(k_set_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c],
                      (const float2*)pos_by_id, (const float2*)vel_by_id, m->pos[c], m->vel[c], m->n);
   DEM_HIP(ctx, hipGetLastError(), "set by id");
+  // the order still follows skey[cur], but new positions may move many
+  // particles to other cells: the next step sorts from scratch
+  if (pos_by_id) m->resort_ok = false;
   return DEM_OK;
 }
 
@@ -878,6 +969,7 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  m->resort_ok = false;  // the cell arrays now hold the tuning step's ranges
   if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
   ctx->bs_hash = best[0];
   ctx->bs_reorder = best[1];

@@ -40,9 +40,17 @@ struct dem_sim {
   uint32_t *sorted_key = nullptr, *sorted_idx = nullptr;
   uint32_t *cell_start = nullptr, *cell_end = nullptr;
   dem::SortScratch sort;
+  // Re-sort (dem::launch_resort): skey[b] holds the sorted keys of the last
+  // step that wrote state buffer b, sidx the sorted index of the last step.
+  uint32_t *skey[2] = {}, *sidx = nullptr;
+  dem::ResortScratch resort;
+  uint32_t* h_count = nullptr;  // host-mapped mirror of the last step's mover count
+  bool resort_ok = false;       // state in the order of skey[cur]; cell arrays describe skey[cur]
+  int sort_mode = 0;            // 0 auto, 1 radix sort every step, 2 re-sort whenever resort_ok
   int cur = 0;
   hipStream_t cap_stream = nullptr;
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};      // two radix-sorted steps, by starting buffer
+  hipGraphExec_t exec_inc[2] = {nullptr, nullptr};  // two re-sorted steps, by starting buffer
   uint64_t graph_gen = 0;
   std::vector<void*> allocs;
 };
@@ -56,7 +64,12 @@ int ensure_device(dem_context* ctx);
 bool multi(const dem_context* ctx);
 int sim_check(dem_sim* m);
 // One step (src/main.cu:431-507) on stream s; ev (7 events) or NULL.
-hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev);
+// resort: sort by dem::launch_resort (needs m->resort_ok) instead of the
+// radix sort.
+hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort);
+// Sort method for the next step of m (auto mode: re-sort while the last
+// observed mover count is small).
+bool sim_use_resort(const dem_sim* m);
 
 }  // namespace dem
 

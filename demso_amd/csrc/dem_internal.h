@@ -49,6 +49,35 @@ struct StepConst {
 hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float2* pos,
                        const uint32_t* type, uint32_t* hash, uint32_t* index, uint32_t n);
 hipError_t launch_fill(hipStream_t s, uint32_t* p, uint32_t value, uint32_t count);
+
+// ---- re-sort of a state already sorted by its previous keys --------------
+// Particles per tile of the mover bit mask and counts.
+constexpr uint32_t kResortTile = 4096;
+// Hash + movers: key[i] of pos[i]; bit i of `bits` set when key[i] differs
+// from prev_key[i]; tile_count[t] = movers of tile t.
+hipError_t launch_hash_movers(const StepConst& c, hipStream_t s, const float2* pos, const uint32_t* type,
+                              const uint32_t* prev_key, uint32_t* key, uint32_t* bits, uint32_t* tile_count,
+                              uint32_t n);
+struct ResortScratch {
+  uint32_t* bits = nullptr;         // ceil(n/32) mover bits
+  uint32_t* word_prefix = nullptr;  // ceil(n/32) movers before each word
+  uint32_t* tile_count = nullptr;   // tiles
+  uint32_t* tile_off = nullptr;     // tiles
+  uint32_t* mkey[2] = {};           // n each: mover keys (compacted, then sorted)
+  uint32_t* mpos[2] = {};           // n each: mover positions
+  uint32_t* info = nullptr;         // [0] mover count, [1] buffer holding the sorted movers
+  uint32_t* host_count = nullptr;   // device address of a host-mapped mirror of the mover count (or null)
+  uint32_t capacity = 0;
+};
+// Stable sort of `key` (the keys of a state whose order is sorted by
+// prev_key) from the mover bits and counts written by launch_hash_movers:
+// writes the sorted keys and the source index of every sorted slot.  Equal
+// to a stable sort of key by position for any number of movers; its cost
+// grows with the number of movers.  prev_cell_start/prev_cell_end are the
+// cell ranges of prev_key when cells_valid (a faster lookup), else unused.
+hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* prev_key, const uint32_t* prev_cell_start,
+                         const uint32_t* prev_cell_end, uint32_t num_cells, bool cells_valid, uint32_t key_bits,
+                         uint32_t* out_key, uint32_t* out_idx, const ResortScratch& r, uint32_t n);
 hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32_t* cell_end,
                           float2* s_pos, float2* s_vel, float* s_theta, float* s_omega,
                           uint32_t* s_id, uint32_t* s_loc, uint32_t* s_type, float* s_radius,

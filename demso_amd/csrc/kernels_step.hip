@@ -70,18 +70,13 @@ __device__ __forceinline__ double pressure_den(float r) {
 }
 
 // ---------------------------------------------------------------- calcHash --
-// calcHashD (src/particles_kernel.cuh:133-151).
+// Cell key of one particle: calcGridHash (src/particles_kernel.cuh:123-130)
+// with the instance's key base; clamped or wrapped as the flags say.
 template <bool kMulti>
-__global__ __launch_bounds__(256) void k_hash(StepConst c, const float2* __restrict__ pos,
-                                              const uint32_t* __restrict__ type,
-                                              uint32_t* __restrict__ hash,
-                                              uint32_t* __restrict__ index, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float2 p = pos[i];
+__device__ __forceinline__ uint32_t cell_key(const StepConst& c, float2 p, const uint32_t* __restrict__ type,
+                                             uint32_t i) {
   const Instance in =
-      kMulti ? c.inst[clamp_idx(c.mat[clamp_idx(type[i], c.num_types)].inst, c.num_instances)]
-             : c.inst0;
+      kMulti ? c.inst[clamp_idx(c.mat[clamp_idx(type[i], c.num_types)].inst, c.num_instances)] : c.inst0;
   int gx, gy;
   grid_pos(in, p, gx, gy);
   uint32_t cx, cy;
@@ -93,8 +88,55 @@ __global__ __launch_bounds__(256) void k_hash(StepConst c, const float2* __restr
     cx = (uint32_t)gx & (in.grid[0] - 1u);
     cy = (uint32_t)gy & (in.grid[1] - 1u);
   }
-  hash[i] = in.cell_base + cy * in.grid[0] + cx;
+  return in.cell_base + cy * in.grid[0] + cx;
+}
+
+// calcHashD (src/particles_kernel.cuh:133-151).
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_hash(StepConst c, const float2* __restrict__ pos,
+                                              const uint32_t* __restrict__ type,
+                                              uint32_t* __restrict__ hash,
+                                              uint32_t* __restrict__ index, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  hash[i] = cell_key<kMulti>(c, pos[i], type, i);
   index[i] = i;
+}
+
+// The hash of a state that is already in the sorted order of its previous
+// keys prev_key: the new key of every particle, plus the particles whose key
+// changed ("movers") as a bit mask (bit i%32 of word i/32) and their count
+// per tile of kResortTile particles.  Wave w of a block covers 1024
+// consecutive particles of the block's tile, 64 at a time.
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_hash_movers(StepConst c, const float2* __restrict__ pos,
+                                                     const uint32_t* __restrict__ type,
+                                                     const uint32_t* __restrict__ prev_key,
+                                                     uint32_t* __restrict__ key, uint32_t* __restrict__ bits,
+                                                     uint32_t* __restrict__ tile_count, uint32_t n) {
+  __shared__ uint32_t wcount[4];
+  const uint32_t lane = __lane_id(), w = threadIdx.x / kWave;
+  const uint32_t base = blockIdx.x * kResortTile + w * (kResortTile / 4);
+  const uint32_t nwords = (n + 31u) / 32u;
+  uint32_t count = 0;
+  for (uint32_t it = 0; it < kResortTile / 4 / kWave; ++it) {
+    const uint32_t i0 = base + it * kWave, i = i0 + lane;
+    bool moved = false;
+    if (i < n) {
+      const uint32_t k = cell_key<kMulti>(c, pos[i], type, i);
+      key[i] = k;
+      moved = k != prev_key[i];
+    }
+    const uint64_t m = __ballot(moved);
+    count += (uint32_t)__popcll(m);
+    if (lane == 0 && i0 < n) {
+      bits[i0 / 32u] = (uint32_t)m;
+      if (i0 / 32u + 1u < nwords) bits[i0 / 32u + 1u] = (uint32_t)(m >> 32);
+    }
+  }
+  if (lane == 0) wcount[w] = count;
+  __syncthreads();
+  if (threadIdx.x == 0) tile_count[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
 }
 
 // -------------------------------------------------------------- cell fill --
@@ -477,6 +519,18 @@ hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float
     hipLaunchKernelGGL(k_hash<true>, grid_for(n, block), dim3(block), 0, s, c, pos, type, hash, index, n);
   else
     hipLaunchKernelGGL(k_hash<false>, grid_for(n, block), dim3(block), 0, s, c, pos, type, hash, index, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_movers(const StepConst& c, hipStream_t s, const float2* pos, const uint32_t* type,
+                              const uint32_t* prev_key, uint32_t* key, uint32_t* bits, uint32_t* tile_count,
+                              uint32_t n) {
+  if (n == 0) return hipSuccess;
+  const dim3 grid((n + kResortTile - 1u) / kResortTile);
+  if (type && c.num_instances > 1)
+    hipLaunchKernelGGL(k_hash_movers<true>, grid, dim3(256), 0, s, c, pos, type, prev_key, key, bits, tile_count, n);
+  else
+    hipLaunchKernelGGL(k_hash_movers<false>, grid, dim3(256), 0, s, c, pos, type, prev_key, key, bits, tile_count, n);
   return hipGetLastError();
 }
 

@@ -1,0 +1,369 @@
+// resort.hip -- stable re-sort of cell keys for a state that is already in
+// the sorted order of its previous keys (the sortParticles stage,
+// src/functions.cu:336-341, of every step after the first).
+//
+// After a step the particles sit in the order of the previous step's sorted
+// keys prev_key (reorder wrote them so and integrate works in place), and
+// only the few particles that crossed a cell boundary ("movers") have a new
+// key that differs from prev_key.  The stable sort of the new keys by
+// position is the ascending order of the pairs (key[i], i).  The stayers
+// keep key = prev_key, which is non-decreasing in i, so they already appear
+// in that order; the result is the merge of the stayers with the movers
+// sorted by (key, position):
+//   stayer i -> slot (stayers before i) + (movers whose (key, pos) < (key[i], i))
+//   mover r  -> slot r + (stayers q whose (prev_key[q], q) < (key, pos))
+// where r is the mover's rank among the sorted movers.  Every slot is
+// written exactly once, and the result equals the full radix sort for any
+// number of movers; the cost grows with the number of movers.
+//
+// Launches (after k_hash_movers, which replaces the plain hash):
+//   compact : per tile, movers before each 32-particle word and the
+//             compacted (key, position) list of the movers;
+//   movers  : one workgroup sorts the movers: a bitonic sort of the unique
+//             64-bit (key << 32 | position) values in LDS up to 4096 movers,
+//             a stable chunked LSD radix sort through global memory beyond;
+//   place   : the two slot formulas above.  A stayer's mover count is a
+//             binary search in the window of sorted movers that falls
+//             between its tile's first and last keys (staged in LDS); a
+//             mover's stayer count comes from the previous step's cell
+//             ranges (cellStart/cellEnd of prev_key) or a binary search
+//             over prev_key.
+#include "dem_internal.h"
+
+namespace dem {
+namespace {
+
+constexpr uint32_t kWordsPerTile = kResortTile / 32;  // 128
+constexpr int kSortThreads = 1024;
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr uint32_t kBitonicMax = 4096;
+constexpr int kPlaceThreads = 256;
+constexpr uint32_t kMoverBlocks = 32;
+constexpr uint32_t kWindowMax = 2048;
+constexpr int kDigitBits = 8;
+constexpr int kDigits = 1 << kDigitBits;
+
+__device__ __forceinline__ uint64_t composite(uint32_t key, uint32_t pos) { return ((uint64_t)key << 32) | pos; }
+
+// Movers among positions [0, p).
+__device__ __forceinline__ uint32_t movers_before(uint32_t p, uint32_t n, uint32_t m, const uint32_t* __restrict__ bits,
+                                                  const uint32_t* __restrict__ word_prefix) {
+  if (p >= n) return m;
+  const uint32_t w = p >> 5, b = p & 31u;
+  return word_prefix[w] + (uint32_t)__popc(bits[w] & ((1u << b) - 1u));
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, kWave);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
+// Lanes of this wave (within `valid`) whose digit equals this lane's digit.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
+  uint64_t peers = valid;
+#pragma unroll
+  for (int b = 0; b < kDigitBits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// Per tile: the movers before each word (word_prefix) and the movers of the
+// tile appended to (mkey, mpos) in position order.  128 threads, one word
+// each; the tile's offset is the sum of the counts of the tiles before it.
+__global__ __launch_bounds__(kWordsPerTile) void k_resort_compact(
+    const uint32_t* __restrict__ key, const uint32_t* __restrict__ bits, const uint32_t* __restrict__ tile_count,
+    uint32_t* __restrict__ word_prefix, uint32_t* __restrict__ mkey, uint32_t* __restrict__ mpos, uint32_t n) {
+  __shared__ uint32_t red[2];
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave, tile = blockIdx.x;
+  uint32_t before = 0;
+  for (uint32_t t = tid; t < tile; t += kWordsPerTile) before += tile_count[t];
+  before = wave_sum(before);
+  if (lane == 0) red[w] = before;
+  const uint32_t nwords = (n + 31u) / 32u;
+  const uint32_t word = tile * kWordsPerTile + tid;
+  const uint32_t b = word < nwords ? bits[word] : 0u;
+  const uint32_t c = (uint32_t)__popc(b);
+  const uint32_t inc = wave_inclusive_scan(c);
+  __shared__ uint32_t wtot[2];
+  if (lane == kWave - 1) wtot[w] = inc;
+  __syncthreads();
+  const uint32_t tile_off = red[0] + red[1];
+  uint32_t r = tile_off + inc - c + (w == 1 ? wtot[0] : 0u);
+  if (word < nwords) word_prefix[word] = r;
+  for (uint32_t v = b; v; v &= v - 1u) {
+    const uint32_t p = word * 32u + (uint32_t)(__ffs(v) - 1);
+    mkey[r] = key[p];
+    mpos[r] = p;
+    ++r;
+  }
+}
+
+// Exclusive scan of one value per thread over a 1024-thread block.
+__device__ __forceinline__ uint32_t block_scan_1024(uint32_t x, uint32_t* wsum) {
+  const uint32_t lane = __lane_id(), w = threadIdx.x / kWave;
+  const uint32_t inc = wave_inclusive_scan(x);
+  if (lane == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int k = 0; k < (int)w; ++k) off += wsum[k];
+  __syncthreads();
+  return off + inc - x;
+}
+
+// One workgroup: the mover count (sum of the tile counts) and the stable
+// sort of (mkey[0], mpos[0]) by key.  info[0] = count, info[1] = the buffer
+// (0 or 1) that holds the sorted movers.
+__global__ __launch_bounds__(kSortThreads) void k_resort_movers(const uint32_t* __restrict__ tile_count, uint32_t tiles,
+                                                                uint32_t* mkey0, uint32_t* mpos0, uint32_t* mkey1,
+                                                                uint32_t* mpos1, uint32_t key_bits,
+                                                                uint32_t* __restrict__ info,
+                                                                uint32_t* __restrict__ host_count) {
+  __shared__ uint64_t v[kBitonicMax];
+  __shared__ uint32_t wsum[kSortWaves];
+  __shared__ uint32_t wcnt[kSortWaves][kDigits];
+  __shared__ uint32_t base[kDigits];
+  __shared__ uint32_t ctot[kDigits];
+  __shared__ uint32_t s_m;
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave;
+
+  uint32_t part = 0;
+  for (uint32_t t = tid; t < tiles; t += kSortThreads) part += tile_count[t];
+  part = wave_sum(part);
+  if (lane == 0) wsum[w] = part;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t m = 0;
+    for (int k = 0; k < kSortWaves; ++k) m += wsum[k];
+    s_m = m;
+    info[0] = m;
+    if (host_count) *host_count = m;
+  }
+  __syncthreads();
+  const uint32_t m = s_m;
+  if (m <= 1u) {
+    if (tid == 0) info[1] = 0;
+    return;
+  }
+
+  if (m <= kBitonicMax) {
+    uint32_t P = 1;
+    while (P < m) P <<= 1;
+    for (uint32_t i = tid; i < P; i += kSortThreads) v[i] = i < m ? composite(mkey0[i], mpos0[i]) : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = tid; i < P; i += kSortThreads) {
+          const uint32_t l = i ^ j;
+          if (l > i) {
+            const uint64_t a = v[i], b = v[l];
+            const bool up = (i & k) == 0;
+            if ((a > b) == up) {
+              v[i] = b;
+              v[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t i = tid; i < m; i += kSortThreads) {
+      mkey0[i] = (uint32_t)(v[i] >> 32);
+      mpos0[i] = (uint32_t)v[i];
+    }
+    if (tid == 0) info[1] = 0;
+    return;
+  }
+
+  // stable LSD radix sort, one digit per pass, chunks of 1024 in order
+  uint32_t* kb[2] = {mkey0, mkey1};
+  uint32_t* pb[2] = {mpos0, mpos1};
+  const uint32_t passes = (key_bits + kDigitBits - 1) / kDigitBits;
+  uint32_t src = 0;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (uint32_t pass = 0; pass < passes; ++pass) {
+    const uint32_t shift = pass * kDigitBits;
+    if (tid < (uint32_t)kDigits) base[tid] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < m; i += kSortThreads) atomicAdd(&base[(kb[src][i] >> shift) & (kDigits - 1)], 1u);
+    __syncthreads();
+    {
+      const uint32_t x = tid < (uint32_t)kDigits ? base[tid] : 0u;
+      const uint32_t ex = block_scan_1024(x, wsum);
+      if (tid < (uint32_t)kDigits) base[tid] = ex;
+      __syncthreads();
+    }
+    for (uint32_t c0 = 0; c0 < m; c0 += kSortThreads) {
+      for (uint32_t k = tid; k < kSortWaves * kDigits; k += kSortThreads) (&wcnt[0][0])[k] = 0;
+      __syncthreads();
+      const uint32_t i = c0 + tid;
+      const bool ok = i < m;
+      const uint32_t key = ok ? kb[src][i] : 0u, pos = ok ? pb[src][i] : 0u;
+      const uint32_t d = (key >> shift) & (kDigits - 1);
+      const uint64_t peers = match_digit(d, __ballot(ok));
+      const uint32_t rank = (uint32_t)__popcll(peers & lt);
+      if (ok && lane == (uint32_t)(__ffsll((unsigned long long)peers) - 1)) wcnt[w][d] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      if (tid < (uint32_t)kDigits) {
+        uint32_t run = 0;
+        for (int k = 0; k < kSortWaves; ++k) {
+          const uint32_t c = wcnt[k][tid];
+          wcnt[k][tid] = run;
+          run += c;
+        }
+        ctot[tid] = run;  // this chunk's count of digit tid
+      }
+      __syncthreads();
+      if (ok) {
+        const uint32_t slot = base[d] + wcnt[w][d] + rank;
+        if (slot < m) {  // always true for a consistent ranking; guards the store
+          kb[src ^ 1u][slot] = key;
+          pb[src ^ 1u][slot] = pos;
+        }
+      }
+      __syncthreads();
+      if (tid < (uint32_t)kDigits) base[tid] += ctot[tid];
+      __syncthreads();
+    }
+    src ^= 1u;
+  }
+  if (tid == 0) info[1] = src;
+}
+
+// Count of sorted movers whose (key, pos) is below x, in [lo, hi).
+__device__ __forceinline__ uint32_t movers_below(const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sp,
+                                                 uint32_t lo, uint32_t hi, uint64_t x) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (composite(sk[mid], sp[mid]) < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
+    const uint32_t* __restrict__ key, const uint32_t* __restrict__ prev_key, const uint32_t* __restrict__ bits,
+    const uint32_t* __restrict__ word_prefix, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
+    const uint32_t* mpos1, const uint32_t* __restrict__ info, const uint32_t* __restrict__ prev_cell_start,
+    const uint32_t* __restrict__ prev_cell_end, uint32_t num_cells, int cells_valid, uint32_t* __restrict__ out_key,
+    uint32_t* __restrict__ out_idx, uint32_t n) {
+  __shared__ uint64_t win[kWindowMax];
+  __shared__ uint32_t s_lo, s_hi;
+  const uint32_t m = info[0];
+  const uint32_t* sk = info[1] ? mkey1 : mkey0;
+  const uint32_t* sp = info[1] ? mpos1 : mpos0;
+  const uint32_t tid = threadIdx.x;
+
+  if (blockIdx.x < kMoverBlocks) {
+    // movers: slot = rank + stayers before their (key, pos)
+    for (uint32_t r = blockIdx.x * kPlaceThreads + tid; r < m; r += kMoverBlocks * kPlaceThreads) {
+      const uint32_t k = sk[r], p = sp[r];
+      uint32_t P = 0;
+      bool found = false;
+      if (cells_valid && k < num_cells) {
+        const uint32_t s0 = prev_cell_start[k];
+        if (s0 != kEmpty) {
+          const uint32_t e0 = prev_cell_end[k];
+          P = p < s0 ? s0 : (p >= e0 ? e0 : p);
+          found = true;
+        } else {
+          for (uint32_t kk = k + 1u; kk < num_cells && kk <= k + 8u; ++kk) {
+            const uint32_t s1 = prev_cell_start[kk];
+            if (s1 != kEmpty) {
+              P = s1;
+              found = true;
+              break;
+            }
+          }
+        }
+      }
+      if (!found) {
+        uint32_t lo = 0, hi = n;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          const uint32_t q = prev_key[mid];
+          if (q < k || (q == k && mid < p)) lo = mid + 1;
+          else hi = mid;
+        }
+        P = lo;
+      }
+      const uint32_t slot = r + P - movers_before(P, n, m, bits, word_prefix);
+      if (slot < n) {  // always true for a consistent input; guards the store
+        out_key[slot] = k;
+        out_idx[slot] = p;
+      }
+    }
+    return;
+  }
+
+  // stayers of one tile
+  const uint32_t tile = blockIdx.x - kMoverBlocks;
+  const uint32_t first = tile * kResortTile;
+  if (first >= n) return;
+  const uint32_t last = min(first + kResortTile, n) - 1u;
+  if (tid == 0) s_lo = m ? movers_below(sk, sp, 0, m, composite(prev_key[first], first)) : 0u;
+  if (tid == 1) s_hi = m ? movers_below(sk, sp, 0, m, composite(prev_key[last], last)) : 0u;
+  __syncthreads();
+  const uint32_t lo = s_lo, hi = s_hi;
+  const bool staged = hi - lo <= kWindowMax;
+  if (staged)
+    for (uint32_t j = tid; j < hi - lo; j += kPlaceThreads) win[j] = composite(sk[lo + j], sp[lo + j]);
+  __syncthreads();
+  for (uint32_t i = first + tid; i <= last; i += kPlaceThreads) {
+    if ((bits[i >> 5] >> (i & 31u)) & 1u) continue;
+    const uint32_t k = key[i];
+    const uint64_t x = composite(k, i);
+    uint32_t below;
+    if (staged) {
+      uint32_t a = 0, b = hi - lo;
+      while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        if (win[mid] < x) a = mid + 1;
+        else b = mid;
+      }
+      below = lo + a;
+    } else {
+      below = movers_below(sk, sp, lo, hi, x);
+    }
+    const uint32_t slot = i - movers_before(i, n, m, bits, word_prefix) + below;
+    if (slot < n) {  // always true for a consistent input; guards the store
+      out_key[slot] = k;
+      out_idx[slot] = i;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* prev_key, const uint32_t* prev_cell_start,
+                         const uint32_t* prev_cell_end, uint32_t num_cells, bool cells_valid, uint32_t key_bits,
+                         uint32_t* out_key, uint32_t* out_idx, const ResortScratch& r, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  if (n > r.capacity || !r.bits || !r.word_prefix || !r.tile_count || !r.mkey[0] || !r.mkey[1] || !r.mpos[0] ||
+      !r.mpos[1] || !r.info || key_bits > 32)
+    return hipErrorInvalidValue;
+  const uint32_t tiles = (n + kResortTile - 1u) / kResortTile;
+  hipLaunchKernelGGL(k_resort_compact, dim3(tiles), dim3(kWordsPerTile), 0, s, key, r.bits, r.tile_count,
+                     r.word_prefix, r.mkey[0], r.mpos[0], n);
+  hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.mkey[0], r.mpos[0],
+                     r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count);
+  hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + tiles), dim3(kPlaceThreads), 0, s, key, prev_key, r.bits,
+                     r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, prev_cell_start,
+                     prev_cell_end, num_cells, cells_valid ? 1 : 0, out_key, out_idx, n);
+  return hipGetLastError();
+}
+
+}  // namespace dem

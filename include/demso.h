@@ -219,6 +219,18 @@ int dem_sim_stage_times(dem_sim* sim, uint32_t reps, float* ms);
  * (the state is restored afterwards).  Writes the choice to block[5]
  * (hash, reorder, collide, integrate, sort items-per-thread) if non-NULL. */
 int dem_sim_autotune(dem_sim* sim, uint32_t* block);
+/* Sort method of the sim's steps.  After a step the state is in the sorted
+ * order of its keys, so the next step can re-sort: only particles whose cell
+ * changed are sorted and merged with the rest (same result as the radix
+ * sort, bit for bit).  mode 0 (default): re-sort while the last step's count
+ * of such particles stays small, else radix sort; 1: radix sort every step;
+ * 2: re-sort whenever the state allows it.  The first step after
+ * dem_sim_upload, dem_sim_set_by_id (positions), dem_sim_autotune or
+ * dem_adj_backward always radix sorts. */
+int dem_sim_set_sort_mode(dem_sim* sim, int mode);
+/* Particles whose cell changed in the last re-sorted step (exact once the
+ * stream is synchronized). */
+int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);
 /* Current positions / velocities gathered by particle ID into device arrays

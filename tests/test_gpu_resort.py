@@ -1,0 +1,99 @@
+"""The re-sort of a state already in the order of its previous keys
+(only the particles that changed cell are sorted and merged back) against
+the radix sort and the oracle: identical keys, permutation, cell ranges and
+state, bit for bit, for few movers (LDS bitonic path), many movers (the
+chunked radix path of the movers workgroup) and ensembles."""
+import numpy as np
+import pytest
+
+from demso_amd import api
+from demso_amd import scene as sc
+from helpers import bits_equal, compressed_pack
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("hash", "index", "cellStart", "id", "type", "loc", "pos", "vel", "theta", "omega", "acc", "alpha",
+          "pressure")
+
+
+def same(a, b):
+    occ = a["cellStart"] != 0xFFFFFFFF
+    assert np.array_equal(a["cellEnd"][occ], b["cellEnd"][occ])
+    for k in FIELDS:
+        assert bits_equal(a[k], b[k]), k
+
+
+def fast_column(n_side=128, speed=30.0, seed=5, dt=1e-4):
+    """A sparse lattice (3 cm pitch, 1.1 cm cells) of soft discs with random
+    velocities of tens of m/s: with dt = 1e-4 s about half of them change
+    cell every step."""
+    s = sc.column_collapse(seed=seed, cols=n_side, rows=n_side, box=(n_side / 16.0, n_side / 32.0), pitch=0.03,
+                           dt=dt, kn=1e4, kt=2e3)
+    rng = np.random.default_rng(seed)
+    s.vel[:] = (rng.normal(size=s.vel.shape) * speed).astype(np.float32)
+    return s
+
+
+CASES = {
+    "pack_few_movers": (lambda: compressed_pack(seed=3), 30),
+    "fast_many_movers": (lambda: fast_column(), 12),
+    "ensemble_7": (lambda: sc.ensemble(instances=7, counts=[3000, 700, 5200, 90, 1640, 4100, 2222], seed=6), 25),
+    "pile_wrap": (lambda: sc.triangle_pile(), 40),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("graph", [False, True])
+def test_resort_equals_radix(gpu_ctx, name, graph):
+    make, steps = CASES[name]
+    s = make()
+    a = api.Simulation(gpu_ctx, s)
+    b = api.Simulation(gpu_ctx, s)
+    a.set_sort_mode(1)  # radix sort every step
+    b.set_sort_mode(2)  # re-sort whenever possible
+    movers = []
+    for chunk in (1, 2, steps):
+        for sim in (a, b):
+            (sim.step_graph if graph else sim.step)(chunk)
+        movers.append(b.mover_count())
+        print(f"{name} graph={graph}: movers after {chunk} more steps: {movers[-1]}", flush=True)
+        same(a.download(), b.download())
+    a.close()
+    b.close()
+    if name == "fast_many_movers":
+        assert max(movers) > 4096  # the chunked radix path of the movers workgroup ran
+
+
+def test_mover_count_is_cell_changes(gpu_ctx):
+    """The mover count of a re-sorted step is the number of particles whose
+    cell key differs from the previous step's."""
+    s = fast_column(n_side=64, speed=3.0)
+    sim = api.Simulation(gpu_ctx, s)
+    sim.set_sort_mode(2)
+    sim.step(2)
+    for _ in range(3):
+        d0 = sim.download()
+        sim.step(1)
+        d1 = sim.download()
+        k0 = np.empty(s.n, np.uint32)
+        k1 = np.empty(s.n, np.uint32)
+        k0[d0["id"]] = d0["hash"]
+        k1[d1["id"]] = d1["hash"]
+        changed = int((k0 != k1).sum())
+        assert sim.mover_count() == changed
+        assert changed > 0
+    sim.close()
+
+
+def test_resort_matches_oracle_long(gpu_ctx, oracle):
+    """Default (automatic) sort choice over 200 steps of a column collapse
+    against the oracle's stable sort."""
+    s = sc.column_collapse(cols=48, rows=48)
+    sim = api.Simulation(gpu_ctx, s)
+    ref = oracle.Sim(s, "f32")
+    sim.step_graph(200)
+    ref.step(200)
+    g, o = sim.download(), ref.state()
+    for k in ("hash", "index", "cellStart", "id", "pos", "vel", "acc", "pressure"):
+        assert bits_equal(g[k], o[k]), k
+    sim.close()
