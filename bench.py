@@ -34,7 +34,9 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 # Algorithmic (compulsory) HBM bytes per particle and per cell for each
 # stage, FP32 state with per-particle radius and 32-bit keys:
 #   hash      R pos 8;                         W key 4, index 4
-#   sort      R key+index 8;                   W key+index 8        (one pass)
+#             (hash + movers: R pos 8, previous key 4; W key 4)
+#   sort      re-sort: R key 4;                W sorted key 4, index 4
+#             (a radix pass would be R key+index 8, W key+index 8)
 #   fill      W cellStart 4 per cell
 #   reorder   R key 4, index 4, state 36;      W state 36, loc 4
 #             (state = pos 8, vel 8, theta 4, omega 4, id 4, type 4, radius 4)
@@ -42,7 +44,7 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 #             + R cellStart/cellEnd 8 per cell
 #   integrate R pos, vel, acc 24, theta, omega, alpha, type, radius, pressure 24;
 #             W pos, vel 16, theta, omega 8
-BYTES_PER_PARTICLE = {"hash": 16, "sort": 16, "cell_fill": 0, "reorder": 84, "collide": 44, "integrate": 72}
+BYTES_PER_PARTICLE = {"hash": 16, "sort": 12, "cell_fill": 0, "reorder": 84, "collide": 44, "integrate": 72}
 BYTES_PER_CELL = {"hash": 0, "sort": 0, "cell_fill": 4, "reorder": 0, "collide": 8, "integrate": 0}
 
 
