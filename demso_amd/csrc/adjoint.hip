@@ -140,12 +140,16 @@ __global__ void k_adj_out(const uint32_t* __restrict__ id, const double2* __rest
 }
 
 // sorted-state snapshot of one step: pos, vel, omega, radius, type, index
+// One step's slot: the sorted pre-integrate state, the step's sorted keys
+// and permutation, and collide's accelerations.
 __global__ void k_store_slot(const float2* __restrict__ pos, const float2* __restrict__ vel,
                              const float* __restrict__ omega, const float* __restrict__ radius,
                              const uint32_t* __restrict__ type, const uint32_t* __restrict__ index,
-                             float2* __restrict__ spos, float2* __restrict__ svel, float* __restrict__ somega,
-                             float* __restrict__ sradius, uint32_t* __restrict__ stype, uint32_t* __restrict__ sindex,
-                             uint32_t n) {
+                             const uint32_t* __restrict__ key, const float2* __restrict__ acc,
+                             const float* __restrict__ alpha, float2* __restrict__ spos, float2* __restrict__ svel,
+                             float* __restrict__ somega, float* __restrict__ sradius, uint32_t* __restrict__ stype,
+                             uint32_t* __restrict__ sindex, uint32_t* __restrict__ skey, float2* __restrict__ sacc,
+                             float* __restrict__ salpha, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   spos[i] = pos[i];
@@ -154,6 +158,9 @@ __global__ void k_store_slot(const float2* __restrict__ pos, const float2* __res
   if (radius) sradius[i] = radius[i];
   stype[i] = type[i];
   sindex[i] = index[i];
+  skey[i] = key[i];
+  sacc[i] = acc[i];
+  salpha[i] = alpha[i];
 }
 
 // cellStart/cellEnd of sorted keys (cellStart pre-filled with EMPTY)
@@ -493,9 +500,10 @@ struct dem_adjoint {
   std::vector<AdjCheckpoint> ckpt;
   AdjCheckpoint final_state;
   // segment slots: K x n
-  float2 *spos = nullptr, *svel = nullptr;
-  float *somega = nullptr, *sradius = nullptr;
-  uint32_t *stype = nullptr, *sindex = nullptr;
+  float2 *spos = nullptr, *svel = nullptr, *sacc = nullptr;
+  float *somega = nullptr, *sradius = nullptr, *salpha = nullptr;
+  uint32_t *stype = nullptr, *sindex = nullptr, *skey = nullptr;
+  size_t slot_off = 0;  // the slot the step hook fills
   // adjoint state (FP64)
   double2 *lx = nullptr, *lv = nullptr, *lxp = nullptr, *lvp = nullptr, *lF = nullptr;
   double *lw = nullptr, *lwp = nullptr, *lM = nullptr, *grad = nullptr;
@@ -544,6 +552,17 @@ hipError_t copy_state(dem_sim* m, AdjCheckpoint& c, bool to_checkpoint, hipStrea
   return hipSuccess;
 }
 
+hipError_t store_slot_hook(void* user, dem_sim* m, hipStream_t s, int t) {
+  auto* a = static_cast<dem_adjoint*>(user);
+  const size_t off = a->slot_off;
+  const uint32_t n = m->n;
+  hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[t], m->vel[t], m->omega[t],
+                     m->rad ? m->radius[t] : nullptr, m->type[t], m->sorted_idx, m->sorted_key, m->acc, m->alpha,
+                     a->spos + off, a->svel + off, a->somega + off, a->sradius ? a->sradius + off : nullptr,
+                     a->stype + off, a->sindex + off, a->skey + off, a->sacc + off, a->salpha + off, n);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 extern "C" {
@@ -567,6 +586,9 @@ int dem_adj_create(dem_sim* sim, uint32_t checkpoint_every, dem_adjoint** out) {
   if (!r && sim->rad) r = adj_alloc(a, &a->sradius, slots);
   if (!r) r = adj_alloc(a, &a->stype, slots);
   if (!r) r = adj_alloc(a, &a->sindex, slots);
+  if (!r) r = adj_alloc(a, &a->skey, slots);
+  if (!r) r = adj_alloc(a, &a->sacc, slots);
+  if (!r) r = adj_alloc(a, &a->salpha, slots);
   if (!r) r = adj_alloc(a, &a->lx, n);
   if (!r) r = adj_alloc(a, &a->lv, n);
   if (!r) r = adj_alloc(a, &a->lw, n);
@@ -637,43 +659,27 @@ int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_ve
   const uint32_t nseg = (a->steps + a->K - 1) / a->K;
   for (int seg = (int)nseg - 1; seg >= 0; --seg) {
     const uint32_t k0 = (uint32_t)seg * a->K, len = std::min(a->K, a->steps - k0);
-    // re-run the segment, storing the sorted pre-collide state of each step
+    // re-run the segment with the owned step (re-sorted after its first
+    // step), storing each step's sorted pre-integrate state, keys,
+    // permutation and accelerations
     DEM_HIP(ctx, copy_state(m, a->ckpt[seg], false, s), "restore checkpoint");
+    m->resort_ok = false;
+    StepHook hook;
+    hook.fn = store_slot_hook;
+    hook.user = a;
     for (uint32_t k = 0; k < len; ++k) {
-      const int o = m->cur, t = o ^ 1;
-      DEM_HIP(ctx, launch_hash(c, s, ctx->bs_hash, m->pos[o], mul ? m->type[o] : nullptr, m->key, m->idx, n), "hash");
-      bool in_alt = false;
-      DEM_HIP(ctx, launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt), "sort");
-      uint32_t* sk = in_alt ? m->sort.keys_alt : m->key;
-      uint32_t* si = in_alt ? m->sort.vals_alt : m->idx;
-      DEM_HIP(ctx, launch_fill(s, m->cell_start, kEmpty, m->cells), "fill");
-      DEM_HIP(ctx, launch_reorder(s, ctx->bs_reorder, m->cell_start, m->cell_end, m->pos[t], m->vel[t], m->theta[t],
-                                  m->omega[t], m->id[t], m->loc[t], m->type[t], m->rad ? m->radius[t] : nullptr, sk, si,
-                                  m->pos[o], m->vel[o], m->theta[o], m->omega[o], m->id[o], m->type[o],
-                                  m->rad ? m->radius[o] : nullptr, n, m->cells),
-              "reorder");
-      const size_t off = (size_t)k * n;
-      hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[t], m->vel[t], m->omega[t],
-                         m->rad ? m->radius[t] : nullptr, m->type[t], si, a->spos + off, a->svel + off,
-                         a->somega + off, a->sradius ? a->sradius + off : nullptr, a->stype + off, a->sindex + off, n);
-      DEM_HIP(ctx, launch_collide(c, s, ctx->bs_collide, m->pos[t], m->vel[t], m->acc, m->omega[t], m->alpha,
-                                  m->type[t], m->rad ? m->radius[t] : nullptr, m->cell_start, m->cell_end, m->pressure, n),
-              "collide");
-      DEM_HIP(ctx, launch_integrate(c, s, ctx->bs_integrate, m->pos[t], m->vel[t], m->acc, m->theta[t], m->omega[t],
-                                    m->alpha, m->type[t], m->rad ? m->radius[t] : nullptr, m->pressure, n),
-              "integrate");
-      m->cur = t;
+      a->slot_off = (size_t)k * n;
+      DEM_HIP(ctx, sim_step_on(m, s, nullptr, sim_use_resort(m), &hook), "segment step");
+      m->resort_ok = true;
     }
-    // reverse through the segment
+    // reverse through the segment: cell ranges from the stored keys, forces
+    // from the stored accelerations
     for (int k = (int)len - 1; k >= 0; --k) {
       const size_t off = (size_t)k * n;
       AdjIn S{a->spos + off, a->svel + off, a->somega + off, a->sradius ? a->sradius + off : nullptr, a->stype + off};
-      DEM_HIP(ctx, launch_hash(c, s, ctx->bs_hash, S.pos, mul ? S.type : nullptr, m->key, m->idx, n), "adj hash");
       DEM_HIP(ctx, launch_fill(s, m->cell_start, kEmpty, m->cells), "adj fill");
-      hipLaunchKernelGGL(k_cell_bounds, grid_of(n), dim3(256), 0, s, m->key, m->cell_start, m->cell_end, n, m->cells);
-      DEM_HIP(ctx, launch_collide(c, s, ctx->bs_collide, S.pos, S.vel, m->acc, S.omega, m->alpha, S.type, S.radius,
-                                  m->cell_start, m->cell_end, m->pressure, n),
-              "adj collide forward");
+      hipLaunchKernelGGL(k_cell_bounds, grid_of(n), dim3(256), 0, s, a->skey + off, m->cell_start, m->cell_end, n,
+                         m->cells);
 #define DEM_ADJ(K, ...)                                                                                 \
   do {                                                                                                  \
     if (m->rad) {                                                                                       \
@@ -684,7 +690,8 @@ int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_ve
       else hipLaunchKernelGGL((K<false, false>), grid_of(n), dim3(256), 0, s, __VA_ARGS__);              \
     }                                                                                                   \
   } while (0)
-      DEM_ADJ(k_adj_integrate, c, S, (const float2*)m->acc, (const float*)m->alpha, (const double2*)a->lx,
+      DEM_ADJ(k_adj_integrate, c, S, (const float2*)(a->sacc + off), (const float*)(a->salpha + off),
+              (const double2*)a->lx,
               (const double2*)a->lv, (const double*)a->lw, a->lxp, a->lvp, a->lwp, a->lF, a->lM, a->grad, n);
       DEM_ADJ(k_adj_collide, c, S, (const uint32_t*)m->cell_start, (const uint32_t*)m->cell_end,
               (const double2*)a->lF, (const double*)a->lM, a->lxp, a->lvp, a->lwp, a->grad, n);

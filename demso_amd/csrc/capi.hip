@@ -457,7 +457,7 @@ void sim_free(dem_sim* m) {
 // events or NULL) brackets each stage for timing.  The sorted keys go to
 // skey[t] and the sorted index to sidx with either sort, so the next step
 // can re-sort from them.
-hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort) {
+hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, const StepHook* hook) {
   dem_context* ctx = m->ctx;
   const StepConst& c = ctx->c;
   const int o = m->cur, t = o ^ 1;
@@ -466,15 +466,18 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort) {
   hipError_t e;
   if (ev) (void)hipEventRecord(ev[0], s);
   if (resort) {
-    e = dem::launch_hash_movers(c, s, m->pos[o], type_for_hash, m->skey[o], m->key, m->resort.bits,
-                                m->resort.tile_count, n);
-    if (e != hipSuccess) return e;
+    // the keys, mover bits and counts were written by the previous step's
+    // integrate (the hash stage is fused into it)
     if (ev) (void)hipEventRecord(ev[1], s);
     e = dem::launch_resort(s, m->key, m->skey[o], m->cell_start, m->cell_end, m->cells, true, ctx->key_bits,
                            m->skey[t], m->sidx, m->resort, n);
     if (e != hipSuccess) return e;
   } else {
     e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], type_for_hash, m->key, m->idx, n);
+    if (e != hipSuccess) return e;
+    // this step's integrate accumulates the next step's mover counts
+    const uint32_t tiles = (n + dem::kResortTile - 1u) / dem::kResortTile;
+    e = dem::launch_fill(s, m->resort.tile_count, 0u, tiles);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
     bool in_alt = false;
@@ -501,9 +504,18 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort) {
   e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[t], m->vel[t], m->acc, m->omega[t], m->alpha, m->type[t],
                           m->rad ? m->radius[t] : nullptr, m->cell_start, m->cell_end, m->pressure, n);
   if (e != hipSuccess) return e;
+  if (hook && hook->fn) {
+    e = hook->fn(hook->user, m, s, t);
+    if (e != hipSuccess) return e;
+  }
   if (ev) (void)hipEventRecord(ev[5], s);
+  dem::NextKeys nk;
+  nk.prev_key = m->skey[t];
+  nk.key = m->key;
+  nk.bits = m->resort.bits;
+  nk.tile_count = m->resort.tile_count;
   e = dem::launch_integrate(c, s, ctx->bs_integrate, m->pos[t], m->vel[t], m->acc, m->theta[t], m->omega[t],
-                            m->alpha, m->type[t], m->rad ? m->radius[t] : nullptr, m->pressure, n);
+                            m->alpha, m->type[t], m->rad ? m->radius[t] : nullptr, m->pressure, n, &nk);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[6], s);
   m->cur = t;

@@ -66,7 +66,13 @@ int sim_check(dem_sim* m);
 // One step (src/main.cu:431-507) on stream s; ev (7 events) or NULL.
 // resort: sort by dem::launch_resort (needs m->resort_ok) instead of the
 // radix sort.
-hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort);
+// hook: launched after collide (before integrate) with the step's target
+// buffer t, e.g. to store the sorted pre-integrate state (the adjoint).
+struct StepHook {
+  hipError_t (*fn)(void* user, dem_sim* m, hipStream_t s, int t) = nullptr;
+  void* user = nullptr;
+};
+hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, const StepHook* hook = nullptr);
 // Sort method for the next step of m (auto mode: re-sort while the last
 // observed mover count is small).
 bool sim_use_resort(const dem_sim* m);

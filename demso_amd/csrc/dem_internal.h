@@ -53,15 +53,10 @@ hipError_t launch_fill(hipStream_t s, uint32_t* p, uint32_t value, uint32_t coun
 // ---- re-sort of a state already sorted by its previous keys --------------
 // Particles per tile of the mover bit mask and counts.
 constexpr uint32_t kResortTile = 4096;
-// Hash + movers: key[i] of pos[i]; bit i of `bits` set when key[i] differs
-// from prev_key[i]; tile_count[t] = movers of tile t.
-hipError_t launch_hash_movers(const StepConst& c, hipStream_t s, const float2* pos, const uint32_t* type,
-                              const uint32_t* prev_key, uint32_t* key, uint32_t* bits, uint32_t* tile_count,
-                              uint32_t n);
 struct ResortScratch {
   uint32_t* bits = nullptr;         // ceil(n/32) mover bits
   uint32_t* word_prefix = nullptr;  // ceil(n/32) movers before each word
-  uint32_t* tile_count = nullptr;   // tiles
+  uint32_t* tile_count = nullptr;   // tiles (accumulated by integrate, cleared by the movers kernel)
   uint32_t* tile_off = nullptr;     // tiles
   uint32_t* mkey[2] = {};           // n each: mover keys (compacted, then sorted)
   uint32_t* mpos[2] = {};           // n each: mover positions
@@ -70,7 +65,7 @@ struct ResortScratch {
   uint32_t capacity = 0;
 };
 // Stable sort of `key` (the keys of a state whose order is sorted by
-// prev_key) from the mover bits and counts written by launch_hash_movers:
+// prev_key) from the mover bits and counts written with it (NextKeys):
 // writes the sorted keys and the source index of every sorted slot.  Equal
 // to a stable sort of key by position for any number of movers; its cost
 // grows with the number of movers.  prev_cell_start/prev_cell_end are the
@@ -89,9 +84,20 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
                           const uint32_t* cell_end, float* pressure, uint32_t n);
+// The next step's hash, fused into integrate: key[i] = cell key of the new
+// position, bit i of `bits` set when it differs from prev_key[i] (the keys
+// the state is sorted by), tile_count[i / kResortTile] += movers (the
+// counts must be zero before).
+struct NextKeys {
+  const uint32_t* prev_key = nullptr;
+  uint32_t* key = nullptr;
+  uint32_t* bits = nullptr;
+  uint32_t* tile_count = nullptr;
+};
 hipError_t launch_integrate(const StepConst& c, hipStream_t s, int block, float2* pos, float2* vel,
                             const float2* acc, float* theta, float* omega, const float* alpha,
-                            const uint32_t* type, const float* radius, float* pressure, uint32_t n);
+                            const uint32_t* type, const float* radius, float* pressure, uint32_t n,
+                            const NextKeys* next = nullptr);
 
 // Radix sort of (key, value) pairs, stable, ascending, on the low key_bits.
 struct SortScratch {

@@ -103,42 +103,6 @@ __global__ __launch_bounds__(256) void k_hash(StepConst c, const float2* __restr
   index[i] = i;
 }
 
-// The hash of a state that is already in the sorted order of its previous
-// keys prev_key: the new key of every particle, plus the particles whose key
-// changed ("movers") as a bit mask (bit i%32 of word i/32) and their count
-// per tile of kResortTile particles.  Wave w of a block covers 1024
-// consecutive particles of the block's tile, 64 at a time.
-template <bool kMulti>
-__global__ __launch_bounds__(256) void k_hash_movers(StepConst c, const float2* __restrict__ pos,
-                                                     const uint32_t* __restrict__ type,
-                                                     const uint32_t* __restrict__ prev_key,
-                                                     uint32_t* __restrict__ key, uint32_t* __restrict__ bits,
-                                                     uint32_t* __restrict__ tile_count, uint32_t n) {
-  __shared__ uint32_t wcount[4];
-  const uint32_t lane = __lane_id(), w = threadIdx.x / kWave;
-  const uint32_t base = blockIdx.x * kResortTile + w * (kResortTile / 4);
-  const uint32_t nwords = (n + 31u) / 32u;
-  uint32_t count = 0;
-  for (uint32_t it = 0; it < kResortTile / 4 / kWave; ++it) {
-    const uint32_t i0 = base + it * kWave, i = i0 + lane;
-    bool moved = false;
-    if (i < n) {
-      const uint32_t k = cell_key<kMulti>(c, pos[i], type, i);
-      key[i] = k;
-      moved = k != prev_key[i];
-    }
-    const uint64_t m = __ballot(moved);
-    count += (uint32_t)__popcll(m);
-    if (lane == 0 && i0 < n) {
-      bits[i0 / 32u] = (uint32_t)m;
-      if (i0 / 32u + 1u < nwords) bits[i0 / 32u + 1u] = (uint32_t)(m >> 32);
-    }
-  }
-  if (lane == 0) wcount[w] = count;
-  __syncthreads();
-  if (threadIdx.x == 0) tile_count[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
-}
-
 // -------------------------------------------------------------- cell fill --
 // cudaMemset(cellStart, 0xff) of src/functions.cu:371 as a kernel (a memset
 // node in a captured graph is avoided on purpose).
@@ -412,12 +376,12 @@ __device__ __forceinline__ float wrap_theta(float th) {
 
 // integrateSystemD (src/particles_kernel.cuh:580-627) with collideBoundary
 // (:388-456) inlined for the bottom wall.
-template <bool kRadius, bool kMulti>
+template <bool kRadius, bool kMulti, bool kNext>
 __global__ __launch_bounds__(256) void k_integrate(
     StepConst c, float2* __restrict__ pos, float2* __restrict__ vel, const float2* __restrict__ acc,
     float* __restrict__ theta, float* __restrict__ omega, const float* __restrict__ alpha,
     const uint32_t* __restrict__ type, const float* __restrict__ radius,
-    float* __restrict__ pressure, uint32_t n) {
+    float* __restrict__ pressure, NextKeys nk, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t t = clamp_idx(type[i], c.num_types);
@@ -506,6 +470,20 @@ __global__ __launch_bounds__(256) void k_integrate(
   vel[i] = v;
   omega[i] = w;
   theta[i] = th;
+  if (kNext) {
+    // the next step's hash: new cell key, and whether it left the cell of
+    // its sorted slot (the state stays in this order until the next sort)
+    const uint32_t k = cell_key<kMulti>(c, p, type, i);
+    nk.key[i] = k;
+    const uint64_t moved = __ballot(k != nk.prev_key[i]);
+    if (__lane_id() == 0) {  // lane 0 holds the wave's first index, a multiple of 64
+      const uint32_t w0 = i / 32u, nwords = (n + 31u) / 32u;
+      nk.bits[w0] = (uint32_t)moved;
+      if (w0 + 1u < nwords) nk.bits[w0 + 1u] = (uint32_t)(moved >> 32);
+      const uint32_t cnt = (uint32_t)__popcll(moved);
+      if (cnt) atomicAdd(&nk.tile_count[i / kResortTile], cnt);
+    }
+  }
 }
 
 inline dim3 grid_for(uint32_t n, int block) { return dim3((n + (uint32_t)block - 1u) / (uint32_t)block); }
@@ -519,18 +497,6 @@ hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float
     hipLaunchKernelGGL(k_hash<true>, grid_for(n, block), dim3(block), 0, s, c, pos, type, hash, index, n);
   else
     hipLaunchKernelGGL(k_hash<false>, grid_for(n, block), dim3(block), 0, s, c, pos, type, hash, index, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_hash_movers(const StepConst& c, hipStream_t s, const float2* pos, const uint32_t* type,
-                              const uint32_t* prev_key, uint32_t* key, uint32_t* bits, uint32_t* tile_count,
-                              uint32_t n) {
-  if (n == 0) return hipSuccess;
-  const dim3 grid((n + kResortTile - 1u) / kResortTile);
-  if (type && c.num_instances > 1)
-    hipLaunchKernelGGL(k_hash_movers<true>, grid, dim3(256), 0, s, c, pos, type, prev_key, key, bits, tile_count, n);
-  else
-    hipLaunchKernelGGL(k_hash_movers<false>, grid, dim3(256), 0, s, c, pos, type, prev_key, key, bits, tile_count, n);
   return hipGetLastError();
 }
 
@@ -581,17 +547,26 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
 
 hipError_t launch_integrate(const StepConst& c, hipStream_t s, int block, float2* pos, float2* vel,
                             const float2* acc, float* theta, float* omega, const float* alpha,
-                            const uint32_t* type, const float* radius, float* pressure, uint32_t n) {
+                            const uint32_t* type, const float* radius, float* pressure, uint32_t n,
+                            const NextKeys* next) {
   if (n == 0) return hipSuccess;
+  if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
   const bool multi = c.num_instances > 1;
-#define DEM_INTEGRATE(R, M)                                                                         \
-  hipLaunchKernelGGL((k_integrate<R, M>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
-                     theta, omega, alpha, type, radius, pressure, n)
+  const NextKeys nk = next ? *next : NextKeys{};
+#define DEM_INTEGRATE(R, M, X)                                                                         \
+  hipLaunchKernelGGL((k_integrate<R, M, X>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
+                     theta, omega, alpha, type, radius, pressure, nk, n)
+#define DEM_INTEGRATE_X(R, M)            \
+  do {                                   \
+    if (next) DEM_INTEGRATE(R, M, true); \
+    else DEM_INTEGRATE(R, M, false);     \
+  } while (0)
   if (radius) {
-    if (multi) DEM_INTEGRATE(true, true); else DEM_INTEGRATE(true, false);
+    if (multi) DEM_INTEGRATE_X(true, true); else DEM_INTEGRATE_X(true, false);
   } else {
-    if (multi) DEM_INTEGRATE(false, true); else DEM_INTEGRATE(false, false);
+    if (multi) DEM_INTEGRATE_X(false, true); else DEM_INTEGRATE_X(false, false);
   }
+#undef DEM_INTEGRATE_X
 #undef DEM_INTEGRATE
   return hipGetLastError();
 }

@@ -16,7 +16,9 @@
 // written exactly once, and the result equals the full radix sort for any
 // number of movers; the cost grows with the number of movers.
 //
-// Launches (after k_hash_movers, which replaces the plain hash):
+// The keys, the mover bit mask (bit i%32 of word i/32) and the movers per
+// tile of kResortTile particles come from the previous step's integrate,
+// which hashes the positions it has just written.  Launches:
 //   compact : per tile, movers before each 32-particle word and the
 //             compacted (key, position) list of the movers;
 //   movers  : one workgroup sorts the movers: a bitonic sort of the unique
@@ -127,7 +129,7 @@ __device__ __forceinline__ uint32_t block_scan_1024(uint32_t x, uint32_t* wsum) 
 // One workgroup: the mover count (sum of the tile counts) and the stable
 // sort of (mkey[0], mpos[0]) by key.  info[0] = count, info[1] = the buffer
 // (0 or 1) that holds the sorted movers.
-__global__ __launch_bounds__(kSortThreads) void k_resort_movers(const uint32_t* __restrict__ tile_count, uint32_t tiles,
+__global__ __launch_bounds__(kSortThreads) void k_resort_movers(uint32_t* __restrict__ tile_count, uint32_t tiles,
                                                                 uint32_t* mkey0, uint32_t* mpos0, uint32_t* mkey1,
                                                                 uint32_t* mpos1, uint32_t key_bits,
                                                                 uint32_t* __restrict__ info,
@@ -153,6 +155,9 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(const uint32_t* 
     if (host_count) *host_count = m;
   }
   __syncthreads();
+  // the counts are consumed (compact ran before): clear them for the next
+  // step's integrate, which accumulates into them
+  for (uint32_t t = tid; t < tiles; t += kSortThreads) tile_count[t] = 0;
   const uint32_t m = s_m;
   if (m <= 1u) {
     if (tid == 0) info[1] = 0;
