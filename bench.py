@@ -32,9 +32,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain step"
 
 # Algorithmic (compulsory) HBM bytes per particle and per cell for each
-# stage, FP32 state with per-particle radius and 32-bit keys:
-#   hash      R pos 8;                         W key 4, index 4
-#             (hash + movers: R pos 8, previous key 4; W key 4)
+# stage of the owned simulation's step, FP32 state with per-particle radius
+# and 32-bit keys:
+#   hash      radix-sorted steps only (re-sorted steps get their keys from
+#             the previous collide): R pos 8; W key 4, index 4
 #   sort      re-sort: R key 4;                W sorted key 4, index 4
 #             (a radix pass would be R key+index 8, W key+index 8)
 #   fill      W cellStart 4 per cell
@@ -42,13 +43,23 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 #             (state = pos 8, vel 8, theta 4, omega 4, id 4, type 4, radius 4)
 #   collide   R pos 8, vel 8, omega 4, radius 4, type 4; W acc 8, alpha 4, pressure 4
 #             + R cellStart/cellEnd 8 per cell
-#   integrate R pos, vel, acc 24, theta, omega, alpha, type, radius, pressure 24;
-#             W pos, vel 16, theta, omega 8
-BYTES_PER_PARTICLE = {"hash": 16, "sort": 12, "cell_fill": 0, "reorder": 84, "collide": 44, "integrate": 72}
+#   integrate (with the next step's hash, sorted buffer -> home buffer)
+#             R pos 8, vel 8, theta 4, omega 4, acc 8, alpha 4, type 4, radius 4,
+#             pressure 4, previous key 4; W pos 8, vel 8, theta 4, omega 4, key 4
+#   When the autotuner fuses collide and integrate into one kernel, that
+#   kernel's bytes are R pos 8, vel 8, omega 4, theta 4, radius 4, type 4,
+#   previous key 4; W acc 8, alpha 4, pressure 4, pos 8, vel 8, theta 4,
+#   omega 4, key 4 (80) + 8 per cell, reported under collide.
+BYTES_PER_PARTICLE = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 84, "collide": 44, "integrate": 80}
 BYTES_PER_CELL = {"hash": 0, "sort": 0, "cell_fill": 4, "reorder": 0, "collide": 8, "integrate": 0}
+FUSED_COLLIDE_BYTES = 80
 
 
-def algorithmic_bytes(stage: str, n: int, cells: int) -> int:
+def algorithmic_bytes(stage: str, n: int, cells: int, fused: bool = False) -> int:
+    if fused and stage == "collide":
+        return FUSED_COLLIDE_BYTES * n + BYTES_PER_CELL[stage] * cells
+    if fused and stage == "integrate":
+        return 0
     return BYTES_PER_PARTICLE[stage] * n + BYTES_PER_CELL[stage] * cells
 
 
@@ -222,11 +233,13 @@ def main() -> int:
     # per-kernel durations with HIP events on the launch stream; the sort is
     # a multi-kernel stage (3 kernels per radix pass) and is reported apart
     stages = sim.stage_times(max(5, min(args.steps, 50)))
-    kernel_stages = {k: v for k, v in stages.items() if k in ("hash", "cell_fill", "reorder", "collide", "integrate")}
+    fused = bool(tuned and tuned.get("fused"))
+    kernel_stages = {k: v for k, v in stages.items()
+                     if k in ("cell_fill", "reorder", "collide", "integrate") and algorithmic_bytes(k, n, cells, fused) > 0}
     dom = max(kernel_stages, key=kernel_stages.get)
-    dom_bytes = algorithmic_bytes(dom, n, cells)
+    dom_bytes = algorithmic_bytes(dom, n, cells, fused)
     achieved = dom_bytes / (kernel_stages[dom] * 1e-3) / 1e9
-    step_bytes = sum(algorithmic_bytes(k, n, cells) for k in BYTES_PER_PARTICLE)
+    step_bytes = sum(algorithmic_bytes(k, n, cells, fused) for k in BYTES_PER_PARTICLE)
 
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{args.workload}.json"

@@ -460,22 +460,27 @@ void sim_free(dem_sim* m) {
 hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, const StepHook* hook) {
   dem_context* ctx = m->ctx;
   const StepConst& c = ctx->c;
+  // static fields (id, type, loc, radius) ping-pong between buffers o and t;
+  // the dynamic fields (pos, vel, theta, omega) live in buffer kHome between
+  // steps and are sorted into kSorted, from which the fused collide +
+  // integrate writes them back
   const int o = m->cur, t = o ^ 1;
+  const int H = kHome, S = kSorted;
   const uint32_t n = m->n;
   const uint32_t* type_for_hash = multi(ctx) ? m->type[o] : nullptr;
   hipError_t e;
   if (ev) (void)hipEventRecord(ev[0], s);
   if (resort) {
     // the keys, mover bits and counts were written by the previous step's
-    // integrate (the hash stage is fused into it)
+    // collide + integrate (the hash stage is fused into it)
     if (ev) (void)hipEventRecord(ev[1], s);
     e = dem::launch_resort(s, m->key, m->skey[o], m->cell_start, m->cell_end, m->cells, true, ctx->key_bits,
                            m->skey[t], m->sidx, m->resort, n);
     if (e != hipSuccess) return e;
   } else {
-    e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], type_for_hash, m->key, m->idx, n);
+    e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[H], type_for_hash, m->key, m->idx, n);
     if (e != hipSuccess) return e;
-    // this step's integrate accumulates the next step's mover counts
+    // this step's collide + integrate accumulates the next step's mover counts
     const uint32_t tiles = (n + dem::kResortTile - 1u) / dem::kResortTile;
     e = dem::launch_fill(s, m->resort.tile_count, 0u, tiles);
     if (e != hipSuccess) return e;
@@ -495,28 +500,40 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   e = dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[3], s);
-  e = dem::launch_reorder(s, ctx->bs_reorder, m->cell_start, m->cell_end, m->pos[t], m->vel[t], m->theta[t],
-                          m->omega[t], m->id[t], m->loc[t], m->type[t], m->rad ? m->radius[t] : nullptr,
-                          m->sorted_key, m->sorted_idx, m->pos[o], m->vel[o], m->theta[o], m->omega[o], m->id[o],
+  e = dem::launch_reorder(s, ctx->bs_reorder, m->cell_start, m->cell_end, m->pos[S], m->vel[S], m->theta[S],
+                          m->omega[S], m->id[t], m->loc[t], m->type[t], m->rad ? m->radius[t] : nullptr,
+                          m->sorted_key, m->sorted_idx, m->pos[H], m->vel[H], m->theta[H], m->omega[H], m->id[o],
                           m->type[o], m->rad ? m->radius[o] : nullptr, n, m->cells);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[4], s);
-  e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[t], m->vel[t], m->acc, m->omega[t], m->alpha, m->type[t],
-                          m->rad ? m->radius[t] : nullptr, m->cell_start, m->cell_end, m->pressure, n);
-  if (e != hipSuccess) return e;
-  if (hook && hook->fn) {
-    e = hook->fn(hook->user, m, s, t);
-    if (e != hipSuccess) return e;
-  }
-  if (ev) (void)hipEventRecord(ev[5], s);
   dem::NextKeys nk;
   nk.prev_key = m->skey[t];
   nk.key = m->key;
   nk.bits = m->resort.bits;
   nk.tile_count = m->resort.tile_count;
-  e = dem::launch_integrate(c, s, ctx->bs_integrate, m->pos[t], m->vel[t], m->acc, m->theta[t], m->omega[t],
-                            m->alpha, m->type[t], m->rad ? m->radius[t] : nullptr, m->pressure, n, &nk);
-  if (e != hipSuccess) return e;
+  const float* S_radius = m->rad ? m->radius[t] : nullptr;
+  if (ctx->fuse) {
+    e = dem::launch_collide_integrate(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->omega[S], m->theta[S],
+                                      m->type[t], S_radius, m->cell_start, m->cell_end, m->acc, m->alpha, m->pressure,
+                                      m->pos[H], m->vel[H], m->omega[H], m->theta[H], nk, n);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[5], s);
+  } else {
+    e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->acc, m->omega[S], m->alpha, m->type[t],
+                            S_radius, m->cell_start, m->cell_end, m->pressure, n);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[5], s);
+    // integrate from the sorted buffer back to the home buffer
+    e = dem::launch_integrate_to(c, s, ctx->bs_integrate, m->pos[S], m->vel[S], m->theta[S], m->omega[S], m->pos[H],
+                                 m->vel[H], m->theta[H], m->omega[H], m->acc, m->alpha, m->type[t], S_radius,
+                                 m->pressure, n, &nk);
+    if (e != hipSuccess) return e;
+  }
+  if (hook && hook->fn) {
+    // the sorted pre-integrate state is still in kSorted / t
+    e = hook->fn(hook->user, m, s, t);
+    if (e != hipSuccess) return e;
+  }
   if (ev) (void)hipEventRecord(ev[6], s);
   m->cur = t;
   return hipSuccess;
@@ -671,10 +688,10 @@ int dem_sim_upload(dem_sim* m, const float* pos, const float* vel, const float* 
   const int c = m->cur;
   const size_t n = m->n;
   hipStream_t s = ctx->stream;
-  DEM_HIP(ctx, hipMemcpyAsync(m->pos[c], pos, 8 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->vel[c], vel, 8 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->theta[c], theta, 4 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->omega[c], omega, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->pos[kHome], pos, 8 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->vel[kHome], vel, 8 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->theta[kHome], theta, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->omega[kHome], omega, 4 * n, hipMemcpyHostToDevice, s), "upload");
   DEM_HIP(ctx, hipMemcpyAsync(m->id[c], id, 4 * n, hipMemcpyHostToDevice, s), "upload");
   DEM_HIP(ctx, hipMemcpyAsync(m->type[c], type, 4 * n, hipMemcpyHostToDevice, s), "upload");
   DEM_HIP(ctx, hipMemcpyAsync(m->loc[c], loc, 4 * n, hipMemcpyHostToDevice, s), "upload");
@@ -695,8 +712,8 @@ int dem_sim_download(dem_sim* m, float* pos, float* vel, float* theta, float* om
   const size_t n = m->n;
   hipStream_t s = ctx->stream;
   struct Item { void* dst; const void* src; size_t bytes; } items[] = {
-      {pos, m->pos[c], 8 * n},          {vel, m->vel[c], 8 * n},           {theta, m->theta[c], 4 * n},
-      {omega, m->omega[c], 4 * n},      {id, m->id[c], 4 * n},             {type, m->type[c], 4 * n},
+      {pos, m->pos[kHome], 8 * n},      {vel, m->vel[kHome], 8 * n},       {theta, m->theta[kHome], 4 * n},
+      {omega, m->omega[kHome], 4 * n},  {id, m->id[c], 4 * n},             {type, m->type[c], 4 * n},
       {loc, m->loc[c], 4 * n},          {radius, m->rad ? m->radius[c] : nullptr, 4 * n},
       {acc, m->acc, 8 * n},             {alpha, m->alpha, 4 * n},          {pressure, m->pressure, 4 * n},
       {hash, m->sorted_key, 4 * n},     {index, m->sorted_idx, 4 * n},
@@ -743,6 +760,13 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
   // the sorted keys are in skey[cur] and the sorted index in sidx.
   m->sorted_key = m->skey[m->cur];
   m->sorted_idx = m->sidx;
+  return DEM_OK;
+}
+
+int dem_set_step_fusion(dem_context* ctx, int fused) {
+  if (!ctx) return DEM_ERR_INVALID;
+  ctx->fuse = fused != 0;
+  ctx->generation++;  // captured graphs follow the choice
   return DEM_OK;
 }
 
@@ -820,7 +844,8 @@ int dem_sim_get_by_id(dem_sim* m, float* pos_by_id, float* vel_by_id) {
   dem_context* ctx = m->ctx;
   if (int r = ensure_device(ctx)) return r;
   const int c = m->cur;
-  hipLaunchKernelGGL(k_get_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c], m->pos[c], m->vel[c],
+  hipLaunchKernelGGL(k_get_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c], m->pos[kHome],
+                     m->vel[kHome],
                      (float2*)pos_by_id, (float2*)vel_by_id, m->n);
   DEM_HIP(ctx, hipGetLastError(), "get by id");
   return DEM_OK;
@@ -832,7 +857,7 @@ int dem_sim_set_by_id(dem_sim* m, const float* pos_by_id, const float* vel_by_id
   if (int r = ensure_device(ctx)) return r;
   const int c = m->cur;
   hipLaunchKernelGGL(k_set_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c],
-                     (const float2*)pos_by_id, (const float2*)vel_by_id, m->pos[c], m->vel[c], m->n);
+                     (const float2*)pos_by_id, (const float2*)vel_by_id, m->pos[kHome], m->vel[kHome], m->n);
   DEM_HIP(ctx, hipGetLastError(), "set by id");
   // the order still follows skey[cur], but new positions may move many
   // particles to other cells: the next step sorts from scratch
@@ -894,8 +919,9 @@ int dem_gather_rows(dem_context* ctx, const uint32_t* follow_ids, uint32_t count
 int dem_sim_gather_rows(dem_sim* m, const uint32_t* follow_ids, uint32_t count, uint32_t* rows) {
   if (!m) return DEM_ERR_INVALID;
   const int c = m->cur;
-  return dem_gather_rows(m->ctx, follow_ids, count, m->loc[c], (const float*)m->pos[c], (const float*)m->vel[c],
-                         (const float*)m->acc, m->theta[c], m->omega[c], m->alpha, m->id[c], m->type[c], m->pressure,
+  return dem_gather_rows(m->ctx, follow_ids, count, m->loc[c], (const float*)m->pos[kHome],
+                         (const float*)m->vel[kHome], (const float*)m->acc, m->theta[kHome], m->omega[kHome], m->alpha,
+                         m->id[c], m->type[c], m->pressure,
                          m->n, rows);
 }
 
@@ -929,15 +955,16 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   // Populate the scratch buffer t (the next step's sorted state) once; the
   // current state in buffer o is never written.
   bool in_alt = false;
-  DEM_HIP(ctx, dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n), "tune");
+  DEM_HIP(ctx, dem::launch_hash(c, s, ctx->bs_hash, m->pos[kHome], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n), "tune");
   DEM_HIP(ctx, dem::launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt), "tune");
   uint32_t* sk = in_alt ? m->sort.keys_alt : m->key;
   uint32_t* si = in_alt ? m->sort.vals_alt : m->idx;
   DEM_HIP(ctx, dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells), "tune");
   auto reorder = [&](int bs) {
-    return dem::launch_reorder(s, bs, m->cell_start, m->cell_end, m->pos[t], m->vel[t], m->theta[t], m->omega[t],
-                               m->id[t], m->loc[t], m->type[t], m->rad ? m->radius[t] : nullptr, sk, si, m->pos[o],
-                               m->vel[o], m->theta[o], m->omega[o], m->id[o], m->type[o],
+    return dem::launch_reorder(s, bs, m->cell_start, m->cell_end, m->pos[kSorted], m->vel[kSorted],
+                               m->theta[kSorted], m->omega[kSorted], m->id[t], m->loc[t], m->type[t],
+                               m->rad ? m->radius[t] : nullptr, sk, si, m->pos[kHome], m->vel[kHome],
+                               m->theta[kHome], m->omega[kHome], m->id[o], m->type[o],
                                m->rad ? m->radius[o] : nullptr, n, m->cells);
   };
   DEM_HIP(ctx, reorder(ctx->bs_reorder), "tune");
@@ -955,45 +982,79 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
     if (e == hipSuccess) e = hipEventElapsedTime(out, e0, e1);
     return e;
   };
-  int best[4] = {ctx->bs_hash, ctx->bs_reorder, ctx->bs_collide, ctx->bs_integrate};
-  float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-  hipError_t e = hipSuccess;
+  // scratch outputs for the integrated state, so the live state is never written
+  float2 *tpos = nullptr, *tvel = nullptr;
+  float *ttheta = nullptr, *tomega = nullptr;
+  hipError_t e = hipMalloc((void**)&tpos, 8 * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void**)&tvel, 8 * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void**)&ttheta, 4 * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void**)&tomega, 4 * (size_t)n);
+  dem::NextKeys nk;
+  nk.prev_key = sk;
+  nk.key = m->key == sk ? m->idx : m->key;  // any n-word buffer other than sk
+  nk.bits = m->resort.bits;
+  nk.tile_count = m->resort.tile_count;
+  const float2* S_pos = m->pos[kSorted];
+  const float2* S_vel = m->vel[kSorted];
+  const float* S_theta = m->theta[kSorted];
+  const float* S_omega = m->omega[kSorted];
+  const float* S_radius = m->rad ? m->radius[t] : nullptr;
+  auto collide = [&](int b) {
+    return dem::launch_collide(c, s, b, S_pos, S_vel, m->acc, S_omega, m->alpha, m->type[t], S_radius, m->cell_start,
+                               m->cell_end, m->pressure, n);
+  };
+  auto integrate = [&](int b) {
+    return dem::launch_integrate_to(c, s, b, S_pos, S_vel, S_theta, S_omega, tpos, tvel, ttheta, tomega, m->acc,
+                                    m->alpha, m->type[t], S_radius, m->pressure, n, &nk);
+  };
+  auto fused = [&](int b) {
+    return dem::launch_collide_integrate(c, s, b, S_pos, S_vel, S_omega, S_theta, m->type[t], S_radius,
+                                         m->cell_start, m->cell_end, m->acc, m->alpha, m->pressure, tpos, tvel,
+                                         tomega, ttheta, nk, n);
+  };
+  int best[5] = {ctx->bs_hash, ctx->bs_reorder, ctx->bs_collide, ctx->bs_integrate, ctx->bs_collide};
+  float best_t[5] = {1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
   for (int bs : cand) {
+    if (e != hipSuccess) break;
     float tm = 0;
-    e = time_it([&](int b) { return dem::launch_hash(c, s, b, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n); }, bs, &tm);
+    e = time_it([&](int b) { return dem::launch_hash(c, s, b, m->pos[kHome], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n); }, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[0]) { best_t[0] = tm; best[0] = bs; }
     e = time_it([&](int b) { return reorder(b); }, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[1]) { best_t[1] = tm; best[1] = bs; }
-    e = time_it([&](int b) {
-          return dem::launch_collide(c, s, b, m->pos[t], m->vel[t], m->acc, m->omega[t], m->alpha, m->type[t],
-                                     m->rad ? m->radius[t] : nullptr, m->cell_start, m->cell_end, m->pressure, n);
-        }, bs, &tm);
+    e = time_it(collide, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[2]) { best_t[2] = tm; best[2] = bs; }
-    e = time_it([&](int b) {
-          return dem::launch_integrate(c, s, b, m->pos[t], m->vel[t], m->acc, m->theta[t], m->omega[t], m->alpha,
-                                       m->type[t], m->rad ? m->radius[t] : nullptr, m->pressure, n);
-        }, bs, &tm);
+    e = time_it(integrate, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[3]) { best_t[3] = tm; best[3] = bs; }
+    e = time_it(fused, bs, &tm);
+    if (e != hipSuccess) break;
+    if (tm < best_t[4]) { best_t[4] = tm; best[4] = bs; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  m->resort_ok = false;  // the cell arrays now hold the tuning step's ranges
+  if (tpos) (void)hipFree(tpos);
+  if (tvel) (void)hipFree(tvel);
+  if (ttheta) (void)hipFree(ttheta);
+  if (tomega) (void)hipFree(tomega);
+  m->resort_ok = false;  // the cell arrays and keys now hold the tuning step's data
   if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
+  // collide + integrate as one kernel or two, whichever is faster here
+  const bool fuse = best_t[4] < best_t[2] + best_t[3];
   ctx->bs_hash = best[0];
   ctx->bs_reorder = best[1];
-  ctx->bs_collide = best[2];
+  ctx->bs_collide = fuse ? best[4] : best[2];
   ctx->bs_integrate = best[3];
+  ctx->fuse = fuse;
   ctx->generation++;
   if (block) {
     block[0] = (uint32_t)best[0];
     block[1] = (uint32_t)best[1];
-    block[2] = (uint32_t)best[2];
+    block[2] = (uint32_t)ctx->bs_collide;
     block[3] = (uint32_t)best[3];
-    block[4] = 16;
+    block[4] = fuse ? 1u : 0u;
   }
   return DEM_OK;
 }

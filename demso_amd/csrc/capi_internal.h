@@ -24,8 +24,14 @@ struct dem_context {
   dem::SortScratch sort;
   uint32_t* d_copy_tmp = nullptr;
   int bs_hash = 256, bs_reorder = 256, bs_collide = 128, bs_integrate = 256;
+  bool fuse = false;  // owned simulation: collide + integrate as one kernel (chosen by dem_sim_autotune)
   uint64_t generation = 1;  // bumped by every parameter change
 };
+
+// Buffer of the owned simulation's dynamic fields (pos, vel, theta, omega)
+// between steps, and the buffer they are sorted into during a step.
+constexpr int kHome = 0;
+constexpr int kSorted = 1;
 
 struct dem_sim {
   dem_context* ctx = nullptr;

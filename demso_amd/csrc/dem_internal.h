@@ -98,6 +98,21 @@ hipError_t launch_integrate(const StepConst& c, hipStream_t s, int block, float2
                             const float2* acc, float* theta, float* omega, const float* alpha,
                             const uint32_t* type, const float* radius, float* pressure, uint32_t n,
                             const NextKeys* next = nullptr);
+// integrate from one set of state arrays into another (may be the same).
+hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, const float2* pos_in,
+                               const float2* vel_in, const float* theta_in, const float* omega_in, float2* pos_out,
+                               float2* vel_out, float* theta_out, float* omega_out, const float2* acc,
+                               const float* alpha, const uint32_t* type, const float* radius, float* pressure,
+                               uint32_t n, const NextKeys* next = nullptr);
+// collide + integrate + the next step's hash in one kernel: forces from the
+// sorted state *_in, acc/alpha/pressure as collide writes them, the
+// integrated state to *_out (a different buffer).
+hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block, const float2* pos_in,
+                                    const float2* vel_in, const float* omega_in, const float* theta_in,
+                                    const uint32_t* type, const float* radius, const uint32_t* cell_start,
+                                    const uint32_t* cell_end, float2* acc, float* alpha, float* pressure,
+                                    float2* pos_out, float2* vel_out, float* omega_out, float* theta_out,
+                                    const NextKeys& next, uint32_t n);
 
 // Radix sort of (key, value) pairs, stable, ascending, on the low key_bits.
 struct SortScratch {

@@ -274,25 +274,40 @@ __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, 
   }
 }
 
-template <bool kRadius, bool kMulti>
-__global__ __launch_bounds__(256) void k_collide(
-    StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,
-    float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha,
-    const uint32_t* __restrict__ type, const float* __restrict__ radius,
-    const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
-    float* __restrict__ pressure, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// Forces on sorted particle i from its 3x3 cell neighbourhood (the body of
+// collideD, src/particles_kernel.cuh:459-549): sum of the pair forces fx,
+// fy, the moment and the pressure, plus the particle's own radius, mass,
+// inertia and material (for integrate).
+struct Own {
+  float r, mass, inertia;
+  uint32_t t;
+  Material m;
+  Instance in;
+  float2 p, v;
+  float w;
+};
 
+template <bool kRadius, bool kMulti>
+__device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i, const float2* __restrict__ pos,
+                                                 const float2* __restrict__ vel, const float* __restrict__ omega,
+                                                 const uint32_t* __restrict__ type, const float* __restrict__ radius,
+                                                 const uint32_t* __restrict__ cell_start,
+                                                 const uint32_t* __restrict__ cell_end, uint32_t n, Own& own,
+                                                 float& fx_out, float& fy_out, float& moment_out, float& pres_out) {
   PairCtx q;
-  const uint32_t tA = clamp_idx(type[i], c.num_types);
-  q.mA = c.mat[tA];
-  const Instance in = kMulti ? c.inst[clamp_idx(q.mA.inst, c.num_instances)] : c.inst0;
-  float massA, inertiaA;
-  body<kRadius>(q.mA, radius, i, q.rA, massA, inertiaA);
+  own.t = clamp_idx(type[i], c.num_types);
+  q.mA = c.mat[own.t];
+  own.m = q.mA;
+  own.in = kMulti ? c.inst[clamp_idx(q.mA.inst, c.num_instances)] : c.inst0;
+  const Instance& in = own.in;
+  body<kRadius>(q.mA, radius, i, q.rA, own.mass, own.inertia);
+  own.r = q.rA;
   q.p = pos[i];
   q.v = vel[i];
   q.w = omega[i];
+  own.p = q.p;
+  own.v = q.v;
+  own.w = q.w;
   q.den = pressure_den(q.rA);
   q.dt = c.dt;
   const bool clamp = c.flags & DEM_GRID_CLAMP;
@@ -360,9 +375,27 @@ __global__ __launch_bounds__(256) void k_collide(
       fy += cfy;
     }
   }
+  fx_out = fx;
+  fy_out = fy;
+  moment_out = moment;
+  pres_out = pres;
+}
 
-  acc[i] = make_float2(fx / massA, fy / massA);
-  alpha[i] = moment / inertiaA;
+template <bool kRadius, bool kMulti>
+__global__ __launch_bounds__(256) void k_collide(
+    StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,
+    float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha,
+    const uint32_t* __restrict__ type, const float* __restrict__ radius,
+    const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+    float* __restrict__ pressure, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Own own;
+  float fx, fy, moment, pres;
+  collide_particle<kRadius, kMulti>(c, i, pos, vel, omega, type, radius, cell_start, cell_end, n, own, fx, fy,
+                                    moment, pres);
+  acc[i] = make_float2(fx / own.mass, fy / own.mass);
+  alpha[i] = moment / own.inertia;
   pressure[i] = pres;
 }
 
@@ -375,31 +408,21 @@ __device__ __forceinline__ float wrap_theta(float th) {
 }
 
 // integrateSystemD (src/particles_kernel.cuh:580-627) with collideBoundary
-// (:388-456) inlined for the bottom wall.
-template <bool kRadius, bool kMulti, bool kNext>
-__global__ __launch_bounds__(256) void k_integrate(
-    StepConst c, float2* __restrict__ pos, float2* __restrict__ vel, const float2* __restrict__ acc,
-    float* __restrict__ theta, float* __restrict__ omega, const float* __restrict__ alpha,
-    const uint32_t* __restrict__ type, const float* __restrict__ radius,
-    float* __restrict__ pressure, NextKeys nk, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t t = clamp_idx(type[i], c.num_types);
-  const Material m = c.mat[t];
-  const Instance in = kMulti ? c.inst[clamp_idx(m.inst, c.num_instances)] : c.inst0;
-  float r, mass, inertia;
-  body<kRadius>(m, radius, i, r, mass, inertia);
+// (:388-456) inlined for the bottom wall, on one particle: p, v, w, th are
+// updated in place; pres gains the bottom wall's term when it is hit (the
+// return value says whether the bottom-wall law ran).  With pres_src, pres
+// is first loaded from it, and only on that branch.
+__device__ __forceinline__ bool integrate_particle(const StepConst& c, const Material& m, const Instance& in, float r,
+                                                   float mass, float inertia, float2 a, float alpha_i, float2& p,
+                                                   float2& v, float& w, float& th, float& pres,
+                                                   const float* pres_src) {
   const float dt = c.dt;
-
-  const float2 a = acc[i];
-  float2 v = vel[i];
   v.x = v.x + (c.gx + a.x) * dt;
   v.y = v.y + (c.gy + a.y) * dt;
-  float2 p = pos[i];
   p.x = p.x + v.x * dt;
   p.y = p.y + v.y * dt;
-  float w = omega[i] + alpha[i] * dt;
-  float th = theta[i] + w * dt;
+  w = w + alpha_i * dt;
+  th = th + w * dt;
   th = wrap_theta(th);
 
   const float bd = -m.bdamp;
@@ -419,7 +442,7 @@ __global__ __launch_bounds__(256) void k_integrate(
   if (p.y < r) {
     // collideBoundary(pos, vel, omega, type, (pos.x, 0), wallAlpha, pressure)
     float wallAlpha = 0.0f;
-    float pr = pressure[i];
+    float pr = pres_src ? *pres_src : pres;
     const float rx = p.x - p.x, ry = 0.0f - p.y;
     const float dist = sqrtf(rx * rx + ry * ry);
     float wfx = 0.0f, wfy = 0.0f;
@@ -464,26 +487,93 @@ __global__ __launch_bounds__(256) void k_integrate(
     th = th + (wallAlpha * dt) * dt;
     p.y = r;
     v.y = v.y * bd;
-    pressure[i] = pr;
+    pres = pr;
+    return true;
   }
-  pos[i] = p;
-  vel[i] = v;
-  omega[i] = w;
-  theta[i] = th;
-  if (kNext) {
-    // the next step's hash: new cell key, and whether it left the cell of
-    // its sorted slot (the state stays in this order until the next sort)
-    const uint32_t k = cell_key<kMulti>(c, p, type, i);
-    nk.key[i] = k;
-    const uint64_t moved = __ballot(k != nk.prev_key[i]);
-    if (__lane_id() == 0) {  // lane 0 holds the wave's first index, a multiple of 64
-      const uint32_t w0 = i / 32u, nwords = (n + 31u) / 32u;
-      nk.bits[w0] = (uint32_t)moved;
-      if (w0 + 1u < nwords) nk.bits[w0 + 1u] = (uint32_t)(moved >> 32);
-      const uint32_t cnt = (uint32_t)__popcll(moved);
-      if (cnt) atomicAdd(&nk.tile_count[i / kResortTile], cnt);
-    }
+  return false;
+}
+
+// The next step's hash of an integrated particle (NextKeys).
+template <bool kMulti>
+__device__ __forceinline__ void next_key(const StepConst& c, const NextKeys& nk, float2 p,
+                                         const uint32_t* __restrict__ type, uint32_t i, uint32_t n) {
+  // the new cell key, and whether it left the cell of its sorted slot (the
+  // state stays in this order until the next sort)
+  const uint32_t k = cell_key<kMulti>(c, p, type, i);
+  nk.key[i] = k;
+  const uint64_t moved = __ballot(k != nk.prev_key[i]);
+  if (__lane_id() == 0) {  // lane 0 holds the wave's first index, a multiple of 64
+    const uint32_t w0 = i / 32u, nwords = (n + 31u) / 32u;
+    nk.bits[w0] = (uint32_t)moved;
+    if (w0 + 1u < nwords) nk.bits[w0 + 1u] = (uint32_t)(moved >> 32);
+    const uint32_t cnt = (uint32_t)__popcll(moved);
+    if (cnt) atomicAdd(&nk.tile_count[i / kResortTile], cnt);
   }
+}
+
+// integrateSystemD over all particles.  The input and output arrays may be
+// the same (in place, as the reference) or different (the owned
+// simulation writes the integrated state back to its home buffer); each
+// lane reads and writes only its own slot.
+template <bool kRadius, bool kMulti, bool kNext>
+__global__ __launch_bounds__(256) void k_integrate(
+    StepConst c, const float2* pos_in, const float2* vel_in, const float* theta_in, const float* omega_in,
+    float2* pos_out, float2* vel_out, float* theta_out, float* omega_out, const float2* __restrict__ acc,
+    const float* __restrict__ alpha, const uint32_t* __restrict__ type, const float* __restrict__ radius,
+    float* __restrict__ pressure, NextKeys nk, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t t = clamp_idx(type[i], c.num_types);
+  const Material m = c.mat[t];
+  const Instance in = kMulti ? c.inst[clamp_idx(m.inst, c.num_instances)] : c.inst0;
+  float r, mass, inertia;
+  body<kRadius>(m, radius, i, r, mass, inertia);
+  float2 p = pos_in[i], v = vel_in[i];
+  float w = omega_in[i], th = theta_in[i];
+  float pres = 0.0f;
+  if (integrate_particle(c, m, in, r, mass, inertia, acc[i], alpha[i], p, v, w, th, pres, pressure + i))
+    pressure[i] = pres;
+  pos_out[i] = p;
+  vel_out[i] = v;
+  omega_out[i] = w;
+  theta_out[i] = th;
+  if (kNext) next_key<kMulti>(c, nk, p, type, i, n);
+}
+
+// collide + integrate + the next step's hash in one pass over the sorted
+// particles (the owned simulation's step): forces from the sorted state
+// (pos, vel, omega, theta in the *_in arrays), accelerations and pressure
+// written as collide writes them, and the integrated state written to the
+// *_out arrays (the unsorted buffer of this step, dead after reorder), so no
+// lane overwrites a neighbour's input.  The arithmetic is that of k_collide
+// followed by k_integrate.
+template <bool kRadius, bool kMulti>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_collide_integrate(
+    StepConst c, const float2* __restrict__ pos_in, const float2* __restrict__ vel_in,
+    const float* __restrict__ omega_in, const float* __restrict__ theta_in, const uint32_t* __restrict__ type,
+    const float* __restrict__ radius, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float2* __restrict__ acc, float* __restrict__ alpha,
+    float* __restrict__ pressure, float2* __restrict__ pos_out, float2* __restrict__ vel_out,
+    float* __restrict__ omega_out, float* __restrict__ theta_out, NextKeys nk, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Own own;
+  float fx, fy, moment, pres;
+  collide_particle<kRadius, kMulti>(c, i, pos_in, vel_in, omega_in, type, radius, cell_start, cell_end, n, own, fx,
+                                    fy, moment, pres);
+  const float2 a = make_float2(fx / own.mass, fy / own.mass);
+  const float al = moment / own.inertia;
+  acc[i] = a;
+  alpha[i] = al;
+  float2 p = own.p, v = own.v;
+  float w = own.w, th = theta_in[i];
+  (void)integrate_particle(c, own.m, own.in, own.r, own.mass, own.inertia, a, al, p, v, w, th, pres, nullptr);
+  pressure[i] = pres;
+  pos_out[i] = p;
+  vel_out[i] = v;
+  omega_out[i] = w;
+  theta_out[i] = th;
+  next_key<kMulti>(c, nk, p, type, i, n);
 }
 
 inline dim3 grid_for(uint32_t n, int block) { return dim3((n + (uint32_t)block - 1u) / (uint32_t)block); }
@@ -545,17 +635,49 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
   return hipGetLastError();
 }
 
+hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block, const float2* pos_in,
+                                    const float2* vel_in, const float* omega_in, const float* theta_in,
+                                    const uint32_t* type, const float* radius, const uint32_t* cell_start,
+                                    const uint32_t* cell_end, float2* acc, float* alpha, float* pressure,
+                                    float2* pos_out, float2* vel_out, float* omega_out, float* theta_out,
+                                    const NextKeys& next, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
+  const bool multi = c.num_instances > 1;
+#define DEM_CI(R, M)                                                                                     \
+  hipLaunchKernelGGL((k_collide_integrate<R, M>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
+                     omega_in, theta_in, type, radius, cell_start, cell_end, acc, alpha, pressure, pos_out,  \
+                     vel_out, omega_out, theta_out, next, n)
+  if (radius) {
+    if (multi) DEM_CI(true, true); else DEM_CI(true, false);
+  } else {
+    if (multi) DEM_CI(false, true); else DEM_CI(false, false);
+  }
+#undef DEM_CI
+  return hipGetLastError();
+}
+
 hipError_t launch_integrate(const StepConst& c, hipStream_t s, int block, float2* pos, float2* vel,
                             const float2* acc, float* theta, float* omega, const float* alpha,
                             const uint32_t* type, const float* radius, float* pressure, uint32_t n,
                             const NextKeys* next) {
+  return launch_integrate_to(c, s, block, pos, vel, theta, omega, pos, vel, theta, omega, acc, alpha, type, radius,
+                             pressure, n, next);
+}
+
+hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, const float2* pos_in,
+                               const float2* vel_in, const float* theta_in, const float* omega_in, float2* pos_out,
+                               float2* vel_out, float* theta_out, float* omega_out, const float2* acc,
+                               const float* alpha, const uint32_t* type, const float* radius, float* pressure,
+                               uint32_t n, const NextKeys* next) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
   const bool multi = c.num_instances > 1;
   const NextKeys nk = next ? *next : NextKeys{};
-#define DEM_INTEGRATE(R, M, X)                                                                         \
-  hipLaunchKernelGGL((k_integrate<R, M, X>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
-                     theta, omega, alpha, type, radius, pressure, nk, n)
+#define DEM_INTEGRATE(R, M, X)                                                                          \
+  hipLaunchKernelGGL((k_integrate<R, M, X>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
+                     theta_in, omega_in, pos_out, vel_out, theta_out, omega_out, acc, alpha, type, radius,  \
+                     pressure, nk, n)
 #define DEM_INTEGRATE_X(R, M)            \
   do {                                   \
     if (next) DEM_INTEGRATE(R, M, true); \

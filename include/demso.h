@@ -213,11 +213,15 @@ int dem_sim_step(dem_sim* sim, uint32_t steps);
 int dem_sim_step_graph(dem_sim* sim, uint32_t steps);
 /* Average device time per stage over `reps` eager steps, measured with HIP
  * events on the context stream: ms[0]=hash ms[1]=sort ms[2]=cell fill
- * ms[3]=reorder ms[4]=collide ms[5]=integrate ms[6]=whole step. */
+ * ms[3]=reorder ms[4]=collide ms[5]=integrate ms[6]=whole step.  The owned
+ * simulation computes the next step's hash in integrate (ms[0] is 0 for
+ * re-sorted steps) and, if dem_sim_autotune chose so, runs collide and
+ * integrate as one kernel (timed in ms[4]; ms[5] is then 0). */
 int dem_sim_stage_times(dem_sim* sim, uint32_t reps, float* ms);
-/* Choose block sizes per kernel by timing candidates on the current state
- * (the state is restored afterwards).  Writes the choice to block[5]
- * (hash, reorder, collide, integrate, sort items-per-thread) if non-NULL. */
+/* Choose block sizes per kernel, and whether the sim runs collide and
+ * integrate as one kernel or two, by timing candidates on the current state
+ * (the state is not modified).  Writes the choice to block[5] (hash,
+ * reorder, collide, integrate, 1 if fused else 0) if non-NULL. */
 int dem_sim_autotune(dem_sim* sim, uint32_t* block);
 /* Sort method of the sim's steps.  After a step the state is in the sorted
  * order of its keys, so the next step can re-sort: only particles whose cell
@@ -231,6 +235,10 @@ int dem_sim_set_sort_mode(dem_sim* sim, int mode);
 /* Particles whose cell changed in the last re-sorted step (exact once the
  * stream is synchronized). */
 int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
+/* Owned simulations of ctx run collide and integrate as one kernel (fused
+ * != 0) or as two (0, the default); dem_sim_autotune sets whichever is
+ * faster.  Results are the same bit for bit. */
+int dem_set_step_fusion(dem_context* ctx, int fused);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);
 /* Current positions / velocities gathered by particle ID into device arrays

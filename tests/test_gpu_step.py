@@ -73,11 +73,36 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
     sim.step(3)
     ref.step(3)
     choice = sim.autotune()
-    assert set(choice.values()) <= {64, 128, 256, 16}
+    fused = choice.pop("fused")
+    assert isinstance(fused, bool)
+    assert set(choice.values()) <= {64, 128, 256}
     sim.step(4)
     ref.step(4)
     compare(sim.download(), ref.state())
     sim.close()
+
+
+@pytest.mark.parametrize("name", ["pack_1024", "ensemble_5", "pile_wrap_attract"])
+def test_fused_step_equals_split(gpu_ctx, name):
+    """collide + integrate as one kernel or as two: the same state, bit for bit."""
+    make, steps = CASES[name]
+    s = make()
+    ctx2 = api.Context(0)
+    ctx2.load_scene(s)
+    gpu_ctx.load_scene(s)
+    gpu_ctx.set_step_fusion(False)
+    ctx2.set_step_fusion(True)
+    a = api.Simulation(gpu_ctx, s)
+    b = api.Simulation(ctx2, s)
+    for chunk in (1, steps):
+        a.step_graph(chunk)
+        b.step_graph(chunk)
+        da, db = a.download(), b.download()
+        for k in ("hash", "index", "cellStart", "id", "pos", "vel", "theta", "omega", "acc", "alpha", "pressure"):
+            assert bits_equal(da[k], db[k]), k
+    a.close()
+    b.close()
+    ctx2.close()
 
 
 @pytest.mark.slow
