@@ -113,7 +113,7 @@ PROTOTYPES = {
     "dem_sim_autotune": (c_int, [c_void_p, POINTER(c_uint32)]),
     "dem_sim_sort_passes": (c_int, [c_void_p]),
     "dem_sim_set_sort_mode": (c_int, [c_void_p, c_int]),
-    "dem_set_step_fusion": (c_int, [c_void_p, c_int]),
+    "dem_set_step_variant": (c_int, [c_void_p, c_int]),
     "dem_sim_mover_count": (c_int, [c_void_p, POINTER(c_uint32)]),
     "dem_sim_get_by_id": (c_int, [c_void_p, P, P]),
     "dem_sim_set_by_id": (c_int, [c_void_p, P, P]),

@@ -402,7 +402,7 @@ int dem_collide_ex(dem_context* ctx, const float* pos, const float* vel, float* 
   if (int r = ensure_device(ctx)) return r;
   DEM_HIP(ctx, dem::launch_collide(ctx->c, ctx->stream, ctx->bs_collide, (const float2*)pos, (const float2*)vel,
                                    (float2*)acc, omega, alpha, type, radius, cellStart, cellEnd, pressure,
-                                   numParticles),
+                                   numParticles, ctx->prefetch),
           "collide kernel");
   return DEM_OK;
 }
@@ -515,12 +515,12 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   if (ctx->fuse) {
     e = dem::launch_collide_integrate(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->omega[S], m->theta[S],
                                       m->type[t], S_radius, m->cell_start, m->cell_end, m->acc, m->alpha, m->pressure,
-                                      m->pos[H], m->vel[H], m->omega[H], m->theta[H], nk, n);
+                                      m->pos[H], m->vel[H], m->omega[H], m->theta[H], nk, n, ctx->prefetch);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[5], s);
   } else {
     e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->acc, m->omega[S], m->alpha, m->type[t],
-                            S_radius, m->cell_start, m->cell_end, m->pressure, n);
+                            S_radius, m->cell_start, m->cell_end, m->pressure, n, ctx->prefetch);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[5], s);
     // integrate from the sorted buffer back to the home buffer
@@ -763,9 +763,11 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
   return DEM_OK;
 }
 
-int dem_set_step_fusion(dem_context* ctx, int fused) {
+int dem_set_step_variant(dem_context* ctx, int flags) {
   if (!ctx) return DEM_ERR_INVALID;
-  ctx->fuse = fused != 0;
+  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH)) return fail(ctx, DEM_ERR_INVALID, "unknown step variant flags");
+  ctx->fuse = (flags & DEM_STEP_FUSED) != 0;
+  ctx->prefetch = (flags & DEM_STEP_PREFETCH) != 0;
   ctx->generation++;  // captured graphs follow the choice
   return DEM_OK;
 }
@@ -999,9 +1001,10 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   const float* S_theta = m->theta[kSorted];
   const float* S_omega = m->omega[kSorted];
   const float* S_radius = m->rad ? m->radius[t] : nullptr;
+  bool pf = false;  // prefetch variant under test
   auto collide = [&](int b) {
     return dem::launch_collide(c, s, b, S_pos, S_vel, m->acc, S_omega, m->alpha, m->type[t], S_radius, m->cell_start,
-                               m->cell_end, m->pressure, n);
+                               m->cell_end, m->pressure, n, pf);
   };
   auto integrate = [&](int b) {
     return dem::launch_integrate_to(c, s, b, S_pos, S_vel, S_theta, S_omega, tpos, tvel, ttheta, tomega, m->acc,
@@ -1010,10 +1013,13 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   auto fused = [&](int b) {
     return dem::launch_collide_integrate(c, s, b, S_pos, S_vel, S_omega, S_theta, m->type[t], S_radius,
                                          m->cell_start, m->cell_end, m->acc, m->alpha, m->pressure, tpos, tvel,
-                                         tomega, ttheta, nk, n);
+                                         tomega, ttheta, nk, n, pf);
   };
-  int best[5] = {ctx->bs_hash, ctx->bs_reorder, ctx->bs_collide, ctx->bs_integrate, ctx->bs_collide};
-  float best_t[5] = {1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
+  int best[4] = {ctx->bs_hash, ctx->bs_reorder, ctx->bs_collide, ctx->bs_integrate};
+  float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  // collide and fused block sizes for each prefetch choice
+  int best_c[2] = {ctx->bs_collide, ctx->bs_collide}, best_f[2] = {ctx->bs_collide, ctx->bs_collide};
+  float t_c[2] = {1e30f, 1e30f}, t_f[2] = {1e30f, 1e30f};
   for (int bs : cand) {
     if (e != hipSuccess) break;
     float tm = 0;
@@ -1023,15 +1029,18 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
     e = time_it([&](int b) { return reorder(b); }, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[1]) { best_t[1] = tm; best[1] = bs; }
-    e = time_it(collide, bs, &tm);
-    if (e != hipSuccess) break;
-    if (tm < best_t[2]) { best_t[2] = tm; best[2] = bs; }
     e = time_it(integrate, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[3]) { best_t[3] = tm; best[3] = bs; }
-    e = time_it(fused, bs, &tm);
-    if (e != hipSuccess) break;
-    if (tm < best_t[4]) { best_t[4] = tm; best[4] = bs; }
+    for (int p = 0; p < 2 && e == hipSuccess; ++p) {
+      pf = p == 1;
+      e = time_it(collide, bs, &tm);
+      if (e != hipSuccess) break;
+      if (tm < t_c[p]) { t_c[p] = tm; best_c[p] = bs; }
+      e = time_it(fused, bs, &tm);
+      if (e != hipSuccess) break;
+      if (tm < t_f[p]) { t_f[p] = tm; best_f[p] = bs; }
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1041,20 +1050,27 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   if (tomega) (void)hipFree(tomega);
   m->resort_ok = false;  // the cell arrays and keys now hold the tuning step's data
   if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
-  // collide + integrate as one kernel or two, whichever is faster here
-  const bool fuse = best_t[4] < best_t[2] + best_t[3];
+  // collide + integrate as one kernel or two, with or without the candidate
+  // prefetch: whichever is fastest here
+  int choice = 0;  // 0/1 split without/with prefetch, 2/3 fused without/with
+  float tbest = t_c[0] + best_t[3];
+  if (t_c[1] + best_t[3] < tbest) { tbest = t_c[1] + best_t[3]; choice = 1; }
+  if (t_f[0] < tbest) { tbest = t_f[0]; choice = 2; }
+  if (t_f[1] < tbest) { tbest = t_f[1]; choice = 3; }
+  const bool fuse = choice >= 2, prefetch = (choice & 1) != 0;
   ctx->bs_hash = best[0];
   ctx->bs_reorder = best[1];
-  ctx->bs_collide = fuse ? best[4] : best[2];
+  ctx->bs_collide = fuse ? best_f[prefetch] : best_c[prefetch];
   ctx->bs_integrate = best[3];
   ctx->fuse = fuse;
+  ctx->prefetch = prefetch;
   ctx->generation++;
   if (block) {
     block[0] = (uint32_t)best[0];
     block[1] = (uint32_t)best[1];
     block[2] = (uint32_t)ctx->bs_collide;
     block[3] = (uint32_t)best[3];
-    block[4] = fuse ? 1u : 0u;
+    block[4] = (fuse ? 1u : 0u) | (prefetch ? 2u : 0u);
   }
   return DEM_OK;
 }

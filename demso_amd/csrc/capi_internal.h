@@ -24,7 +24,8 @@ struct dem_context {
   dem::SortScratch sort;
   uint32_t* d_copy_tmp = nullptr;
   int bs_hash = 256, bs_reorder = 256, bs_collide = 128, bs_integrate = 256;
-  bool fuse = false;  // owned simulation: collide + integrate as one kernel (chosen by dem_sim_autotune)
+  bool fuse = false;      // owned simulation: collide + integrate as one kernel (chosen by dem_sim_autotune)
+  bool prefetch = false;  // collide fetches each candidate one iteration ahead (chosen by dem_sim_autotune)
   uint64_t generation = 1;  // bumped by every parameter change
 };
 

@@ -192,20 +192,10 @@ struct PairCtx {
 };
 
 template <bool kRadius>
-__device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j,
-                                          const float2* __restrict__ pos, const float2* __restrict__ vel,
-                                          const float* __restrict__ omega, const uint32_t* __restrict__ type,
-                                          const float* __restrict__ radius, float& cfx, float& cfy,
-                                          float& moment, float& pres) {
-  const float2 pB = pos[j];
-  uint32_t tB = 0;
-  float rB;
-  if (kRadius) {
-    rB = radius[j];
-  } else {
-    tB = clamp_idx(type[j], c.num_types);
-    rB = c.mat[tB].radius;
-  }
+__device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, float2 pB, float rB,
+                                             uint32_t tB, const float2* __restrict__ vel,
+                                             const float* __restrict__ omega, const uint32_t* __restrict__ type,
+                                             float& cfx, float& cfy, float& moment, float& pres) {
   const float rx = pB.x - q.p.x, ry = pB.y - q.p.y;
   const float d2 = rx * rx + ry * ry;
   const float R = q.rA + rB;
@@ -274,6 +264,35 @@ __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, 
   }
 }
 
+// Candidate j's position and radius (and, without a radius array, its
+// material row), as the pair law reads them.
+template <bool kRadius>
+__device__ __forceinline__ void candidate_load(const StepConst& c, uint32_t j, const float2* __restrict__ pos,
+                                               const uint32_t* __restrict__ type, const float* __restrict__ radius,
+                                               float2& pB, float& rB, uint32_t& tB) {
+  pB = pos[j];
+  if (kRadius) {
+    rB = radius[j];
+    tB = 0;
+  } else {
+    tB = clamp_idx(type[j], c.num_types);
+    rB = c.mat[tB].radius;
+  }
+}
+
+template <bool kRadius>
+__device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j,
+                                          const float2* __restrict__ pos, const float2* __restrict__ vel,
+                                          const float* __restrict__ omega, const uint32_t* __restrict__ type,
+                                          const float* __restrict__ radius, float& cfx, float& cfy,
+                                          float& moment, float& pres) {
+  float2 pB;
+  float rB;
+  uint32_t tB;
+  candidate_load<kRadius>(c, j, pos, type, radius, pB, rB, tB);
+  candidate_at<kRadius>(c, q, j, pB, rB, tB, vel, omega, type, cfx, cfy, moment, pres);
+}
+
 // Forces on sorted particle i from its 3x3 cell neighbourhood (the body of
 // collideD, src/particles_kernel.cuh:459-549): sum of the pair forces fx,
 // fy, the moment and the pressure, plus the particle's own radius, mass,
@@ -287,7 +306,7 @@ struct Own {
   float w;
 };
 
-template <bool kRadius, bool kMulti>
+template <bool kRadius, bool kMulti, bool kPrefetch>
 __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i, const float2* __restrict__ pos,
                                                  const float2* __restrict__ vel, const float* __restrict__ omega,
                                                  const uint32_t* __restrict__ type, const float* __restrict__ radius,
@@ -325,54 +344,93 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   const bool contiguous = clamp || (x0 + 1u == x1 && x1 + 1u == x2);
 
   float fx = 0.0f, fy = 0.0f, moment = 0.0f, pres = 0.0f;
-  for (int y = -1; y <= 1; ++y) {
-    uint32_t cy;
-    if (clamp) {
-      const int iy = gy + y;
-      if (iy < 0 || iy >= G1) continue;
-      cy = (uint32_t)iy;
-    } else {
-      cy = ((uint32_t)gy + (uint32_t)y) & ((uint32_t)G1 - 1u);
-    }
-    const uint32_t row = in.cell_base + cy * (uint32_t)G0;
-    uint32_t s0 = vx0 ? cell_start[row + x0] : kEmpty;
-    uint32_t s1 = cell_start[row + x1];
-    uint32_t s2 = vx2 ? cell_start[row + x2] : kEmpty;
-    const bool n0 = s0 != kEmpty, n1 = s1 != kEmpty, n2 = s2 != kEmpty;
-    const uint32_t e0 = n0 ? min(cell_end[row + x0], n) : 0u;
-    const uint32_t e1 = n1 ? min(cell_end[row + x1], n) : 0u;
-    const uint32_t e2 = n2 ? min(cell_end[row + x2], n) : 0u;
-    if (!n0) s0 = 0;
-    if (!n1) s1 = 0;
-    if (!n2) s2 = 0;
-    // contiguous row: one range [rs, re) with cell ends b0, b1 inside it;
-    // otherwise the three cells one by one, in the reference order
-    const int nseg = contiguous ? 1 : 3;
-    for (int sg = 0; sg < nseg; ++sg) {
-      uint32_t rs, re, b0, b1;
-      if (contiguous) {
-        rs = n0 ? s0 : (n1 ? s1 : s2);
-        re = n2 ? e2 : (n1 ? e1 : e0);
-        b0 = n0 ? e0 : kNoBoundary;
-        b1 = n1 ? e1 : kNoBoundary;
+  if (contiguous) {
+    // The usual case: every row's three cells are consecutive keys, so a row
+    // is one candidate range [rs, re) with the cell ends b0, b1 inside it.
+    // The ranges of all three rows are loaded first (their loads overlap),
+    // and each candidate's position and radius are fetched one iteration
+    // ahead; the summation order is unchanged.
+    uint32_t rs[3], re[3], b0[3], b1[3];
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
+      const int iy = gy + y - 1;
+      uint32_t cy;
+      bool ok = true;
+      if (clamp) {
+        ok = iy >= 0 && iy < G1;
+        cy = ok ? (uint32_t)iy : 0u;
       } else {
-        rs = sg == 0 ? s0 : (sg == 1 ? s1 : s2);
-        re = sg == 0 ? e0 : (sg == 1 ? e1 : e2);
-        b0 = kNoBoundary;
-        b1 = kNoBoundary;
+        cy = (uint32_t)iy & ((uint32_t)G1 - 1u);
       }
+      const uint32_t row = in.cell_base + cy * (uint32_t)G0;
+      const uint32_t s0 = ok && vx0 ? cell_start[row + x0] : kEmpty;
+      const uint32_t s1 = ok ? cell_start[row + x1] : kEmpty;
+      const uint32_t s2 = ok && vx2 ? cell_start[row + x2] : kEmpty;
+      const bool n0 = s0 != kEmpty, n1 = s1 != kEmpty, n2 = s2 != kEmpty;
+      const uint32_t e0 = n0 ? min(cell_end[row + x0], n) : 0u;
+      const uint32_t e1 = n1 ? min(cell_end[row + x1], n) : 0u;
+      const uint32_t e2 = n2 ? min(cell_end[row + x2], n) : 0u;
+      rs[y] = n0 ? s0 : (n1 ? s1 : (n2 ? s2 : 0u));
+      re[y] = n2 ? e2 : (n1 ? e1 : e0);
+      b0[y] = n0 ? e0 : kNoBoundary;
+      b1[y] = n1 ? e1 : kNoBoundary;
+    }
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
       float cfx = 0.0f, cfy = 0.0f;
-      for (uint32_t j = rs; j < re; ++j) {
-        if (j == b0 || j == b1) {
-          fx += cfx;
-          fy += cfy;
-          cfx = 0.0f;
-          cfy = 0.0f;
+      const uint32_t end = re[y];
+      if (kPrefetch) {
+        // each candidate's position and radius fetched one iteration ahead
+        // (pays off with many candidates per row, costs registers)
+        uint32_t j = rs[y];
+        float2 pn = make_float2(0.0f, 0.0f);
+        float rn = 0.0f;
+        uint32_t tn = 0;
+        if (j < end) candidate_load<kRadius>(c, j, pos, type, radius, pn, rn, tn);
+        for (; j < end; ++j) {
+          const float2 pB = pn;
+          const float rB = rn;
+          const uint32_t tB = tn;
+          if (j + 1u < end) candidate_load<kRadius>(c, j + 1u, pos, type, radius, pn, rn, tn);
+          if (j == b0[y] || j == b1[y]) {
+            fx += cfx;
+            fy += cfy;
+            cfx = 0.0f;
+            cfy = 0.0f;
+          }
+          if (j != i) candidate_at<kRadius>(c, q, j, pB, rB, tB, vel, omega, type, cfx, cfy, moment, pres);
         }
-        if (j != i) candidate<kRadius>(c, q, j, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
+      } else {
+        for (uint32_t j = rs[y]; j < end; ++j) {
+          if (j == b0[y] || j == b1[y]) {
+            fx += cfx;
+            fy += cfy;
+            cfx = 0.0f;
+            cfy = 0.0f;
+          }
+          if (j != i) candidate<kRadius>(c, q, j, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
+        }
       }
       fx += cfx;
       fy += cfy;
+    }
+  } else {
+    // wrap mode at the grid's x edge: the three cells one by one, in the
+    // reference order
+    for (int y = -1; y <= 1; ++y) {
+      const uint32_t cy = ((uint32_t)gy + (uint32_t)y) & ((uint32_t)G1 - 1u);
+      const uint32_t row = in.cell_base + cy * (uint32_t)G0;
+      const uint32_t xs[3] = {x0, x1, x2};
+      for (int sg = 0; sg < 3; ++sg) {
+        const uint32_t st = cell_start[row + xs[sg]];
+        if (st == kEmpty) continue;
+        const uint32_t en = min(cell_end[row + xs[sg]], n);
+        float cfx = 0.0f, cfy = 0.0f;
+        for (uint32_t j = st; j < en; ++j)
+          if (j != i) candidate<kRadius>(c, q, j, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
+        fx += cfx;
+        fy += cfy;
+      }
     }
   }
   fx_out = fx;
@@ -381,7 +439,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   pres_out = pres;
 }
 
-template <bool kRadius, bool kMulti>
+template <bool kRadius, bool kMulti, bool kPrefetch>
 __global__ __launch_bounds__(256) void k_collide(
     StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,
     float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha,
@@ -392,8 +450,8 @@ __global__ __launch_bounds__(256) void k_collide(
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
-  collide_particle<kRadius, kMulti>(c, i, pos, vel, omega, type, radius, cell_start, cell_end, n, own, fx, fy,
-                                    moment, pres);
+  collide_particle<kRadius, kMulti, kPrefetch>(c, i, pos, vel, omega, type, radius, cell_start, cell_end, n, own, fx,
+                                               fy, moment, pres);
   acc[i] = make_float2(fx / own.mass, fy / own.mass);
   alpha[i] = moment / own.inertia;
   pressure[i] = pres;
@@ -547,7 +605,7 @@ __global__ __launch_bounds__(256) void k_integrate(
 // *_out arrays (the unsorted buffer of this step, dead after reorder), so no
 // lane overwrites a neighbour's input.  The arithmetic is that of k_collide
 // followed by k_integrate.
-template <bool kRadius, bool kMulti>
+template <bool kRadius, bool kMulti, bool kPrefetch>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_collide_integrate(
     StepConst c, const float2* __restrict__ pos_in, const float2* __restrict__ vel_in,
     const float* __restrict__ omega_in, const float* __restrict__ theta_in, const uint32_t* __restrict__ type,
@@ -559,8 +617,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
-  collide_particle<kRadius, kMulti>(c, i, pos_in, vel_in, omega_in, type, radius, cell_start, cell_end, n, own, fx,
-                                    fy, moment, pres);
+  collide_particle<kRadius, kMulti, kPrefetch>(c, i, pos_in, vel_in, omega_in, type, radius, cell_start, cell_end, n,
+                                               own, fx, fy, moment, pres);
   const float2 a = make_float2(fx / own.mass, fy / own.mass);
   const float al = moment / own.inertia;
   acc[i] = a;
@@ -620,17 +678,23 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
-                          const uint32_t* cell_end, float* pressure, uint32_t n) {
+                          const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch) {
   if (n == 0) return hipSuccess;
   const bool multi = c.num_instances > 1;
-#define DEM_COLLIDE(R, M)                                                                        \
-  hipLaunchKernelGGL((k_collide<R, M>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
+#define DEM_COLLIDE(R, M, P)                                                                        \
+  hipLaunchKernelGGL((k_collide<R, M, P>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
                      omega, alpha, type, radius, cell_start, cell_end, pressure, n)
+#define DEM_COLLIDE_P(R, M)               \
+  do {                                    \
+    if (prefetch) DEM_COLLIDE(R, M, true); \
+    else DEM_COLLIDE(R, M, false);        \
+  } while (0)
   if (radius) {
-    if (multi) DEM_COLLIDE(true, true); else DEM_COLLIDE(true, false);
+    if (multi) DEM_COLLIDE_P(true, true); else DEM_COLLIDE_P(true, false);
   } else {
-    if (multi) DEM_COLLIDE(false, true); else DEM_COLLIDE(false, false);
+    if (multi) DEM_COLLIDE_P(false, true); else DEM_COLLIDE_P(false, false);
   }
+#undef DEM_COLLIDE_P
 #undef DEM_COLLIDE
   return hipGetLastError();
 }
@@ -640,19 +704,25 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
                                     const uint32_t* type, const float* radius, const uint32_t* cell_start,
                                     const uint32_t* cell_end, float2* acc, float* alpha, float* pressure,
                                     float2* pos_out, float2* vel_out, float* omega_out, float* theta_out,
-                                    const NextKeys& next, uint32_t n) {
+                                    const NextKeys& next, uint32_t n, bool prefetch) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
   const bool multi = c.num_instances > 1;
-#define DEM_CI(R, M)                                                                                     \
-  hipLaunchKernelGGL((k_collide_integrate<R, M>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
-                     omega_in, theta_in, type, radius, cell_start, cell_end, acc, alpha, pressure, pos_out,  \
+#define DEM_CI(R, M, P)                                                                                     \
+  hipLaunchKernelGGL((k_collide_integrate<R, M, P>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
+                     omega_in, theta_in, type, radius, cell_start, cell_end, acc, alpha, pressure, pos_out,     \
                      vel_out, omega_out, theta_out, next, n)
+#define DEM_CI_P(R, M)               \
+  do {                               \
+    if (prefetch) DEM_CI(R, M, true); \
+    else DEM_CI(R, M, false);        \
+  } while (0)
   if (radius) {
-    if (multi) DEM_CI(true, true); else DEM_CI(true, false);
+    if (multi) DEM_CI_P(true, true); else DEM_CI_P(true, false);
   } else {
-    if (multi) DEM_CI(false, true); else DEM_CI(false, false);
+    if (multi) DEM_CI_P(false, true); else DEM_CI_P(false, false);
   }
+#undef DEM_CI_P
 #undef DEM_CI
   return hipGetLastError();
 }

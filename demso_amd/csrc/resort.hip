@@ -24,9 +24,10 @@
 //   movers  : one workgroup sorts the movers: a bitonic sort of the unique
 //             64-bit (key << 32 | position) values in LDS up to 4096 movers,
 //             a stable chunked LSD radix sort through global memory beyond;
-//   place   : the two slot formulas above.  A stayer's mover count is a
-//             binary search in the window of sorted movers that falls
-//             between its tile's first and last keys (staged in LDS); a
+//   place   : the two slot formulas above, one particle per lane.  A
+//             stayer's mover count is a binary search in the window of
+//             sorted movers that falls between the first and last keys of
+//             its block's 256 particles (staged in LDS); a
 //             mover's stayer count comes from the previous step's cell
 //             ranges (cellStart/cellEnd of prev_key) or a binary search
 //             over prev_key.
@@ -41,7 +42,7 @@ constexpr int kSortWaves = kSortThreads / kWave;
 constexpr uint32_t kBitonicMax = 4096;
 constexpr int kPlaceThreads = 256;
 constexpr uint32_t kMoverBlocks = 32;
-constexpr uint32_t kWindowMax = 2048;
+constexpr uint32_t kWindowMax = 1024;
 constexpr int kDigitBits = 8;
 constexpr int kDigits = 1 << kDigitBits;
 
@@ -314,11 +315,10 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     return;
   }
 
-  // stayers of one tile
-  const uint32_t tile = blockIdx.x - kMoverBlocks;
-  const uint32_t first = tile * kResortTile;
+  // stayers: one particle per lane, a tile of kPlaceThreads per block
+  const uint32_t first = (blockIdx.x - kMoverBlocks) * kPlaceThreads;
   if (first >= n) return;
-  const uint32_t last = min(first + kResortTile, n) - 1u;
+  const uint32_t last = min(first + kPlaceThreads, n) - 1u;
   if (tid == 0) s_lo = m ? movers_below(sk, sp, 0, m, composite(prev_key[first], first)) : 0u;
   if (tid == 1) s_hi = m ? movers_below(sk, sp, 0, m, composite(prev_key[last], last)) : 0u;
   __syncthreads();
@@ -327,27 +327,26 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
   if (staged)
     for (uint32_t j = tid; j < hi - lo; j += kPlaceThreads) win[j] = composite(sk[lo + j], sp[lo + j]);
   __syncthreads();
-  for (uint32_t i = first + tid; i <= last; i += kPlaceThreads) {
-    if ((bits[i >> 5] >> (i & 31u)) & 1u) continue;
-    const uint32_t k = key[i];
-    const uint64_t x = composite(k, i);
-    uint32_t below;
-    if (staged) {
-      uint32_t a = 0, b = hi - lo;
-      while (a < b) {
-        const uint32_t mid = (a + b) >> 1;
-        if (win[mid] < x) a = mid + 1;
-        else b = mid;
-      }
-      below = lo + a;
-    } else {
-      below = movers_below(sk, sp, lo, hi, x);
+  const uint32_t i = first + tid;
+  if (i > last || ((bits[i >> 5] >> (i & 31u)) & 1u)) return;
+  const uint32_t k = key[i];
+  const uint64_t x = composite(k, i);
+  uint32_t below;
+  if (staged) {
+    uint32_t a = 0, b = hi - lo;
+    while (a < b) {
+      const uint32_t mid = (a + b) >> 1;
+      if (win[mid] < x) a = mid + 1;
+      else b = mid;
     }
-    const uint32_t slot = i - movers_before(i, n, m, bits, word_prefix) + below;
-    if (slot < n) {  // always true for a consistent input; guards the store
-      out_key[slot] = k;
-      out_idx[slot] = i;
-    }
+    below = lo + a;
+  } else {
+    below = movers_below(sk, sp, lo, hi, x);
+  }
+  const uint32_t slot = i - movers_before(i, n, m, bits, word_prefix) + below;
+  if (slot < n) {  // always true for a consistent input; guards the store
+    out_key[slot] = k;
+    out_idx[slot] = i;
   }
 }
 
@@ -365,7 +364,8 @@ hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* pre
                      r.word_prefix, r.mkey[0], r.mpos[0], n);
   hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.mkey[0], r.mpos[0],
                      r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count);
-  hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + tiles), dim3(kPlaceThreads), 0, s, key, prev_key, r.bits,
+  const uint32_t place_tiles = (n + kPlaceThreads - 1u) / kPlaceThreads;
+  hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + place_tiles), dim3(kPlaceThreads), 0, s, key, prev_key, r.bits,
                      r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, prev_cell_start,
                      prev_cell_end, num_cells, cells_valid ? 1 : 0, out_key, out_idx, n);
   return hipGetLastError();

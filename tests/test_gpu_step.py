@@ -74,7 +74,8 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
     ref.step(3)
     choice = sim.autotune()
     fused = choice.pop("fused")
-    assert isinstance(fused, bool)
+    prefetch = choice.pop("prefetch")
+    assert isinstance(fused, bool) and isinstance(prefetch, bool)
     assert set(choice.values()) <= {64, 128, 256}
     sim.step(4)
     ref.step(4)
@@ -82,16 +83,18 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
     sim.close()
 
 
+@pytest.mark.parametrize("variant", [(True, False), (False, True), (True, True)])
 @pytest.mark.parametrize("name", ["pack_1024", "ensemble_5", "pile_wrap_attract"])
-def test_fused_step_equals_split(gpu_ctx, name):
-    """collide + integrate as one kernel or as two: the same state, bit for bit."""
+def test_step_variants_equal(gpu_ctx, name, variant):
+    """collide + integrate as one kernel or two, with or without collide's
+    candidate prefetch: the same state, bit for bit."""
     make, steps = CASES[name]
     s = make()
     ctx2 = api.Context(0)
     ctx2.load_scene(s)
     gpu_ctx.load_scene(s)
-    gpu_ctx.set_step_fusion(False)
-    ctx2.set_step_fusion(True)
+    gpu_ctx.set_step_variant(False, False)
+    ctx2.set_step_variant(*variant)
     a = api.Simulation(gpu_ctx, s)
     b = api.Simulation(ctx2, s)
     for chunk in (1, steps):
