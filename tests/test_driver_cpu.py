@@ -12,7 +12,7 @@ from dsml_scenes import MIXED, MIXED_BOX, MIXED_COUNTS, TYPES, run_driver, write
 from demso_amd import dumps
 from demso_amd import scene as sc
 
-EXAMPLES = Path("/root/reference/exemplos")
+EXAMPLES = Path(__file__).resolve().parent / "golden" / "dsml"
 
 
 def initial(tmp_path, text, *extra):

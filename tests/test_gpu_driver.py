@@ -87,3 +87,30 @@ def test_output_file(runs):
                               [final["theta"][pid], final["omega"][pid], final["alpha"][pid],
                                final["pressure"][pid]]])
         assert np.allclose(r[2:], exp, rtol=0, atol=5e-7 + 1e-6 * np.abs(exp)), (pid, r[2:], exp)
+
+
+def test_reference_pile_scene_matches_oracle(tmp_path, oracle):
+    """DemSo's own pilha.dsml (read with the controlled-particle build's rules:
+    136 discs on the 16 x 16 wrapping grid, attraction 10) through the
+    reference's stage loop for 300 steps, against the oracle's run of the same
+    pile (scene.triangle_pile)."""
+    from pathlib import Path
+
+    from demso_amd import scene as sc
+
+    pile = Path(__file__).resolve().parent / "golden" / "dsml" / "pilha.dsml"
+    print("driver: pilha.dsml --steps 300", flush=True)
+    r = run_driver("--controlled", "--steps", 300, "--output", tmp_path / "out.csv", "--dump-final", "--dump-prefix",
+                   tmp_path / "pile_", pile, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = dumps.read_dump(tmp_path / "pile_300.bin")
+    ref = oracle.Sim(sc.triangle_pile(), "f32")
+    ref.step(300)
+    o = ref.state()
+    order = np.argsort(o["id"])
+    for k in ("pos", "vel", "theta", "omega", "acc", "alpha", "pressure"):
+        assert np.array_equal(g[k].view(np.uint32), np.ascontiguousarray(o[k][order]).view(np.uint32)), k
+    ref.close()
+    # pilha.dsml follows IDs 500 and 2200 of 136 discs: reported, nothing written
+    assert "followed particle 500 does not exist" in r.stderr
+    assert (tmp_path / "out.csv").read_text() == ""
