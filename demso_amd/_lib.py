@@ -76,6 +76,24 @@ class InstanceProps(ctypes.Structure):
     ]
 
 
+class ControlledProps(ctypes.Structure):
+    """dem_controlled."""
+
+    _fields_ = [
+        ("pos", c_float * 2),
+        ("vel", c_float * 2),
+        ("theta", c_float),
+        ("omega", c_float),
+        ("type", c_uint32),
+        ("enabled", c_uint32),
+        ("move", ctypes.c_double * 2),
+        ("wrap_below_y", ctypes.c_double),
+        ("wrap_above_x", ctypes.c_double),
+        ("reset_y", c_float),
+        ("reset_x", c_float),
+    ]
+
+
 P = c_void_p  # every device pointer travels as a plain address
 
 # name -> (restype, argtypes)
@@ -119,6 +137,13 @@ PROTOTYPES = {
     "dem_sim_set_by_id": (c_int, [c_void_p, P, P]),
     "dem_gather_rows": (c_int, [c_void_p, P, c_uint32] + [P] * 10 + [c_uint32, P]),
     "dem_sim_gather_rows": (c_int, [c_void_p, P, c_uint32, P]),
+    "dem_set_controlled": (c_int, [c_void_p, POINTER(ControlledProps)]),
+    "dem_get_controlled": (c_int, [c_void_p, POINTER(ControlledProps), POINTER(c_float)]),
+    "dem_collide_controlled": (
+        c_int,
+        [c_void_p] + [P] * 8 + [c_uint32, c_uint32, POINTER(c_float), c_uint32] + [P] * 6
+        + [POINTER(c_float), POINTER(c_float), P],
+    ),
     "dem_set_contact_params": (c_int, [c_void_p, POINTER(ctypes.c_double), c_uint32]),
     "dem_adj_create": (c_int, [c_void_p, c_uint32, POINTER(c_void_p)]),
     "dem_adj_destroy": (c_int, [c_void_p]),

@@ -107,6 +107,7 @@ class Context:
         check(self.lib.dem_create(byref(h), device, stream), None)
         self.handle = h.value
         self.device = device
+        self.box = None  # (Lx, Ly) of the last scene loaded
 
     def close(self) -> None:
         if self.handle:
@@ -135,6 +136,7 @@ class Context:
         """Upload the scene's parameters: one system, or an ensemble (geometry
         first, then the materials that name their instances)."""
         self.set_params(system_props(scene), particle_props(scene, single_instance=True))
+        self.box = tuple(scene.instances[0].box)
         if len(scene.instances) > 1:
             self.set_instances(instance_props(scene))
             self.set_materials(particle_props(scene, single_instance=False))
@@ -146,6 +148,37 @@ class Context:
 
     def set_gravity(self, gx: float, gy: float) -> None:
         self._check(self.lib.dem_set_gravity(self.handle, gx, gy))
+
+    def set_controlled(self, pos=(5.0, 9.5), type: int = 0, vel=(0.0, 0.0), theta: float = 0.0,
+                       omega: float = 0.0, move=(0.0, -0.005), wrap_below_y: float = -1.0,
+                       wrap_above_x: Optional[float] = None, reset_y: float = 10.5, reset_x: float = -0.5,
+                       enabled: bool = True) -> None:
+        """The controlled particle of this context's owned simulations
+        (src/main.cu:510-514 and src/particles_kernel.cuh:525-543); the
+        defaults are DemSo's.  wrap_above_x defaults to Lx + 0.5."""
+        if wrap_above_x is None:
+            if self.box is None:
+                raise ValueError("wrap_above_x needs the box: load a scene first or pass it")
+            wrap_above_x = float(np.float32(self.box[0])) + 0.5
+        c = _lib.ControlledProps()
+        c.pos[:] = [float(pos[0]), float(pos[1])]
+        c.vel[:] = [float(vel[0]), float(vel[1])]
+        c.theta, c.omega, c.type, c.enabled = float(theta), float(omega), int(type), int(bool(enabled))
+        c.move[:] = [float(move[0]), float(move[1])]
+        c.wrap_below_y, c.wrap_above_x = float(wrap_below_y), float(wrap_above_x)
+        c.reset_y, c.reset_x = float(reset_y), float(reset_x)
+        self._check(self.lib.dem_set_controlled(self.handle, byref(c)))
+
+    def controlled(self):
+        """(state dict, reaction (fx, fy, moment) as float32) after the steps
+        run so far."""
+        c = _lib.ControlledProps()
+        r = (c_float * 3)()
+        self._check(self.lib.dem_get_controlled(self.handle, byref(c), r))
+        state = {"pos": np.array(c.pos[:], np.float32), "vel": np.array(c.vel[:], np.float32),
+                 "theta": np.float32(c.theta), "omega": np.float32(c.omega), "type": int(c.type),
+                 "enabled": bool(c.enabled)}
+        return state, np.array(r[:], np.float32)
 
     def set_contact_params(self, params) -> None:
         """params: (num_types, 4) array of (kn, kt, cn, mu) per material."""
@@ -196,6 +229,25 @@ def collide(ctx: Context, oldPos, oldVel, newAcc, oldOmega, newAlpha, oldType, c
     ctx._check(ctx.lib.dem_collide_ex(
         ctx.handle, _ptr(oldPos), _ptr(oldVel), _ptr(newAcc), _ptr(oldOmega), _ptr(newAlpha), _ptr(oldType),
         _ptr(radius), _ptr(cellStart), _ptr(cellEnd), numParticles, numCells, _ptr(pressure)))
+
+
+def collide_controlled(ctx: Context, oldPos, oldVel, newAcc, oldOmega, newAlpha, oldType, cellStart, cellEnd,
+                       numParticles: int, numCells: int, controlPos, controlVel, controlTheta: float,
+                       controlOmega: float, controlType: int, controlForceVecX, hCFVx, controlForceVecY, hCFVy,
+                       controlMomentVec, hCMV, pressure):
+    """collide with the controlled particle (src/functions.cu:398-521,
+    USE_BIG_PARTICLE 1, USE_ATOMIC 0); returns (ctrlF (2,), ctrlM) as
+    float32."""
+    state = (c_float * 6)(float(controlPos[0]), float(controlPos[1]), float(controlVel[0]), float(controlVel[1]),
+                          float(controlTheta), float(controlOmega))
+    f = (c_float * 2)()
+    m = (c_float * 1)()
+    ctx._check(ctx.lib.dem_collide_controlled(
+        ctx.handle, _ptr(oldPos), _ptr(oldVel), _ptr(newAcc), _ptr(oldOmega), _ptr(newAlpha), _ptr(oldType),
+        _ptr(cellStart), _ptr(cellEnd), numParticles, numCells, state, int(controlType), _ptr(controlForceVecX),
+        _ptr(hCFVx), _ptr(controlForceVecY), _ptr(hCFVy), _ptr(controlMomentVec), _ptr(hCMV), f, m,
+        _ptr(pressure)))
+    return np.array(f[:], np.float32), np.float32(m[0])
 
 
 def integrateSystem(ctx: Context, pos, vel, acc, theta, omega, alpha, type, pressure, numParticles: int,

@@ -573,6 +573,7 @@ int dem_adj_create(dem_sim* sim, uint32_t checkpoint_every, dem_adjoint** out) {
   dem_context* ctx = sim->ctx;
   if (checkpoint_every == 0) return fail(ctx, DEM_ERR_INVALID, "checkpoint_every must be >= 1");
   if (ctx->c.attract) return fail(ctx, DEM_ERR_INVALID, "the adjoint does not cover attraction");
+  if (ctx->h_control.enabled) return fail(ctx, DEM_ERR_INVALID, "the adjoint does not cover the controlled particle");
   if (int r = ensure_device(ctx)) return r;
   auto* a = new (std::nothrow) dem_adjoint();
   if (!a) return fail(ctx, DEM_ERR_ALLOC, "host allocation failed");
@@ -622,6 +623,7 @@ int dem_adj_forward(dem_adjoint* a, uint32_t steps) {
   dem_context* ctx = m->ctx;
   if (int r = sim_check(m)) return r;
   if (ctx->c.attract) return fail(ctx, DEM_ERR_INVALID, "the adjoint does not cover attraction");
+  if (ctx->h_control.enabled) return fail(ctx, DEM_ERR_INVALID, "the adjoint does not cover the controlled particle");
   const uint32_t nck = steps / a->K + 1;
   while (a->ckpt.size() < nck) {
     a->ckpt.emplace_back();

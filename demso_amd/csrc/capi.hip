@@ -194,6 +194,8 @@ int dem_destroy(dem_context* ctx) {
   if (ctx->d_mat) (void)hipFree(ctx->d_mat);
   if (ctx->d_inst) (void)hipFree(ctx->d_inst);
   if (ctx->d_copy_tmp) (void)hipFree(ctx->d_copy_tmp);
+  if (ctx->d_control) (void)hipFree(ctx->d_control);
+  if (ctx->d_stage_control) (void)hipFree(ctx->d_stage_control);
   free_sort(ctx->sort);
   delete ctx;
   return DEM_OK;
@@ -512,16 +514,38 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   nk.bits = m->resort.bits;
   nk.tile_count = m->resort.tile_count;
   const float* S_radius = m->rad ? m->radius[t] : nullptr;
+  // the controlled particle: pair term in collide, then its reaction and
+  // move (src/main.cu:466-514); the move does not feed this step's integrate
+  ControlOut co;
+  const bool control = ctx->h_control.enabled != 0;
+  if (control) {
+    co.state = ctx->d_control;
+    co.react_x = m->react_x;
+    co.react_y = m->react_y;
+    co.react_m = m->react_m;
+  }
   if (ctx->fuse) {
     e = dem::launch_collide_integrate(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->omega[S], m->theta[S],
                                       m->type[t], S_radius, m->cell_start, m->cell_end, m->acc, m->alpha, m->pressure,
-                                      m->pos[H], m->vel[H], m->omega[H], m->theta[H], nk, n, ctx->prefetch);
+                                      m->pos[H], m->vel[H], m->omega[H], m->theta[H], nk, n, ctx->prefetch,
+                                      control ? &co : nullptr);
     if (e != hipSuccess) return e;
+    if (control) {
+      e = dem::launch_control_finish(s, co, m->react_bx, m->react_by, m->react_bm, n, ctx->d_reaction, ctx->d_control,
+                                     true);
+      if (e != hipSuccess) return e;
+    }
     if (ev) (void)hipEventRecord(ev[5], s);
   } else {
     e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->acc, m->omega[S], m->alpha, m->type[t],
-                            S_radius, m->cell_start, m->cell_end, m->pressure, n, ctx->prefetch);
+                            S_radius, m->cell_start, m->cell_end, m->pressure, n, ctx->prefetch,
+                            control ? &co : nullptr);
     if (e != hipSuccess) return e;
+    if (control) {
+      e = dem::launch_control_finish(s, co, m->react_bx, m->react_by, m->react_bm, n, ctx->d_reaction, ctx->d_control,
+                                     true);
+      if (e != hipSuccess) return e;
+    }
     if (ev) (void)hipEventRecord(ev[5], s);
     // integrate from the sorted buffer back to the home buffer
     e = dem::launch_integrate_to(c, s, ctx->bs_integrate, m->pos[S], m->vel[S], m->theta[S], m->omega[S], m->pos[H],
@@ -624,6 +648,13 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   if (!r) r = sim_alloc(m, &m->acc, n);
   if (!r) r = sim_alloc(m, &m->alpha, n);
   if (!r) r = sim_alloc(m, &m->pressure, n);
+  if (!r) r = sim_alloc(m, &m->react_x, n);
+  if (!r) r = sim_alloc(m, &m->react_y, n);
+  if (!r) r = sim_alloc(m, &m->react_m, n);
+  const uint32_t react_blocks = (n + dem::kReactBlock - 1u) / dem::kReactBlock;
+  if (!r) r = sim_alloc(m, &m->react_bx, react_blocks);
+  if (!r) r = sim_alloc(m, &m->react_by, react_blocks);
+  if (!r) r = sim_alloc(m, &m->react_bm, react_blocks);
   if (!r) r = sim_alloc(m, &m->key, n);
   if (!r) r = sim_alloc(m, &m->idx, n);
   if (!r) r = sim_alloc(m, &m->cell_start, num_cells);
@@ -925,6 +956,99 @@ int dem_sim_gather_rows(dem_sim* m, const uint32_t* follow_ids, uint32_t count, 
                          (const float*)m->vel[kHome], (const float*)m->acc, m->theta[kHome], m->omega[kHome], m->alpha,
                          m->id[c], m->type[c], m->pressure,
                          m->n, rows);
+}
+
+int dem_set_controlled(dem_context* ctx, const dem_controlled* state) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!state) return fail(ctx, DEM_ERR_INVALID, "state is NULL");
+  if (state->enabled) {
+    if (multi(ctx)) return fail(ctx, DEM_ERR_INVALID, "the controlled particle needs a single instance");
+    if (state->type >= ctx->c.num_types) return fail(ctx, DEM_ERR_INVALID, "controlled particle type out of range");
+  }
+  if (int r = ensure_device(ctx)) return r;
+  if (!ctx->d_control) {
+    // the state, then the 3-float reaction, in one allocation
+    DEM_HIP(ctx, hipMalloc((void**)&ctx->d_control, sizeof(dem_controlled) + 4 * sizeof(float)),
+            "hipMalloc(controlled)");
+    ctx->d_reaction = (float*)(ctx->d_control + 1);
+  }
+  const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  DEM_HIP(ctx, hipMemcpyAsync(ctx->d_control, state, sizeof(dem_controlled), hipMemcpyHostToDevice, ctx->stream),
+          "controlled upload");
+  DEM_HIP(ctx, hipMemcpyAsync(ctx->d_reaction, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream),
+          "controlled upload");
+  DEM_HIP(ctx, hipStreamSynchronize(ctx->stream), "controlled upload");
+  if ((ctx->h_control.enabled != 0) != (state->enabled != 0)) ++ctx->generation;  // step kernels change
+  ctx->h_control = *state;
+  return DEM_OK;
+}
+
+int dem_get_controlled(dem_context* ctx, dem_controlled* state, float reaction[3]) {
+  if (int r = check_ctx(ctx)) return r;
+  if (int r = ensure_device(ctx)) return r;
+  DEM_HIP(ctx, hipStreamSynchronize(ctx->stream), "controlled download");
+  if (!ctx->d_control) {
+    if (state) *state = ctx->h_control;
+    if (reaction) reaction[0] = reaction[1] = reaction[2] = 0.0f;
+    return DEM_OK;
+  }
+  if (state) DEM_HIP(ctx, hipMemcpy(state, ctx->d_control, sizeof(dem_controlled), hipMemcpyDeviceToHost),
+                     "controlled download");
+  if (reaction) DEM_HIP(ctx, hipMemcpy(reaction, ctx->d_reaction, 3 * sizeof(float), hipMemcpyDeviceToHost),
+                        "controlled download");
+  return DEM_OK;
+}
+
+int dem_collide_controlled(dem_context* ctx, const float* pos, const float* vel, float* acc, const float* omega,
+                           float* alpha, const uint32_t* type, const uint32_t* cellStart, const uint32_t* cellEnd,
+                           uint32_t numParticles, uint32_t numCells, const float control_state[6],
+                           uint32_t control_type, float* forceVecX, float* blockX, float* forceVecY, float* blockY,
+                           float* momentVec, float* blockM, float* ctrlF, float* ctrlM, float* pressure) {
+  if (int r = check_ctx(ctx)) return r;
+  if (int r = check_cells(ctx, numCells)) return r;
+  if (!control_state || !ctrlF || !ctrlM) return fail(ctx, DEM_ERR_INVALID, "NULL control argument");
+  if (multi(ctx)) return fail(ctx, DEM_ERR_INVALID, "the controlled particle needs a single instance");
+  if (control_type >= ctx->c.num_types) return fail(ctx, DEM_ERR_INVALID, "control_type out of range");
+  if (numParticles && (!pos || !vel || !acc || !omega || !alpha || !type || !cellStart || !cellEnd || !pressure ||
+                       !forceVecX || !forceVecY || !momentVec || !blockX || !blockY || !blockM))
+    return fail(ctx, DEM_ERR_INVALID, "NULL array");
+  if (int r = ensure_device(ctx)) return r;
+  if (!ctx->d_stage_control) {
+    DEM_HIP(ctx, hipMalloc((void**)&ctx->d_stage_control, sizeof(dem_controlled) + 4 * sizeof(float)),
+            "hipMalloc(controlled)");
+    ctx->d_stage_reaction = (float*)(ctx->d_stage_control + 1);
+  }
+  dem_controlled st{};
+  st.pos[0] = control_state[0];
+  st.pos[1] = control_state[1];
+  st.vel[0] = control_state[2];
+  st.vel[1] = control_state[3];
+  st.theta = control_state[4];
+  st.omega = control_state[5];
+  st.type = control_type;
+  st.enabled = 1;
+  DEM_HIP(ctx, hipMemcpyAsync(ctx->d_stage_control, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream),
+          "controlled upload");
+  ControlOut co;
+  co.state = ctx->d_stage_control;
+  co.react_x = forceVecX;
+  co.react_y = forceVecY;
+  co.react_m = momentVec;
+  DEM_HIP(ctx, dem::launch_collide(ctx->c, ctx->stream, ctx->bs_collide, (const float2*)pos, (const float2*)vel,
+                                   (float2*)acc, omega, alpha, type, nullptr, cellStart, cellEnd, pressure,
+                                   numParticles, false, &co),
+          "collide kernel");
+  DEM_HIP(ctx, dem::launch_control_finish(ctx->stream, co, blockX, blockY, blockM, numParticles,
+                                          ctx->d_stage_reaction, ctx->d_stage_control, false),
+          "reaction kernel");
+  float r[3];
+  DEM_HIP(ctx, hipMemcpyAsync(r, ctx->d_stage_reaction, sizeof(r), hipMemcpyDeviceToHost, ctx->stream),
+          "reaction download");
+  DEM_HIP(ctx, hipStreamSynchronize(ctx->stream), "reaction download");
+  ctrlF[0] = r[0];
+  ctrlF[1] = r[1];
+  ctrlM[0] = r[2];
+  return DEM_OK;
 }
 
 int dem_set_contact_params(dem_context* ctx, const double* p, uint32_t num_types) {

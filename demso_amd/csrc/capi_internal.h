@@ -27,6 +27,15 @@ struct dem_context {
   bool fuse = false;      // owned simulation: collide + integrate as one kernel (chosen by dem_sim_autotune)
   bool prefetch = false;  // collide fetches each candidate one iteration ahead (chosen by dem_sim_autotune)
   uint64_t generation = 1;  // bumped by every parameter change
+  // Controlled particle of the owned simulations: host copy of the last
+  // state set, and the device state (moved by every step) followed by the
+  // last step's reaction (3 floats).  d_stage_control is the same pair for
+  // dem_collide_controlled, so that stage calls leave the owned state alone.
+  dem_controlled h_control{};
+  dem_controlled* d_control = nullptr;
+  float* d_reaction = nullptr;
+  dem_controlled* d_stage_control = nullptr;
+  float* d_stage_reaction = nullptr;
 };
 
 // Buffer of the owned simulation's dynamic fields (pos, vel, theta, omega)
@@ -59,6 +68,10 @@ struct dem_sim {
   hipGraphExec_t exec[2] = {nullptr, nullptr};      // two radix-sorted steps, by starting buffer
   hipGraphExec_t exec_inc[2] = {nullptr, nullptr};  // two re-sorted steps, by starting buffer
   uint64_t graph_gen = 0;
+  // reaction of each sorted particle on the controlled particle, and its
+  // per-block sums (ceil(n / kReactBlock) each)
+  float *react_x = nullptr, *react_y = nullptr, *react_m = nullptr;
+  float *react_bx = nullptr, *react_by = nullptr, *react_bm = nullptr;
   std::vector<void*> allocs;
 };
 

@@ -45,6 +45,16 @@ struct StepConst {
   Instance inst0;
 };
 
+// The controlled particle's state (device) and the per-particle reaction
+// vectors the pair with it writes.
+struct ControlOut {
+  const dem_controlled* state = nullptr;
+  float* react_x = nullptr;
+  float* react_y = nullptr;
+  float* react_m = nullptr;
+};
+constexpr uint32_t kReactBlock = 512;  // the reference's reduction block (src/functions.cu:475)
+
 // Launch helpers (defined in the .hip files).
 hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float2* pos,
                        const uint32_t* type, uint32_t* hash, uint32_t* index, uint32_t n);
@@ -83,7 +93,8 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
-                          const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch = false);
+                          const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch = false,
+                          const struct ControlOut* ctl = nullptr);
 // The next step's hash, fused into integrate: key[i] = cell key of the new
 // position, bit i of `bits` set when it differs from prev_key[i] (the keys
 // the state is sorted by), tile_count[i / kResortTile] += movers (the
@@ -112,7 +123,13 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
                                     const uint32_t* type, const float* radius, const uint32_t* cell_start,
                                     const uint32_t* cell_end, float2* acc, float* alpha, float* pressure,
                                     float2* pos_out, float2* vel_out, float* omega_out, float* theta_out,
-                                    const NextKeys& next, uint32_t n, bool prefetch = false);
+                                    const NextKeys& next, uint32_t n, bool prefetch = false,
+                                    const struct ControlOut* ctl = nullptr);
+// The controlled particle's reaction: per-block tree sums of the three
+// per-particle reaction vectors into bx/by/bm (ceil(n/kReactBlock) each),
+// their ordered sum into reaction[3], then (move) its kinematic move.
+hipError_t launch_control_finish(hipStream_t s, const struct ControlOut& ctl, float* bx, float* by, float* bm,
+                                 uint32_t n, float* reaction, dem_controlled* state, bool move);
 
 // Radix sort of (key, value) pairs, stable, ascending, on the low key_bits.
 struct SortScratch {

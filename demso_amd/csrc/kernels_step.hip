@@ -191,6 +191,72 @@ struct PairCtx {
   Material mA;
 };
 
+// collideSpheres' contact law (src/particles_kernel.cuh:255-303) for a
+// partner B of material tB at offset (rx, ry) and distance dist < R: the
+// force on A as collideSpheres returns it (ffx, ffy), its pressure term, and
+// the friction moment added to `moment`.
+__device__ __forceinline__ void contact_force(const StepConst& c, const PairCtx& q, float rx, float ry, float dist,
+                                              float R, float rB, uint32_t tB, float2 vB, float wB, float& ffx,
+                                              float& ffy, float& moment, float& pres) {
+  const float kBn = c.mat[tB].kn, kBt = c.mat[tB].kt, cBn = c.mat[tB].cn, muB = c.mat[tB].mu;
+  const float nx = rx / dist, ny = ry / dist;
+  const float tx = -ny, ty = nx;
+  const float rvx = vB.x - q.v.x, rvy = vB.y - q.v.y;
+  const float vn = rvx * nx + rvy * ny;
+  const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
+  const float kn = series(q.mA.kn, kBn);
+  const float kt = series(q.mA.kt, kBt);
+  const float cn = series(q.mA.cn, cBn);
+  const float mu = (q.mA.mu + muB) / 2;
+  const float s = -kn * (R - dist);
+  ffx = s * nx;
+  ffy = s * ny;
+  if (vn < 0.0f) {
+    const float s2 = cn * vn;
+    ffx += s2 * nx;
+    ffy += s2 * ny;
+  }
+  const float flen = sqrtf(ffx * ffx + ffy * ffy);
+  pres = add_pressure(pres, flen, q.den);
+  if (mu != 0.0f) {
+    const float sr = q.rA * q.w + rB * wB;
+    const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
+    float ftx = (kt * cvx) * q.dt, fty = (kt * cvy) * q.dt;
+    const float ftmax = mu * flen;
+    if (!(sqrtf(ftx * ftx + fty * fty) < ftmax)) {
+      const float cl = sqrtf(cvx * cvx + cvy * cvy);
+      ftx = (ftmax * cvx) / cl;
+      fty = (ftmax * cvy) / cl;
+    }
+    ffx += ftx;
+    ffy += fty;
+    moment += q.rA * (ftx * tx + fty * ty);
+  }
+}
+
+// attractSpheres (src/particles_kernel.cuh:310-342) for a partner with
+// attraction coefficient aB: the force on A and its pressure term.
+__device__ __forceinline__ void attract_force(const PairCtx& q, float rx, float ry, float dist, float rB, float aB,
+                                              float& afx, float& afy, float& pres) {
+  const float aA = q.mA.attract;
+  const float adA = (float)((double)(q.rA * 2.0f) * 1.2);
+  const float adB = (float)((double)(rB * 2.0f) * 1.2);
+  const float nx = rx / dist, ny = ry / dist;
+  afx = 0.0f;
+  afy = 0.0f;
+  if (dist < adA) {
+    const float s = aA * (adA - dist);
+    afx += s * nx;
+    afy += s * ny;
+  }
+  if (dist < adB) {
+    const float s = aB * (adB - dist);
+    afx += s * nx;
+    afy += s * ny;
+  }
+  pres = add_pressure(pres, sqrtf(afx * afx + afy * afy), q.den);
+}
+
 template <bool kRadius>
 __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, float2 pB, float rB,
                                              uint32_t tB, const float2* __restrict__ vel,
@@ -204,64 +270,49 @@ __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& 
   const float dist = sqrtf(d2);
   if (kRadius) tB = clamp_idx(type[j], c.num_types);
   if (dist < R) {
-    const float kBn = c.mat[tB].kn, kBt = c.mat[tB].kt, cBn = c.mat[tB].cn, muB = c.mat[tB].mu;
-    const float2 vB = vel[j];
-    const float wB = omega[j];
-    const float nx = rx / dist, ny = ry / dist;
-    const float tx = -ny, ty = nx;
-    const float rvx = vB.x - q.v.x, rvy = vB.y - q.v.y;
-    const float vn = rvx * nx + rvy * ny;
-    const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
-    const float kn = series(q.mA.kn, kBn);
-    const float kt = series(q.mA.kt, kBt);
-    const float cn = series(q.mA.cn, cBn);
-    const float mu = (q.mA.mu + muB) / 2;
-    const float s = -kn * (R - dist);
-    float ffx = s * nx, ffy = s * ny;
-    if (vn < 0.0f) {
-      const float s2 = cn * vn;
-      ffx += s2 * nx;
-      ffy += s2 * ny;
-    }
-    const float flen = sqrtf(ffx * ffx + ffy * ffy);
-    pres = add_pressure(pres, flen, q.den);
-    if (mu != 0.0f) {
-      const float sr = q.rA * q.w + rB * wB;
-      const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
-      float ftx = (kt * cvx) * q.dt, fty = (kt * cvy) * q.dt;
-      const float ftmax = mu * flen;
-      if (!(sqrtf(ftx * ftx + fty * fty) < ftmax)) {
-        const float cl = sqrtf(cvx * cvx + cvy * cvy);
-        ftx = (ftmax * cvx) / cl;
-        fty = (ftmax * cvy) / cl;
-      }
-      ffx += ftx;
-      ffy += fty;
-      moment += q.rA * (ftx * tx + fty * ty);
-    }
+    float ffx, ffy;
+    contact_force(c, q, rx, ry, dist, R, rB, tB, vel[j], omega[j], ffx, ffy, moment, pres);
     cfx += ffx;
     cfy += ffy;
   }
   if (c.attract) {
-    const float aA = q.mA.attract, aB = c.mat[tB].attract;
-    const float adA = (float)((double)(q.rA * 2.0f) * 1.2);
-    const float adB = (float)((double)(rB * 2.0f) * 1.2);
-    const float nx = rx / dist, ny = ry / dist;
-    float afx = 0.0f, afy = 0.0f;
-    if (dist < adA) {
-      const float s = aA * (adA - dist);
-      afx += s * nx;
-      afy += s * ny;
-    }
-    if (dist < adB) {
-      const float s = aB * (adB - dist);
-      afx += s * nx;
-      afy += s * ny;
-    }
-    pres = add_pressure(pres, sqrtf(afx * afx + afy * afy), q.den);
+    float afx, afy;
+    attract_force(q, rx, ry, dist, rB, c.mat[tB].attract, afx, afy, pres);
     cfx += afx;
     cfy += afy;
   }
+}
+
+// The pair with the controlled particle (src/particles_kernel.cuh:527-541,
+// USE_ATOMIC 0): f = collideSpheres + attractSpheres, summed on its own and
+// then added to the particle's force and moment; the reaction on the
+// controlled particle, -f and -m / rA * rC, goes to per-particle vectors.
+__device__ __forceinline__ void control_pair(const StepConst& c, const PairCtx& q, const dem_controlled& C,
+                                             float& fx, float& fy, float& moment, float& pres, float& react_x,
+                                             float& react_y, float& react_m) {
+  const uint32_t tC = clamp_idx(C.type, c.num_types);
+  const float rC = c.mat[tC].radius;
+  const float rx = C.pos[0] - q.p.x, ry = C.pos[1] - q.p.y;
+  const float dist = sqrtf(rx * rx + ry * ry);
+  const float R = q.rA + rC;
+  float m = 0.0f, f_x = 0.0f, f_y = 0.0f;
+  if (dist < R) {
+    float mterm = 0.0f;
+    contact_force(c, q, rx, ry, dist, R, rC, tC, make_float2(C.vel[0], C.vel[1]), C.omega, f_x, f_y, mterm, pres);
+    m = m + mterm;  // collideSpheres adds into the caller's m (0)
+  }
+  if (c.attract) {
+    float afx, afy;
+    attract_force(q, rx, ry, dist, rC, c.mat[tC].attract, afx, afy, pres);
+    f_x += afx;
+    f_y += afy;
+  }
+  fx += f_x;
+  fy += f_y;
+  moment += m;
+  react_x = -f_x;
+  react_y = -f_y;
+  react_m = -m / q.rA * rC;
 }
 
 // Candidate j's position and radius (and, without a radius array, its
@@ -306,13 +357,14 @@ struct Own {
   float w;
 };
 
-template <bool kRadius, bool kMulti, bool kPrefetch>
+template <bool kRadius, bool kMulti, bool kPrefetch, bool kControl>
 __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i, const float2* __restrict__ pos,
                                                  const float2* __restrict__ vel, const float* __restrict__ omega,
                                                  const uint32_t* __restrict__ type, const float* __restrict__ radius,
                                                  const uint32_t* __restrict__ cell_start,
                                                  const uint32_t* __restrict__ cell_end, uint32_t n, Own& own,
-                                                 float& fx_out, float& fy_out, float& moment_out, float& pres_out) {
+                                                 const ControlOut& ctl, float& fx_out, float& fy_out,
+                                                 float& moment_out, float& pres_out) {
   PairCtx q;
   own.t = clamp_idx(type[i], c.num_types);
   q.mA = c.mat[own.t];
@@ -433,25 +485,32 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       }
     }
   }
+  if (kControl) {
+    float rx, ry, rm;
+    control_pair(c, q, *ctl.state, fx, fy, moment, pres, rx, ry, rm);
+    ctl.react_x[i] = rx;
+    ctl.react_y[i] = ry;
+    ctl.react_m[i] = rm;
+  }
   fx_out = fx;
   fy_out = fy;
   moment_out = moment;
   pres_out = pres;
 }
 
-template <bool kRadius, bool kMulti, bool kPrefetch>
+template <bool kRadius, bool kMulti, bool kPrefetch, bool kControl>
 __global__ __launch_bounds__(256) void k_collide(
     StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,
     float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha,
     const uint32_t* __restrict__ type, const float* __restrict__ radius,
     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
-    float* __restrict__ pressure, uint32_t n) {
+    float* __restrict__ pressure, ControlOut ctl, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
-  collide_particle<kRadius, kMulti, kPrefetch>(c, i, pos, vel, omega, type, radius, cell_start, cell_end, n, own, fx,
-                                               fy, moment, pres);
+  collide_particle<kRadius, kMulti, kPrefetch, kControl>(c, i, pos, vel, omega, type, radius, cell_start, cell_end, n,
+                                                         own, ctl, fx, fy, moment, pres);
   acc[i] = make_float2(fx / own.mass, fy / own.mass);
   alpha[i] = moment / own.inertia;
   pressure[i] = pres;
@@ -605,20 +664,20 @@ __global__ __launch_bounds__(256) void k_integrate(
 // *_out arrays (the unsorted buffer of this step, dead after reorder), so no
 // lane overwrites a neighbour's input.  The arithmetic is that of k_collide
 // followed by k_integrate.
-template <bool kRadius, bool kMulti, bool kPrefetch>
+template <bool kRadius, bool kMulti, bool kPrefetch, bool kControl>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_collide_integrate(
     StepConst c, const float2* __restrict__ pos_in, const float2* __restrict__ vel_in,
     const float* __restrict__ omega_in, const float* __restrict__ theta_in, const uint32_t* __restrict__ type,
     const float* __restrict__ radius, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float2* __restrict__ acc, float* __restrict__ alpha,
     float* __restrict__ pressure, float2* __restrict__ pos_out, float2* __restrict__ vel_out,
-    float* __restrict__ omega_out, float* __restrict__ theta_out, NextKeys nk, uint32_t n) {
+    float* __restrict__ omega_out, float* __restrict__ theta_out, NextKeys nk, ControlOut ctl, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
-  collide_particle<kRadius, kMulti, kPrefetch>(c, i, pos_in, vel_in, omega_in, type, radius, cell_start, cell_end, n,
-                                               own, fx, fy, moment, pres);
+  collide_particle<kRadius, kMulti, kPrefetch, kControl>(c, i, pos_in, vel_in, omega_in, type, radius, cell_start,
+                                                         cell_end, n, own, ctl, fx, fy, moment, pres);
   const float2 a = make_float2(fx / own.mass, fy / own.mass);
   const float al = moment / own.inertia;
   acc[i] = a;
@@ -675,26 +734,49 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
   return hipGetLastError();
 }
 
+// Kernel variant dispatch: kRadius x kMulti x (prefetch, control), where
+// the controlled particle always uses the variant without prefetch.
+#define DEM_DISPATCH(MACRO)                                              \
+  do {                                                                   \
+    if (ctl) {                                                           \
+      if (radius) {                                                      \
+        if (multi) MACRO(true, true, false, true);                       \
+        else MACRO(true, false, false, true);                            \
+      } else {                                                           \
+        if (multi) MACRO(false, true, false, true);                      \
+        else MACRO(false, false, false, true);                           \
+      }                                                                  \
+    } else if (prefetch) {                                               \
+      if (radius) {                                                      \
+        if (multi) MACRO(true, true, true, false);                       \
+        else MACRO(true, false, true, false);                            \
+      } else {                                                           \
+        if (multi) MACRO(false, true, true, false);                      \
+        else MACRO(false, false, true, false);                           \
+      }                                                                  \
+    } else {                                                             \
+      if (radius) {                                                      \
+        if (multi) MACRO(true, true, false, false);                      \
+        else MACRO(true, false, false, false);                           \
+      } else {                                                           \
+        if (multi) MACRO(false, true, false, false);                     \
+        else MACRO(false, false, false, false);                          \
+      }                                                                  \
+    }                                                                    \
+  } while (0)
+
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
-                          const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch) {
+                          const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch,
+                          const ControlOut* ctl) {
   if (n == 0) return hipSuccess;
   const bool multi = c.num_instances > 1;
-#define DEM_COLLIDE(R, M, P)                                                                        \
-  hipLaunchKernelGGL((k_collide<R, M, P>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
-                     omega, alpha, type, radius, cell_start, cell_end, pressure, n)
-#define DEM_COLLIDE_P(R, M)               \
-  do {                                    \
-    if (prefetch) DEM_COLLIDE(R, M, true); \
-    else DEM_COLLIDE(R, M, false);        \
-  } while (0)
-  if (radius) {
-    if (multi) DEM_COLLIDE_P(true, true); else DEM_COLLIDE_P(true, false);
-  } else {
-    if (multi) DEM_COLLIDE_P(false, true); else DEM_COLLIDE_P(false, false);
-  }
-#undef DEM_COLLIDE_P
+  const ControlOut co = ctl ? *ctl : ControlOut{};
+#define DEM_COLLIDE(R, M, P, X)                                                                        \
+  hipLaunchKernelGGL((k_collide<R, M, P, X>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
+                     omega, alpha, type, radius, cell_start, cell_end, pressure, co, n)
+  DEM_DISPATCH(DEM_COLLIDE);
 #undef DEM_COLLIDE
   return hipGetLastError();
 }
@@ -704,26 +786,80 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
                                     const uint32_t* type, const float* radius, const uint32_t* cell_start,
                                     const uint32_t* cell_end, float2* acc, float* alpha, float* pressure,
                                     float2* pos_out, float2* vel_out, float* omega_out, float* theta_out,
-                                    const NextKeys& next, uint32_t n, bool prefetch) {
+                                    const NextKeys& next, uint32_t n, bool prefetch, const ControlOut* ctl) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
   const bool multi = c.num_instances > 1;
-#define DEM_CI(R, M, P)                                                                                     \
-  hipLaunchKernelGGL((k_collide_integrate<R, M, P>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
-                     omega_in, theta_in, type, radius, cell_start, cell_end, acc, alpha, pressure, pos_out,     \
-                     vel_out, omega_out, theta_out, next, n)
-#define DEM_CI_P(R, M)               \
-  do {                               \
-    if (prefetch) DEM_CI(R, M, true); \
-    else DEM_CI(R, M, false);        \
-  } while (0)
-  if (radius) {
-    if (multi) DEM_CI_P(true, true); else DEM_CI_P(true, false);
-  } else {
-    if (multi) DEM_CI_P(false, true); else DEM_CI_P(false, false);
-  }
-#undef DEM_CI_P
+  const ControlOut co = ctl ? *ctl : ControlOut{};
+#define DEM_CI(R, M, P, X)                                                                                      \
+  hipLaunchKernelGGL((k_collide_integrate<R, M, P, X>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
+                     omega_in, theta_in, type, radius, cell_start, cell_end, acc, alpha, pressure, pos_out,        \
+                     vel_out, omega_out, theta_out, next, co, n)
+  DEM_DISPATCH(DEM_CI);
 #undef DEM_CI
+  return hipGetLastError();
+}
+#undef DEM_DISPATCH
+
+// ---- controlled particle: reaction sum and move ------------------------------
+// reduceVecD (src/particles_kernel.cuh:552-578) on the three reaction
+// vectors: block b sums values [512 b, 512 b + 512) (zeros past n) by the
+// same pairwise tree.
+__global__ __launch_bounds__(kReactBlock) void k_reduce_reaction(const float* __restrict__ rx,
+                                                                const float* __restrict__ ry,
+                                                                const float* __restrict__ rm, float* __restrict__ bx,
+                                                                float* __restrict__ by, float* __restrict__ bm,
+                                                                uint32_t n) {
+  __shared__ float cx[kReactBlock], cy[kReactBlock], cm[kReactBlock];
+  const uint32_t t = threadIdx.x, i = blockIdx.x * kReactBlock + t;
+  cx[t] = i < n ? rx[i] : 0.0f;
+  cy[t] = i < n ? ry[i] : 0.0f;
+  cm[t] = i < n ? rm[i] : 0.0f;
+  __syncthreads();
+  for (uint32_t h = kReactBlock / 2; h != 0; h /= 2) {
+    if (t < h) {
+      cx[t] += cx[t + h];
+      cy[t] += cy[t + h];
+      cm[t] += cm[t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    bx[blockIdx.x] = cx[0];
+    by[blockIdx.x] = cy[0];
+    bm[blockIdx.x] = cm[0];
+  }
+}
+
+// The block sums added in order (src/functions.cu:505-513, there on the
+// host), one lane per quantity, then the kinematic move of the controlled
+// particle (src/main.cu:510-514) when `move` is set.
+__global__ __launch_bounds__(64) void k_control_finish(const float* __restrict__ bx, const float* __restrict__ by,
+                                                      const float* __restrict__ bm, uint32_t blocks,
+                                                      float* __restrict__ reaction, dem_controlled* state, int move) {
+  const uint32_t t = threadIdx.x;
+  if (t < 3) {
+    const float* b = t == 0 ? bx : (t == 1 ? by : bm);
+    float acc = 0.0f;
+    for (uint32_t k = 0; k < blocks; ++k) acc += b[k];
+    reaction[t] = acc;
+  }
+  if (t == 0 && move) {
+    dem_controlled& C = *state;
+    C.pos[0] = (float)((double)C.pos[0] + C.move[0]);
+    C.pos[1] = (float)((double)C.pos[1] + C.move[1]);
+    if ((double)C.pos[1] < C.wrap_below_y) C.pos[1] = C.reset_y;
+    if ((double)C.pos[0] > C.wrap_above_x) C.pos[0] = C.reset_x;
+  }
+}
+
+hipError_t launch_control_finish(hipStream_t s, const ControlOut& ctl, float* bx, float* by, float* bm, uint32_t n,
+                                 float* reaction, dem_controlled* state, bool move) {
+  const uint32_t blocks = (n + kReactBlock - 1u) / kReactBlock;
+  if (n) hipLaunchKernelGGL(k_reduce_reaction, dim3(blocks), dim3(kReactBlock), 0, s, ctl.react_x, ctl.react_y,
+                            ctl.react_m, bx, by, bm, n);
+  hipLaunchKernelGGL(k_control_finish, dim3(1), dim3(64), 0, s, bx, by, bm, n ? blocks : 0u, reaction, state,
+                     move ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -26,8 +26,12 @@
 //   --dump-final       write a state dump after the last step
 //   --dump-prefix P    dump file prefix (default "state_"); files P<step>.bin
 //   --seed S           seed of the rectangles' jitter (default 0)
-//   --controlled       read <particles> as the controlled-particle build does
-//                      (tag required, its type left out of the grid sizing)
+//   --controlled       the controlled-particle build (USE_BIG_PARTICLE): the
+//                      <controlled> tag is required, its type is left out of
+//                      the grid sizing, and the particle starts at (5, 9.5)
+//                      and moves down 0.005 per step (src/main.cu:141-149,
+//                      510-514); its final position and the last step's
+//                      reaction on it are printed at the end
 //   --device D         HIP device (default 0)
 //   --print-config     print the loaded scene and exit (no GPU)
 //   --dump-initial F   write the prepared initial state to F and exit (no GPU)
@@ -202,6 +206,12 @@ void write_line(FILE* out, float elapsed, const uint32_t* rows, size_t count) {
   std::fprintf(out, "\n");
 }
 
+// Final state of the controlled particle; %a keeps every bit.
+void print_controlled(float x, float y, float fx, float fy, float m) {
+  std::printf("controlled particle: pos %a %a reaction %a %a %a\n", (double)x, (double)y, (double)fx, (double)fy,
+              (double)m);
+}
+
 #define HIP_OK(call)                                                                                  \
   do {                                                                                                \
     const hipError_t e_ = (call);                                                                     \
@@ -345,6 +355,21 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
     HIP_OK(b.get(&b.index, n));
     HIP_OK(b.get(&b.cellStart, ps.sys.num_cells));
     HIP_OK(b.get(&b.cellEnd, ps.sys.num_cells));
+    // the controlled particle's reaction vectors and block sums
+    // (src/functions.cu:104-115)
+    const bool control = ps.controlled_type >= 0;
+    const uint32_t react_blocks = (n + 511u) / 512u;
+    float *cfx = nullptr, *cfy = nullptr, *cm = nullptr, *hx = nullptr, *hy = nullptr, *hm = nullptr;
+    if (control) {
+      HIP_OK(b.get(&cfx, n));
+      HIP_OK(b.get(&cfy, n));
+      HIP_OK(b.get(&cm, n));
+      HIP_OK(b.get(&hx, react_blocks));
+      HIP_OK(b.get(&hy, react_blocks));
+      HIP_OK(b.get(&hm, react_blocks));
+    }
+    Float2 ctrl_pos{5.0f, 9.5f}, ctrl_vel{0.0f, 0.0f}, ctrl_f{0.0f, 0.0f};
+    float ctrl_m = 0.0f;
     HIP_OK(hipMemset(b.cellEnd, 0, ps.sys.num_cells * 4));  // zeroed once (src/functions.cu:140)
     HIP_OK(hipMemset(b.acc, 0, 8 * (size_t)n));
     HIP_OK(hipMemset(b.alpha, 0, 4 * (size_t)n));
@@ -365,10 +390,21 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
       reorderDataAndFindCellStart(b.cellStart, b.cellEnd, b.pos[t], b.vel[t], b.theta[t], b.omega[t], b.id[t],
                                   b.loc[t], b.type[t], b.hash, b.index, b.pos[o], b.vel[o], b.theta[o], b.omega[o],
                                   b.id[o], b.type[o], n, ps.sys.num_cells);
-      collide(b.pos[t], b.vel[t], b.acc, b.omega[t], b.alpha, b.type[t], b.cellStart, b.cellEnd, n,
-              ps.sys.num_cells, b.pressure);
+      if (control)
+        collide(b.pos[t], b.vel[t], b.acc, b.omega[t], b.alpha, b.type[t], b.cellStart, b.cellEnd, n,
+                ps.sys.num_cells, ctrl_pos, ctrl_vel, 0.0f, 0.0f, (unsigned)ps.controlled_type, cfx, hx, cfy, hy, cm,
+                hm, &ctrl_f, &ctrl_m, b.pressure);
+      else
+        collide(b.pos[t], b.vel[t], b.acc, b.omega[t], b.alpha, b.type[t], b.cellStart, b.cellEnd, n,
+                ps.sys.num_cells, b.pressure);
       integrateSystem(b.pos[t], b.vel[t], b.acc, b.theta[t], b.omega[t], b.alpha, b.type[t], b.pressure, n);
       if (demso_shim::last_status() != DEM_OK) return 1;
+      if (control) {
+        // src/main.cu:510-514
+        ctrl_pos.y += -.005;
+        if (ctrl_pos.y < -1) ctrl_pos.y = 10.5;
+        if (ctrl_pos.x > sis.cubeDimension.x + 0.5) ctrl_pos.x = -0.5;
+      }
       cur = t;
       if (run.d_rows) {
         if (run.pending == 0) run.first_pending_step = tempo;
@@ -396,6 +432,7 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
     }
     if (run.flush_rows(s)) return 1;
     HIP_OK(hipDeviceSynchronize());
+    if (control) print_controlled(ctrl_pos.x, ctrl_pos.y, ctrl_f.x, ctrl_f.y, ctrl_m);
     return 0;
   }
 
@@ -419,6 +456,19 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
   DEM_OK_OR_FAIL(dem_sim_upload(sim, ps.pos.data(), ps.vel.data(), ps.theta.data(), ps.omega.data(), ps.id.data(),
                                 ps.type.data(), ps.loc.data(), nullptr),
                  ctx);
+  if (ps.controlled_type >= 0) {
+    dem_controlled c{};
+    c.pos[0] = 5.0f;
+    c.pos[1] = 9.5f;
+    c.type = (uint32_t)ps.controlled_type;
+    c.enabled = 1;
+    c.move[1] = -.005;
+    c.wrap_below_y = -1.0;
+    c.wrap_above_x = (double)ps.sys.cube_dimension[0] + 0.5;
+    c.reset_y = 10.5f;
+    c.reset_x = -0.5f;
+    DEM_OK_OR_FAIL(dem_set_controlled(ctx, &c), ctx);
+  }
   hipStream_t s = nullptr;
   long long done = 0;
   while (done < steps) {
@@ -447,6 +497,12 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
   }
   if (run.flush_rows(s)) return 1;
   DEM_OK_OR_FAIL(dem_synchronize(ctx), ctx);
+  if (ps.controlled_type >= 0) {
+    dem_controlled c{};
+    float r[3];
+    DEM_OK_OR_FAIL(dem_get_controlled(ctx, &c, r), ctx);
+    print_controlled(c.pos[0], c.pos[1], r[0], r[1], r[2]);
+  }
   return 0;
 }
 
@@ -477,7 +533,6 @@ int main(int argc, char** argv) {
   }
   std::printf("\nNumero de Particulas = %u\ngrid %u x %u\n\n", ps.sys.num_particles, ps.sys.grid_size[0],
               ps.sys.grid_size[1]);
-  if (opt.controlled) std::printf("note: the controlled particle is not simulated by this driver\n");
   for (int f : ps.missing_followed)
     std::fprintf(stderr, "warning: followed particle %d does not exist (%u particles); it is not written\n", f,
                  ps.sys.num_particles);

@@ -109,6 +109,7 @@ PreparedScene prepare_sim(const DEMSimulation& sim, uint64_t seed) {
     if (max_radius < out.types[i].radius) max_radius = out.types[i].radius;
   }
   out.max_radius = max_radius;
+  out.controlled_type = P.controlledType;
   reference_grid(s.cube_dimension[0], s.cube_dimension[1], max_radius, &s.grid_size[0], &s.grid_size[1]);
   s.num_cells = s.grid_size[0] * s.grid_size[1];
 

@@ -21,6 +21,9 @@ struct PreparedScene {
   std::vector<int> missing_followed;  // followed IDs beyond the particle count (not written)
   long long duration_steps = -1;  // simDuration / timeStep, -1 when absent (src/main.cu:158)
   float max_radius = 0.0f;
+  // material row of the controlled particle, -1 when the scene was not
+  // loaded as the controlled-particle build (src/main.cu:141-149)
+  int controlled_type = -1;
 };
 
 // SplitMix64 counter generator (Steele, Lea & Flood 2014): value i of stream
@@ -40,7 +43,7 @@ void reference_grid(float lx, float ly, float max_radius, uint32_t* gx, uint32_t
 // (src/main.cu:291, src/particles_kernel.cuh:65-69), which no run can
 // reproduce; here the two draws of particle p of rectangle r are
 // normal01(seed, 1000 + 2r, p) and normal01(seed, 1001 + 2r, p), in the same
-// formula.  The controlled particle itself is not simulated.
+// formula.
 PreparedScene prepare_sim(const DEMSimulation& sim, uint64_t seed);
 
 }  // namespace dsml

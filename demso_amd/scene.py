@@ -339,11 +339,15 @@ def triangle_pile(rows: int = 16, start: Tuple[float, float] = (5.0, 0.125), wid
                   radius: float = 0.125, mass: float = 1e-2, kn: float = 1000.0, kt: float = 1000.0,
                   cn: float = 0.2, bdamp: float = 0.5, mu: float = 0.6, attract: float = 10.0,
                   box: Tuple[float, float] = (10.0, 10.0), bkn: float = 1000.0, bkt: float = 100.0,
-                  dt: float = 1e-3, steps: int = 1000) -> Scene:
-    """The pile of exemplos/pilha.dsml without its controlled particle: one
-    particle type with table radius/mass (inertia 2/5 m r^2, src/main.cu:193),
-    the triangle layout of createTrianglesD (src/particles_kernel.cuh:76-111)
-    and the wrapping power-of-two grid of PrepareSim."""
+                  dt: float = 1e-3, steps: int = 1000, controlled: bool = False) -> Scene:
+    """The pile of exemplos/pilha.dsml: one particle type with table
+    radius/mass (inertia 2/5 m r^2, src/main.cu:193), the triangle layout of
+    createTrianglesD (src/particles_kernel.cuh:76-111) and the wrapping
+    power-of-two grid of PrepareSim.  controlled: add the scene's "MOVE"
+    type (material row 1) and note the controlled particle that DemSo's
+    default build drives with it (start (5, 9.5), 0.005 down per step,
+    src/main.cu:141-149, 510-514); the grid ignores that type's radius, as
+    PrepareSim does."""
     f32 = np.float32
     N = rows
     count = N * (N + 1) // 2
@@ -363,5 +367,13 @@ def triangle_pile(rows: int = 16, start: Tuple[float, float] = (5.0, 0.125), wid
                    attract=attract)
     grid = reference_grid_size(box, radius)
     st = _state(pos, None, np.zeros(count, np.uint32))
-    return Scene(name="triangle_pile", materials=[mat], dt=dt, bkn=bkn, bkt=bkt,
-                 instances=[Instance(box=box, extent=box, grid=grid)], flags=0, steps=steps, **st)
+    mats = [mat]
+    notes = {}
+    if controlled:
+        mc, rc = 100.0, 0.5
+        mats.append(Material(radius=rc, mass=mc, inertia=float(f32(2) * f32(mc) * f32(rc) * f32(rc) / f32(5)),
+                             kn=1e20, kt=1e20, cn=0.2, bdamp=0.5, mu=0.4, attract=0.0))
+        notes["controlled"] = dict(pos=(5.0, 9.5), type=1, move=(0.0, -0.005), wrap_below_y=-1.0,
+                                   wrap_above_x=float(f32(box[0])) + 0.5, reset_y=10.5, reset_x=-0.5)
+    return Scene(name="triangle_pile", materials=mats, dt=dt, bkn=bkn, bkt=bkt,
+                 instances=[Instance(box=box, extent=box, grid=grid)], flags=0, steps=steps, notes=notes, **st)

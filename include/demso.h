@@ -95,6 +95,27 @@ typedef struct dem_instance {
   uint32_t reserved;
 } dem_instance;
 
+/* The controlled ("big") particle of DemSo's default build (USE_BIG_PARTICLE,
+ * src/main.cuh:29): a disc of material row `type` that every particle
+ * collides with after its neighbours (src/particles_kernel.cuh:525-543) and
+ * that is moved kinematically after each step (src/main.cu:510-514): pos +=
+ * move (computed in double), then pos.y < wrap_below_y -> reset_y and
+ * pos.x > wrap_above_x -> reset_x.  DemSo's values: pos (5, 9.5), vel 0,
+ * move (0, -0.005), wrap_below_y -1 -> 10.5, wrap_above_x Lx + 0.5 -> -0.5. */
+typedef struct dem_controlled {
+  float pos[2];
+  float vel[2];
+  float theta;
+  float omega;
+  uint32_t type;
+  uint32_t enabled;          /* 0: no controlled particle */
+  double move[2];
+  double wrap_below_y;
+  double wrap_above_x;
+  float reset_y;
+  float reset_x;
+} dem_controlled;
+
 typedef struct dem_context dem_context;
 
 /* ---- context, parameters ------------------------------------------------ */
@@ -267,6 +288,27 @@ int dem_gather_rows(dem_context* ctx, const uint32_t* follow_ids, uint32_t count
                     uint32_t numParticles, uint32_t* rows);
 /* The same rows from the sim's current state. */
 int dem_sim_gather_rows(dem_sim* sim, const uint32_t* follow_ids, uint32_t count, uint32_t* rows);
+/* ---- controlled particle -------------------------------------------------- */
+/* Owned simulations of ctx: with state->enabled, every step adds the pair
+ * with the controlled particle to each particle's force, reduces the
+ * reaction on it (the reference's reduceVecD order: a pairwise tree over
+ * blocks of 512 sorted particles, the block sums added in order) and then
+ * moves it.  All on the device, graph-capturable. */
+int dem_set_controlled(dem_context* ctx, const dem_controlled* state);
+/* Current state of the controlled particle (after the steps run so far) and
+ * the last step's reaction: force x, y and moment on it.  Synchronizes. */
+int dem_get_controlled(dem_context* ctx, dem_controlled* state, float reaction[3]);
+/* collide with the controlled particle, the reference's USE_BIG_PARTICLE
+ * collide (src/functions.cu:398-521, USE_ATOMIC 0): control_state = (pos.x,
+ * pos.y, vel.x, vel.y, theta, omega) on the host; forceVecX/Y, momentVec
+ * (device, n floats) receive the per-particle reaction and blockX/Y/M
+ * (device, ceil(n/512) floats) its block sums; ctrlF (2) and ctrlM (1) on the
+ * host receive the totals.  Synchronizes, as the reference does. */
+int dem_collide_controlled(dem_context* ctx, const float* pos, const float* vel, float* acc, const float* omega,
+                           float* alpha, const uint32_t* type, const uint32_t* cellStart, const uint32_t* cellEnd,
+                           uint32_t numParticles, uint32_t numCells, const float control_state[6],
+                           uint32_t control_type, float* forceVecX, float* blockX, float* forceVecY, float* blockY,
+                           float* momentVec, float* blockM, float* ctrlF, float* ctrlM, float* pressure);
 /* Update the contact parameters of every material from a host array of
  * num_types rows (kn, kt, cn, mu), leaving the other fields unchanged. */
 int dem_set_contact_params(dem_context* ctx, const double* kn_kt_cn_mu, uint32_t num_types);

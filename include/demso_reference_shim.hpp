@@ -2,8 +2,9 @@
 //
 // Header-only C++ shim for the DemSo code base: the functions below have the
 // exact signatures of src/functions.cuh:50-72 (calcHash, sortParticles,
-// reorderDataAndFindCellStart, integrateSystem, the collide overload without
-// the controlled particle) and set_gravity (src/functions.cuh:83), and
+// reorderDataAndFindCellStart, integrateSystem, collide in both builds:
+// without the controlled particle and with it, USE_BIG_PARTICLE 1 /
+// USE_ATOMIC 0) and set_gravity (src/functions.cuh:83), and
 // forward to include/demso.h.  A maintainer compiles this in place of the
 // kernel launches of src/functions.cu.  The reference's hidden __constant__
 // state becomes one process-wide dem_context, created by demso_shim::init()
@@ -128,6 +129,28 @@ inline void collide(float* oldPos, float* oldVel, float* newAcc, float* oldOmega
   demso_shim::report(dem_collide(demso_shim::context(), oldPos, oldVel, newAcc, oldOmega, newAlpha, oldType, cellStart,
                                  cellEnd, numParticles, numCells, pressure),
                      "collide");
+}
+
+// collide as compiled with USE_BIG_PARTICLE 1, USE_ATOMIC 0
+// (src/functions.cu:398-521): the pair with the controlled particle, its
+// reaction per particle in controlForceVecX/Y and controlMomentVec, their
+// block sums in hCFVx/hCFVy/hCMV (device, ceil(n/512) each) and the totals in
+// *ctrlF, *ctrlM (host).  F2 is the reference's float2.
+template <class F2>
+inline void collide(float* oldPos, float* oldVel, float* newAcc, float* oldOmega, float* newAlpha, unsigned* oldType,
+                    unsigned* cellStart, unsigned* cellEnd, unsigned numParticles, unsigned numCells, F2 controlPos,
+                    F2 controlVel, float controlTheta, float controlOmega, unsigned controlType,
+                    float* controlForceVecX, float* hCFVx, float* controlForceVecY, float* hCFVy,
+                    float* controlMomentVec, float* hCMV, F2* ctrlF, float* ctrlM, float* pressure) {
+  const float state[6] = {controlPos.x, controlPos.y, controlVel.x, controlVel.y, controlTheta, controlOmega};
+  float f[2] = {0.0f, 0.0f};
+  demso_shim::report(dem_collide_controlled(demso_shim::context(), oldPos, oldVel, newAcc, oldOmega, newAlpha, oldType,
+                                            cellStart, cellEnd, numParticles, numCells, state, controlType,
+                                            controlForceVecX, hCFVx, controlForceVecY, hCFVy, controlMomentVec, hCMV,
+                                            f, ctrlM, pressure),
+                     "collide");
+  ctrlF->x = f[0];
+  ctrlF->y = f[1];
 }
 
 inline void integrateSystem(float* pos, float* vel, float* acc, float* theta, float* omega, float* alpha,

@@ -90,6 +90,21 @@ struct System {
 };
 
 template <class T> struct V2 { T x, y; };
+
+// The controlled ("big") particle of the reference's default build
+// (USE_BIG_PARTICLE, src/main.cuh:29): a disc of material `type` at pos with
+// velocity vel and spin omega that every particle collides with
+// (src/particles_kernel.cuh:525-543), moved kinematically after each step
+// (src/main.cu:510-514: pos += move, computed in double as the reference's
+// double literal; y < wrap_below_y -> reset_y; x > wrap_above_x -> reset_x).
+struct Control {
+  float pos[2], vel[2];
+  float theta, omega;
+  uint32_t type, enabled;
+  double move[2];
+  double wrap_below_y, wrap_above_x;
+  float reset_y, reset_x;
+};
 template <class T> inline V2<T> mk(T x, T y) { return V2<T>{x, y}; }
 template <class T> inline V2<T> operator+(V2<T> a, V2<T> b) { return mk(a.x + b.x, a.y + b.y); }
 template <class T> inline V2<T> operator-(V2<T> a, V2<T> b) { return mk(a.x - b.x, a.y - b.y); }
@@ -293,7 +308,8 @@ V2<T> attract_spheres(const System& s, V2<T> posA, V2<T> posB, uint32_t typeA, u
 template <class T>
 void collide(const System& s, const V2<T>* pos, const V2<T>* vel, V2<T>* acc, const T* omega,
              T* alpha, const uint32_t* type, const T* radius, const uint32_t* cell_start,
-             const uint32_t* cell_end, uint32_t n, T* pressure) {
+             const uint32_t* cell_end, uint32_t n, T* pressure, const Control* ctl = nullptr,
+             T* react_x = nullptr, T* react_y = nullptr, T* react_m = nullptr) {
   const bool clamp = s.flags & GRID_CLAMP;
 #pragma omp parallel for schedule(dynamic, 1024)
   for (long long ii = 0; ii < (long long)n; ++ii) {
@@ -332,10 +348,54 @@ void collide(const System& s, const V2<T>* pos, const V2<T>* vel, V2<T>* acc, co
         force += cf;
       }
     }
+    if (ctl && ctl->enabled) {
+      // src/particles_kernel.cuh:527-541 (USE_ATOMIC 0): the pair with the
+      // controlled particle, summed on its own, then added; the reaction on
+      // the controlled particle goes to per-particle vectors
+      const uint32_t tC = ctl->type;
+      const T rC = T(s.material(tC).radius);
+      const V2<T> pC = mk(T(ctl->pos[0]), T(ctl->pos[1])), vC = mk(T(ctl->vel[0]), T(ctl->vel[1]));
+      T m = T(0);
+      V2<T> f = collide_spheres<T>(s, p, pC, v, vC, w, T(ctl->omega), tA, tC, bA.r, rC, m, pres);
+      f += attract_spheres<T>(s, p, pC, tA, tC, bA.r, rC, pres);
+      force += f;
+      moment += m;
+      react_x[i] = -f.x;
+      react_y[i] = -f.y;
+      react_m[i] = -m / bA.r * rC;
+    }
     acc[i] = force / bA.m;
     alpha[i] = moment / bA.inertia;
     pressure[i] = pres;
   }
+}
+
+// Sum of the per-particle reactions as the reference reduces them
+// (reduceVecD, src/particles_kernel.cuh:552-578, launched with 512 threads by
+// src/functions.cu:475-513): a pairwise tree over each block of 512
+// consecutive values (the last block padded with zeros), then the block sums
+// added in order.  (For n < 512 the reference launches n threads and its tree
+// drops values when n is not a power of two; padding to 512 sums them all.)
+template <class T> T reduce_reaction(const T* v, uint32_t n) {
+  const uint32_t B = 512;
+  T total = T(0);
+  std::vector<T> cache(B);
+  for (uint32_t b0 = 0; b0 < n; b0 += B) {
+    for (uint32_t t = 0; t < B; ++t) cache[t] = b0 + t < n ? v[b0 + t] : T(0);
+    for (uint32_t i = B / 2; i != 0; i /= 2)
+      for (uint32_t t = 0; t < i; ++t) cache[t] += cache[t + i];
+    total += cache[0];
+  }
+  return total;
+}
+
+// The kinematic move of the controlled particle after a step
+// (src/main.cu:511-514).
+inline void move_control(Control& c) {
+  c.pos[0] = float(double(c.pos[0]) + c.move[0]);
+  c.pos[1] = float(double(c.pos[1]) + c.move[1]);
+  if (double(c.pos[1]) < c.wrap_below_y) c.pos[1] = c.reset_y;
+  if (double(c.pos[0]) > c.wrap_above_x) c.pos[0] = c.reset_x;
 }
 
 // collideBoundary (src/particles_kernel.cuh:389-456): the bottom-wall law.
@@ -432,6 +492,9 @@ template <class T> struct Sim {
   std::vector<T> theta[2], omega[2], alpha, pressure, radius[2];
   std::vector<uint32_t> id[2], type[2], loc[2], hash, index, cell_start, cell_end;
   int cur = 0;
+  Control ctl{};
+  std::vector<T> react_x, react_y, react_m;
+  T reaction[3] = {T(0), T(0), T(0)};  // force on the controlled particle (x, y) and its moment
 
   State<T> st(int b) {
     return State<T>{pos[b].data(), vel[b].data(), theta[b].data(), omega[b].data(), id[b].data(),
@@ -454,10 +517,22 @@ template <class T> struct Sim {
     sort_particles(hash.data(), index.data(), n);
     reorder_and_find_cell_start<T>(cell_start.data(), cell_end.data(), s, hash.data(), index.data(), o,
                                    n, num_cells);
+    if (ctl.enabled) {
+      react_x.resize(n);
+      react_y.resize(n);
+      react_m.resize(n);
+    }
     collide<T>(sys, s.pos, s.vel, acc.data(), s.omega, alpha.data(), s.type, s.radius,
-               cell_start.data(), cell_end.data(), n, pressure.data());
+               cell_start.data(), cell_end.data(), n, pressure.data(), &ctl, react_x.data(), react_y.data(),
+               react_m.data());
+    if (ctl.enabled) {
+      reaction[0] = reduce_reaction(react_x.data(), n);
+      reaction[1] = reduce_reaction(react_y.data(), n);
+      reaction[2] = reduce_reaction(react_m.data(), n);
+    }
     integrate<T>(sys, s.pos, s.vel, acc.data(), s.theta, s.omega, alpha.data(), s.type, s.radius,
                  pressure.data(), n);
+    if (ctl.enabled) move_control(ctl);
     cur ^= 1;
   }
 };

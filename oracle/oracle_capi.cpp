@@ -224,6 +224,46 @@ int orc_sim_get_state_f64(void* p, double* pos, double* vel, double* theta, doub
                        radius, acc, alpha, pressure, hash, index, cell_start, cell_end);
 }
 
+// ---- controlled particle --------------------------------------------------------
+
+int orc_sim_set_control_f32(void* p, const orc::Control* c) {
+  static_cast<OwnedSim<float>*>(p)->sim.ctl = *c;
+  return 0;
+}
+int orc_sim_set_control_f64(void* p, const orc::Control* c) {
+  static_cast<OwnedSim<double>*>(p)->sim.ctl = *c;
+  return 0;
+}
+// Current controlled-particle state and the last step's reaction
+// (force x, y and moment on the controlled particle).
+int orc_sim_get_control_f32(void* p, orc::Control* c, double* reaction) {
+  auto& m = static_cast<OwnedSim<float>*>(p)->sim;
+  if (c) *c = m.ctl;
+  for (int k = 0; k < 3 && reaction; ++k) reaction[k] = m.reaction[k];
+  return 0;
+}
+int orc_sim_get_control_f64(void* p, orc::Control* c, double* reaction) {
+  auto& m = static_cast<OwnedSim<double>*>(p)->sim;
+  if (c) *c = m.ctl;
+  for (int k = 0; k < 3 && reaction; ++k) reaction[k] = m.reaction[k];
+  return 0;
+}
+
+// collide with the controlled particle (FP32), the per-particle reaction
+// vectors and their reduction (force x, y, moment) into reaction[3].
+int orc_collide_controlled(const System* s, const float* pos, const float* vel, float* acc, const float* omega,
+                           float* alpha, const uint32_t* type, const float* radius, const uint32_t* cell_start,
+                           const uint32_t* cell_end, uint32_t n, float* pressure, const orc::Control* c,
+                           float* react_x, float* react_y, float* react_m, float* reaction) {
+  orc::collide<float>(*s, reinterpret_cast<const V2<float>*>(pos), reinterpret_cast<const V2<float>*>(vel),
+                      reinterpret_cast<V2<float>*>(acc), omega, alpha, type, radius, cell_start, cell_end, n,
+                      pressure, c, react_x, react_y, react_m);
+  reaction[0] = orc::reduce_reaction(react_x, n);
+  reaction[1] = orc::reduce_reaction(react_y, n);
+  reaction[2] = orc::reduce_reaction(react_m, n);
+  return 0;
+}
+
 // ---- FP64 adjoint -------------------------------------------------------------
 
 void* orc_adj_create(const System* s, uint32_t nt, uint32_t ni, uint32_t n, uint32_t cells, int rad) {

@@ -70,6 +70,11 @@ def load():
         "orc_sim_set_state_f64": (c_int, [c_void_p] + [P] * 8),
         "orc_sim_step_f32": (c_double, [c_void_p, c_uint32]),
         "orc_sim_step_f64": (c_double, [c_void_p, c_uint32]),
+        "orc_sim_set_control_f32": (c_int, [c_void_p, c_void_p]),
+        "orc_sim_set_control_f64": (c_int, [c_void_p, c_void_p]),
+        "orc_sim_get_control_f32": (c_int, [c_void_p, c_void_p, P]),
+        "orc_sim_get_control_f64": (c_int, [c_void_p, c_void_p, P]),
+        "orc_collide_controlled": (c_int, [c_void_p] + [P] * 9 + [c_uint32, P, c_void_p, P, P, P, P]),
         "orc_sim_get_state_f32": (c_int, [c_void_p] + [P] * 15),
         "orc_sim_get_state_f64": (c_int, [c_void_p] + [P] * 15),
         "orc_adj_create": (c_void_p, [POINTER(OrcSystem), c_uint32, c_uint32, c_uint32, c_uint32, c_int]),
@@ -187,6 +192,45 @@ def integrate(system: System, st: dict, acc, alpha, pressure):
     return out, pres
 
 
+class Control(ctypes.Structure):
+    """orc::Control: the controlled particle (pos, vel, theta, omega, type,
+    enabled, per-step move in double, wrap thresholds in double, resets)."""
+
+    _fields_ = [
+        ("pos", c_float * 2), ("vel", c_float * 2), ("theta", c_float), ("omega", c_float),
+        ("type", c_uint32), ("enabled", c_uint32), ("move", ctypes.c_double * 2),
+        ("wrap_below_y", ctypes.c_double), ("wrap_above_x", ctypes.c_double),
+        ("reset_y", c_float), ("reset_x", c_float),
+    ]
+
+
+def control(pos, type_, vel=(0.0, 0.0), omega=0.0, theta=0.0, move=(0.0, 0.0), wrap_below_y=-1e300,
+            wrap_above_x=1e300, reset_y=0.0, reset_x=0.0) -> Control:
+    c = Control()
+    c.pos[:] = pos
+    c.vel[:] = vel
+    c.theta, c.omega, c.type, c.enabled = theta, omega, type_, 1
+    c.move[:] = move
+    c.wrap_below_y, c.wrap_above_x, c.reset_y, c.reset_x = wrap_below_y, wrap_above_x, reset_y, reset_x
+    return c
+
+
+def collide_controlled(system: System, st: dict, cs, ce, ctl: Control):
+    """collide with the controlled particle: (acc, alpha, pressure,
+    (react_x, react_y, react_m), reaction[3])."""
+    lib = load()
+    n = st["pos"].shape[0]
+    acc = np.zeros((n, 2), np.float32)
+    alpha = np.zeros(n, np.float32)
+    pres = np.zeros(n, np.float32)
+    rx, ry, rm = (np.zeros(n, np.float32) for _ in range(3))
+    red = np.zeros(3, np.float32)
+    lib.orc_collide_controlled(ctypes.byref(system.sys), _p(st["pos"]), _p(st["vel"]), _p(acc), _p(st["omega"]),
+                               _p(alpha), _p(st["type"]), _p(st.get("radius")), _p(cs), _p(ce), n, _p(pres),
+                               ctypes.byref(ctl), _p(rx), _p(ry), _p(rm), _p(red))
+    return acc, alpha, pres, (rx, ry, rm), red
+
+
 # ---- owned simulation ----------------------------------------------------------------
 
 class Sim:
@@ -225,6 +269,16 @@ class Sim:
     def step(self, steps: int = 1) -> float:
         """Runs `steps` steps; returns the wall time in seconds."""
         return float(getattr(self.lib, f"orc_sim_step_{self.prec}")(self.h, steps))
+
+    def set_control(self, ctl: Control) -> None:
+        getattr(self.lib, f"orc_sim_set_control_{self.prec}")(self.h, ctypes.byref(ctl))
+
+    def control(self):
+        """(Control, reaction[3] float64) after the steps run so far."""
+        c = Control()
+        r = np.zeros(3, np.float64)
+        getattr(self.lib, f"orc_sim_get_control_{self.prec}")(self.h, ctypes.byref(c), _p(r))
+        return c, r
 
     def state(self) -> dict:
         n, c = self.n, self.cells

@@ -47,12 +47,13 @@ def test_struct_layouts_match_header(tmp_path):
     """ctypes struct sizes equal the C compiler's."""
     src = tmp_path / "s.c"
     src.write_text('#include <stdio.h>\n#include "demso.h"\nint main(void){ printf("%zu %zu %zu\\n", '
-                   'sizeof(dem_system_props), sizeof(dem_particle_props), sizeof(dem_instance)); return 0; }\n')
+                   'sizeof(dem_system_props), sizeof(dem_particle_props), sizeof(dem_instance)); '
+                   'printf("%zu\\n", sizeof(dem_controlled)); return 0; }\n')
     exe = tmp_path / "s"
     subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     assert sizes == [ctypes.sizeof(_lib.SystemProps), ctypes.sizeof(_lib.ParticleProps),
-                     ctypes.sizeof(_lib.InstanceProps)]
+                     ctypes.sizeof(_lib.InstanceProps), ctypes.sizeof(_lib.ControlledProps)]
 
 
 def test_create_without_device_reports_error():
@@ -87,6 +88,8 @@ void (*f3)(unsigned*, unsigned*, float*, float*, float*, float*, unsigned*, unsi
            unsigned*, float*, float*, float*, float*, unsigned*, unsigned*, unsigned, unsigned) = reorderDataAndFindCellStart;
 void (*f4)(float*, float*, float*, float*, float*, unsigned*, unsigned*, unsigned*, unsigned, unsigned, float*) = collide;
 void (*f5)(float*, float*, float*, float*, float*, float*, unsigned*, float*, unsigned) = integrateSystem;
+void (*f6)(float*, float*, float*, float*, float*, unsigned*, unsigned*, unsigned*, unsigned, unsigned, V2f, V2f, float,
+           float, unsigned, float*, float*, float*, float*, float*, float*, V2f*, float*, float*) = collide<V2f>;
 int main() {
   Sys s{}; Part p[1]{};
   s.numParticles = 1; s.cubeDimension = {1, 1}; s.gridSize = {4, 4}; s.numCells = 16; s.timeStep = 1e-3f;

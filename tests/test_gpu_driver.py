@@ -89,28 +89,36 @@ def test_output_file(runs):
         assert np.allclose(r[2:], exp, rtol=0, atol=5e-7 + 1e-6 * np.abs(exp)), (pid, r[2:], exp)
 
 
-def test_reference_pile_scene_matches_oracle(tmp_path, oracle):
-    """DemSo's own pilha.dsml (read with the controlled-particle build's rules:
-    136 discs on the 16 x 16 wrapping grid, attraction 10) through the
-    reference's stage loop for 300 steps, against the oracle's run of the same
-    pile (scene.triangle_pile)."""
+@pytest.mark.parametrize("loop", ["stages", "graph"])
+def test_reference_pile_scene_matches_oracle(tmp_path, oracle, loop):
+    """DemSo's own pilha.dsml as its default build runs it (controlled
+    particle: 136 discs on the 16 x 16 wrapping grid, attraction 10, the MOVE
+    disc driven down through the pile) for 1400 steps, through the
+    reference's stage loop and through graph replays, against the oracle's
+    run of the same scene (scene.triangle_pile(controlled=True)): state, the
+    controlled particle and the last step's reaction on it, bit for bit."""
     from pathlib import Path
 
     from demso_amd import scene as sc
+    from test_gpu_control import oracle_run
 
+    steps = 1400
     pile = Path(__file__).resolve().parent / "golden" / "dsml" / "pilha.dsml"
-    print("driver: pilha.dsml --steps 300", flush=True)
-    r = run_driver("--controlled", "--steps", 300, "--output", tmp_path / "out.csv", "--dump-final", "--dump-prefix",
-                   tmp_path / "pile_", pile, timeout=300)
+    print(f"driver: pilha.dsml --controlled --loop {loop} --steps {steps}", flush=True)
+    r = run_driver("--controlled", "--loop", loop, "--steps", steps, "--output", tmp_path / "out.csv", "--dump-final",
+                   "--dump-prefix", tmp_path / "pile_", pile, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    g = dumps.read_dump(tmp_path / "pile_300.bin")
-    ref = oracle.Sim(sc.triangle_pile(), "f32")
-    ref.step(300)
-    o = ref.state()
-    order = np.argsort(o["id"])
+    g = dumps.read_dump(tmp_path / f"pile_{steps}.bin")
+    s = sc.triangle_pile(controlled=True)
+    o, cs, red = oracle_run(oracle, s, s.notes["controlled"], steps)
     for k in ("pos", "vel", "theta", "omega", "acc", "alpha", "pressure"):
-        assert np.array_equal(g[k].view(np.uint32), np.ascontiguousarray(o[k][order]).view(np.uint32)), k
-    ref.close()
+        assert np.array_equal(g[k].view(np.uint32), o[k].view(np.uint32)), k
+    line = [l for l in r.stdout.splitlines() if l.startswith("controlled particle:")]
+    assert len(line) == 1, r.stdout
+    w = line[0].split()
+    x, y, fx, fy, m = (float.fromhex(v) for v in (w[3], w[4], w[6], w[7], w[8]))
+    assert [x, y] == list(cs.pos) and cs.pos[1] < 3.0
+    assert [fx, fy, m] == [float(np.float32(v)) for v in red] and max(map(abs, red)) > 0
     # pilha.dsml follows IDs 500 and 2200 of 136 discs: reported, nothing written
     assert "followed particle 500 does not exist" in r.stderr
     assert (tmp_path / "out.csv").read_text() == ""
