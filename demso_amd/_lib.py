@@ -137,6 +137,16 @@ PROTOTYPES = {
     "dem_sim_set_by_id": (c_int, [c_void_p, P, P]),
     "dem_gather_rows": (c_int, [c_void_p, P, c_uint32] + [P] * 10 + [c_uint32, P]),
     "dem_sim_gather_rows": (c_int, [c_void_p, P, c_uint32, P]),
+    "dem_create_rectangles": (
+        c_int, [c_void_p] + [P] * 4 + [POINTER(c_float), POINTER(c_float), POINTER(c_uint32), c_uint32, c_uint32,
+                                       POINTER(c_uint32), ctypes.c_uint64]),
+    "dem_create_rectangles_ex": (
+        c_int, [c_void_p] + [P] * 4 + [POINTER(c_float), POINTER(c_float), POINTER(c_uint32), c_uint32, c_uint32,
+                                       POINTER(c_uint32), ctypes.c_uint64, ctypes.c_uint64]),
+    "dem_create_triangles": (
+        c_int, [c_void_p] + [P] * 4 + [POINTER(c_float), c_uint32, c_uint32, POINTER(c_uint32), c_float, c_float,
+                                       c_uint32, c_uint32]),
+    "dem_create_single_particles": (c_int, [c_void_p] + [P] * 12 + [c_uint32, c_uint32]),
     "dem_set_controlled": (c_int, [c_void_p, POINTER(ControlledProps)]),
     "dem_get_controlled": (c_int, [c_void_p, POINTER(ControlledProps), POINTER(c_float)]),
     "dem_collide_controlled": (

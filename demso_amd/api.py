@@ -250,6 +250,50 @@ def collide_controlled(ctx: Context, oldPos, oldVel, newAcc, oldOmega, newAlpha,
     return np.array(f[:], np.float32), np.float32(m[0])
 
 
+def createRectangles(ctx: Context, pos, ID, loc, type, start, sideLenght, side, startID: int,
+                     particleTypeVec, seed: int, stream: Optional[int] = None) -> None:
+    """createRectangles (src/functions.cu:198-245) on the device; the jitter
+    draws are the reproducible normals of streams `stream` (x) and stream + 1
+    (y), default DEM_RECT_STREAM (1000)."""
+    tv = np.ascontiguousarray(particleTypeVec, dtype=np.uint32)
+    st = (c_float * 2)(float(start[0]), float(start[1]))
+    ln = (c_float * 2)(float(sideLenght[0]), float(sideLenght[1]))
+    sd = (c_uint32 * 2)(int(side[0]), int(side[1]))
+    tvp = tv.ctypes.data_as(ctypes.POINTER(c_uint32))
+    if stream is None:
+        ctx._check(ctx.lib.dem_create_rectangles(ctx.handle, _ptr(pos), _ptr(ID), _ptr(loc), _ptr(type), st, ln, sd,
+                                                 int(startID), tv.size, tvp, int(seed)))
+    else:
+        ctx._check(ctx.lib.dem_create_rectangles_ex(ctx.handle, _ptr(pos), _ptr(ID), _ptr(loc), _ptr(type), st, ln,
+                                                    sd, int(startID), tv.size, tvp, int(seed), int(stream)))
+
+
+def createTriangles(ctx: Context, pos, ID, loc, type, start, N: int, particleTypeVec, space: float, height: float,
+                    startID: int, numParticles: int) -> None:
+    """createTriangles (src/functions.cu:247-281) on the device."""
+    tv = np.ascontiguousarray(particleTypeVec, dtype=np.uint32)
+    st = (c_float * 2)(float(start[0]), float(start[1]))
+    ctx._check(ctx.lib.dem_create_triangles(ctx.handle, _ptr(pos), _ptr(ID), _ptr(loc), _ptr(type), st, int(N),
+                                            tv.size, tv.ctypes.data_as(ctypes.POINTER(c_uint32)), float(space),
+                                            float(height), int(startID), int(numParticles)))
+
+
+def createSingleParticles(ctx: Context, pos, vel, theta, omega, ID, loc, type, usrPos, usrVel, usrTheta, usrOmega,
+                          usrType, numParticles: int, startID: int) -> None:
+    """createSingleParticles (src/functions.cu:283-311): usr* are host numpy
+    arrays or device tensors."""
+    def src(a, dt):
+        if a is None or hasattr(a, "data_ptr"):
+            return _ptr(a), None
+        keep = np.ascontiguousarray(a, dtype=dt)
+        return keep.ctypes.data, keep
+    srcs = [src(usrPos, np.float32), src(usrVel, np.float32), src(usrTheta, np.float32), src(usrOmega, np.float32),
+            src(usrType, np.uint32)]
+    ctx._check(ctx.lib.dem_create_single_particles(
+        ctx.handle, _ptr(pos), _ptr(vel), _ptr(theta), _ptr(omega), _ptr(ID), _ptr(loc), _ptr(type),
+        *[p for p, _ in srcs], int(numParticles), int(startID)))
+
+
 def integrateSystem(ctx: Context, pos, vel, acc, theta, omega, alpha, type, pressure, numParticles: int,
                     radius=None) -> None:
     """integrateSystem (src/functions.cu:529-551)."""

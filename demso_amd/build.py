@@ -72,8 +72,11 @@ def build_driver(verbose: bool = False) -> Path:
     HIP runtime) into demso_amd/demso_run."""
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     tmp = DRIVER.with_name("demso_run.tmp")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include",
-           f"-I{ROOT / 'include'}", *[str(PKG / "driver" / f) for f in DRIVER_SOURCES],
+    # -ffp-contract=off: the rectangle draws (csrc/scene_rng.h) must round as
+    # the device generator does
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+           f"-I{rocm}/include", f"-I{ROOT / 'include'}", f"-I{CSRC}",
+           *[str(PKG / "driver" / f) for f in DRIVER_SOURCES],
            f"-L{PKG}", "-ldemso", f"-L{rocm}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -196,6 +196,7 @@ int dem_destroy(dem_context* ctx) {
   if (ctx->d_copy_tmp) (void)hipFree(ctx->d_copy_tmp);
   if (ctx->d_control) (void)hipFree(ctx->d_control);
   if (ctx->d_stage_control) (void)hipFree(ctx->d_stage_control);
+  if (ctx->d_typevec) (void)hipFree(ctx->d_typevec);
   free_sort(ctx->sort);
   delete ctx;
   return DEM_OK;
@@ -714,19 +715,19 @@ int dem_sim_upload(dem_sim* m, const float* pos, const float* vel, const float* 
   if (!m) return DEM_ERR_INVALID;
   dem_context* ctx = m->ctx;
   if (!pos || !vel || !theta || !omega || !id || !type || !loc || (m->rad && !radius))
-    return fail(ctx, DEM_ERR_INVALID, "NULL host array");
+    return fail(ctx, DEM_ERR_INVALID, "NULL array");
   if (int r = ensure_device(ctx)) return r;
   const int c = m->cur;
   const size_t n = m->n;
   hipStream_t s = ctx->stream;
-  DEM_HIP(ctx, hipMemcpyAsync(m->pos[kHome], pos, 8 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->vel[kHome], vel, 8 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->theta[kHome], theta, 4 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->omega[kHome], omega, 4 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->id[c], id, 4 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->type[c], type, 4 * n, hipMemcpyHostToDevice, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->loc[c], loc, 4 * n, hipMemcpyHostToDevice, s), "upload");
-  if (m->rad) DEM_HIP(ctx, hipMemcpyAsync(m->radius[c], radius, 4 * n, hipMemcpyHostToDevice, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->pos[kHome], pos, 8 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->vel[kHome], vel, 8 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->theta[kHome], theta, 4 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->omega[kHome], omega, 4 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->id[c], id, 4 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->type[c], type, 4 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->loc[c], loc, 4 * n, hipMemcpyDefault, s), "upload");
+  if (m->rad) DEM_HIP(ctx, hipMemcpyAsync(m->radius[c], radius, 4 * n, hipMemcpyDefault, s), "upload");
   DEM_HIP(ctx, hipStreamSynchronize(s), "upload sync");
   m->resort_ok = false;  // an uploaded state is in no known sorted order
   *m->h_count = 0;

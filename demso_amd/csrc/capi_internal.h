@@ -36,6 +36,9 @@ struct dem_context {
   float* d_reaction = nullptr;
   dem_controlled* d_stage_control = nullptr;
   float* d_stage_reaction = nullptr;
+  // particle type vector of the scene generators (grown as needed)
+  uint32_t* d_typevec = nullptr;
+  uint32_t typevec_cap = 0;
 };
 
 // Buffer of the owned simulation's dynamic fields (pos, vel, theta, omega)

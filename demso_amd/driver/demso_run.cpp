@@ -250,6 +250,37 @@ struct Buffers {
   }
 };
 
+// PrepareSim's particle creation on the device (src/main.cu:249-324): the
+// single particles, then each rectangle and triangle with the library's
+// generators, into device arrays of n particles; vel, theta and omega of the
+// generated blocks start at zero.
+int create_on_device(dem_context* ctx, const dsml::PreparedScene& ps, float* pos, float* vel, float* theta,
+                     float* omega, uint32_t* id, uint32_t* loc, uint32_t* type) {
+  const size_t n = ps.sys.num_particles;
+  HIP_OK(hipMemset(vel, 0, 8 * n));
+  HIP_OK(hipMemset(theta, 0, 4 * n));
+  HIP_OK(hipMemset(omega, 0, 4 * n));
+  const uint32_t ns = (uint32_t)ps.num_singles;
+  DEM_OK_OR_FAIL(dem_create_single_particles(ctx, pos, vel, theta, omega, id, loc, type, ps.pos.data(), ps.vel.data(),
+                                             ps.theta.data(), ps.omega.data(), ps.type.data(), ns, 0),
+                 ctx);
+  for (const dsml::RectBlock& r : ps.rects) {
+    const size_t f = r.first;
+    DEM_OK_OR_FAIL(dem_create_rectangles_ex(ctx, pos + 2 * f, id + f, loc + f, type + f, r.start, r.side_length,
+                                            r.side, (uint32_t)f, (uint32_t)r.types.size(), r.types.data(), ps.seed,
+                                            r.stream),
+                   ctx);
+  }
+  for (const dsml::TriBlock& t : ps.tris) {
+    const size_t f = t.first;
+    DEM_OK_OR_FAIL(dem_create_triangles(ctx, pos + 2 * f, id + f, loc + f, type + f, t.start, t.N,
+                                        (uint32_t)t.types.size(), t.types.data(), t.space, t.height, (uint32_t)f,
+                                        (uint32_t)t.count),
+                   ctx);
+  }
+  return 0;
+}
+
 struct Run {
   const Options& opt;
   const dsml::PreparedScene& ps;
@@ -374,15 +405,21 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
     HIP_OK(hipMemset(b.acc, 0, 8 * (size_t)n));
     HIP_OK(hipMemset(b.alpha, 0, 4 * (size_t)n));
     HIP_OK(hipMemset(b.pressure, 0, 4 * (size_t)n));
-    HIP_OK(hipMemcpy(b.pos[0], ps.pos.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(b.vel[0], ps.vel.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(b.theta[0], ps.theta.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(b.omega[0], ps.omega.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(b.id[0], ps.id.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(b.loc[0], ps.loc.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(b.type[0], ps.type.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
+    if (create_on_device(ctx, ps, b.pos[0], b.vel[0], b.theta[0], b.omega[0], b.id[0], b.loc[0], b.type[0]))
+      return 1;
     hipStream_t s = nullptr;  // the shim's context launches on the null stream
     int cur = 0;
+    if (opt.dump_final && steps == 0) {
+      // the created state itself (acc, alpha, pressure are zero)
+      HIP_OK(hipDeviceSynchronize());
+      HIP_OK(hipMemcpy(host.pos.data(), b.pos[0], 8 * (size_t)n, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(host.vel.data(), b.vel[0], 8 * (size_t)n, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(host.theta.data(), b.theta[0], 4 * (size_t)n, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(host.omega.data(), b.omega[0], 4 * (size_t)n, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(host.id.data(), b.id[0], 4 * (size_t)n, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(host.type.data(), b.type[0], 4 * (size_t)n, hipMemcpyDeviceToHost));
+      if (!write_dump(dump_name(0), host, 0, 0.0)) return 1;
+    }
     for (long long tempo = 1; tempo <= steps; ++tempo) {
       const int o = cur, t = cur ^ 1;
       calcHash(b.pos[o], b.index, b.hash, n);
@@ -453,9 +490,21 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
   DEM_OK_OR_FAIL(dem_set_params(ctx, &ps.sys, ps.types.data(), (uint32_t)ps.types.size()), ctx);
   DEM_OK_OR_FAIL(dem_sim_create(ctx, n, ps.sys.num_cells, 0, &down.sim), ctx);
   dem_sim* sim = down.sim;
-  DEM_OK_OR_FAIL(dem_sim_upload(sim, ps.pos.data(), ps.vel.data(), ps.theta.data(), ps.omega.data(), ps.id.data(),
-                                ps.type.data(), ps.loc.data(), nullptr),
-                 ctx);
+  {
+    // create on the device, then hand the arrays to the sim (device to device)
+    Buffers g;
+    float *pos = nullptr, *vel = nullptr, *theta = nullptr, *omega = nullptr;
+    uint32_t *id = nullptr, *loc = nullptr, *type = nullptr;
+    HIP_OK(g.get(&pos, 2 * (size_t)n));
+    HIP_OK(g.get(&vel, 2 * (size_t)n));
+    HIP_OK(g.get(&theta, n));
+    HIP_OK(g.get(&omega, n));
+    HIP_OK(g.get(&id, n));
+    HIP_OK(g.get(&loc, n));
+    HIP_OK(g.get(&type, n));
+    if (create_on_device(ctx, ps, pos, vel, theta, omega, id, loc, type)) return 1;
+    DEM_OK_OR_FAIL(dem_sim_upload(sim, pos, vel, theta, omega, id, type, loc, nullptr), ctx);
+  }
   if (ps.controlled_type >= 0) {
     dem_controlled c{};
     c.pos[0] = 5.0f;
@@ -471,6 +520,13 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
   }
   hipStream_t s = nullptr;
   long long done = 0;
+  if (opt.dump_final && steps == 0) {
+    DEM_OK_OR_FAIL(dem_sim_download(sim, host.pos.data(), host.vel.data(), host.theta.data(), host.omega.data(),
+                                    host.id.data(), host.type.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                    nullptr, nullptr, nullptr, nullptr),
+                   ctx);
+    if (!write_dump(dump_name(0), host, 0, 0.0)) return 1;
+  }
   while (done < steps) {
     // run up to the next step that needs output: every step with followed
     // particles, else the next dump

@@ -4,29 +4,9 @@
 #include <cmath>
 #include <string>
 
+#include "scene_rng.h"
+
 namespace dsml {
-
-namespace {
-constexpr uint64_t kG = 0x9E3779B97F4A7C15ull;
-constexpr uint64_t kH = 0xD1B54A32D192ED03ull;
-
-uint64_t mix(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-}  // namespace
-
-double uniform01(uint64_t seed, uint64_t stream, uint64_t i) {
-  const uint64_t z = seed * kG + stream * kH + (i + 1) * kG;
-  return (double)(mix(z) >> 11) * 0x1.0p-53;
-}
-
-double normal01(uint64_t seed, uint64_t stream, uint64_t i) {
-  const double u1 = uniform01(seed, stream, i);
-  const double u2 = uniform01(seed, stream + 1000003ull, i);
-  return std::sqrt(-2.0 * std::log1p(-u1)) * std::cos(2.0 * M_PI * u2);
-}
 
 void reference_grid(float lx, float ly, float max_radius, uint32_t* gx, uint32_t* gy) {
   if (!(max_radius > 0.0f)) throw DsmlError("no particle type has a positive radius");
@@ -123,6 +103,8 @@ PreparedScene prepare_sim(const DEMSimulation& sim, uint64_t seed) {
   for (uint64_t i = 0; i < n; ++i) out.id[i] = out.loc[i] = (uint32_t)i;
 
   uint64_t at = 0;
+  out.seed = seed;
+  out.num_singles = P.pos.size();
   // single particles (createSingleParticles, src/functions.cu:283-311)
   for (size_t k = 0; k < P.pos.size(); ++k, ++at) {
     out.pos[2 * at] = P.pos[k].x;
@@ -141,11 +123,22 @@ PreparedScene prepare_sim(const DEMSimulation& sim, uint64_t seed) {
     const auto& tv = P.types[r];
     const float dx = side_len.x / (float)(side.x - 1);
     const float dy = side_len.y / (float)(side.y - 1);
+    RectBlock blk;
+    blk.start[0] = st.x;
+    blk.start[1] = st.y;
+    blk.side_length[0] = side_len.x;
+    blk.side_length[1] = side_len.y;
+    blk.side[0] = side.x;
+    blk.side[1] = side.y;
+    for (int t : tv) blk.types.push_back((uint32_t)t);
+    blk.first = at;
+    blk.stream = 1000 + 2 * r;
+    out.rects.push_back(blk);
     for (uint32_t y = 0; y < side.y; ++y)
       for (uint32_t x = 0; x < side.x; ++x) {
         const uint64_t p = (uint64_t)x + (uint64_t)y * side.x;
-        const float nx = (float)normal01(seed, 1000 + 2 * r, p);
-        const float ny = (float)normal01(seed, 1001 + 2 * r, p);
+        const float nx = (float)dem_rng::jitter_normal(seed, 1000 + 2 * r, p);
+        const float ny = (float)dem_rng::jitter_normal(seed, 1001 + 2 * r, p);
         const uint64_t i = at + p;
         out.pos[2 * i] = st.x + dx * ((float)x + (nx - 0.5f) / 100.0f);
         out.pos[2 * i + 1] = st.y + dy * ((float)y + (ny - 0.5f) / 100.0f);
@@ -162,6 +155,16 @@ PreparedScene prepare_sim(const DEMSimulation& sim, uint64_t seed) {
     const float height = (float)((double)space * std::sqrt(3.0) / 2.0);
     const auto& tv = P.t_types[t];
     const Vec2 st = P.t_pos[t];
+    TriBlock blk;
+    blk.start[0] = st.x;
+    blk.start[1] = st.y;
+    blk.N = N;
+    blk.space = space;
+    blk.height = height;
+    for (int k : tv) blk.types.push_back((uint32_t)k);
+    blk.first = at;
+    blk.count = count;
+    out.tris.push_back(blk);
     for (uint64_t index = 0; index < count; ++index) {
       const uint32_t radicand = N * N + N - 2u * (uint32_t)(index + 1) + 1u;
       const int L1 = (int)((1.0f + 2.0f * std::sqrt((float)radicand)) / 2.0f);

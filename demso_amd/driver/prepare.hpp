@@ -10,6 +10,24 @@
 
 namespace dsml {
 
+// A rectangle block as createRectangles takes it (src/functions.cu:198-245):
+// its particles are first, first + 1, ...; draws from streams stream and
+// stream + 1.
+struct RectBlock {
+  float start[2], side_length[2];
+  uint32_t side[2];
+  std::vector<uint32_t> types;
+  uint64_t first = 0, stream = 0;
+};
+// A triangle block as createTriangles takes it (src/functions.cu:247-281).
+struct TriBlock {
+  float start[2];
+  uint32_t N = 0;
+  float space = 0.0f, height = 0.0f;
+  std::vector<uint32_t> types;
+  uint64_t first = 0, count = 0;
+};
+
 struct PreparedScene {
   dem_system_props sys{};                  // SystemProperties (src/main.cuh:108-122)
   std::vector<dem_particle_props> types;   // ParticleProperties rows (src/main.cuh:39-54)
@@ -24,14 +42,14 @@ struct PreparedScene {
   // material row of the controlled particle, -1 when the scene was not
   // loaded as the controlled-particle build (src/main.cu:141-149)
   int controlled_type = -1;
+  // the blocks again, for generation on the device: particles [0,
+  // num_singles) are the single particles (host arrays above), then the
+  // rectangles and triangles
+  uint64_t num_singles = 0, seed = 0;
+  std::vector<RectBlock> rects;
+  std::vector<TriBlock> tris;
 };
 
-// SplitMix64 counter generator (Steele, Lea & Flood 2014): value i of stream
-// s under seed k is mix(k*G + s*H + (i+1)*G); the same generator as the
-// Python scene module, so both produce identical draws.
-double uniform01(uint64_t seed, uint64_t stream, uint64_t i);
-// Box-Muller normal from streams (s, s + 1000003).
-double normal01(uint64_t seed, uint64_t stream, uint64_t i);
 
 // Power-of-two grid of PrepareSim (src/main.cu:218-236), including its use of
 // the box width for the y cell size (src/main.cu:231).  Throws DsmlError
@@ -42,8 +60,9 @@ void reference_grid(float lx, float ly, float max_radius, uint32_t* gx, uint32_t
 // Rectangle jitter: the reference draws curand_normal with a time(NULL) seed
 // (src/main.cu:291, src/particles_kernel.cuh:65-69), which no run can
 // reproduce; here the two draws of particle p of rectangle r are
-// normal01(seed, 1000 + 2r, p) and normal01(seed, 1001 + 2r, p), in the same
-// formula.
+// dem_rng::jitter_normal(seed, 1000 + 2r, p) and (seed, 1001 + 2r, p)
+// (csrc/scene_rng.h: the same bits as the device generator
+// dem_create_rectangles_ex and demso_amd/scene.py), in the same formula.
 PreparedScene prepare_sim(const DEMSimulation& sim, uint64_t seed);
 
 }  // namespace dsml

@@ -58,6 +58,84 @@ def normal(seed: int, stream: int, n: int) -> np.ndarray:
     return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
 
 
+def _series_log(x: np.ndarray) -> np.ndarray:
+    """ln x on [2^-53, 1] with correctly rounded operations only: x = f 2^e,
+    f in [sqrt(1/2), sqrt(2)), ln f = 2 atanh((f-1)/(f+1)) to t^21, e ln 2 in
+    fdlibm's two parts.  The same sequence as the device and host generators,
+    so the three agree bit for bit."""
+    f, e = np.frexp(x)
+    low = f < 0.70710678118654752440
+    f = np.where(low, f * 2.0, f)
+    e = np.where(low, e - 1, e)
+    t = (f - 1.0) / (f + 1.0)
+    t2 = t * t
+    p = np.full_like(t, 1.0 / 21.0)
+    for k in (19.0, 17.0, 15.0, 13.0, 11.0, 9.0, 7.0, 5.0, 3.0):
+        p = p * t2 + 1.0 / k
+    p = p * t2 + 1.0
+    lnf = 2.0 * t * p
+    de = e.astype(np.float64)
+    return de * 6.93147180369123816490e-01 + (de * 1.90821492927058770002e-10 + lnf)
+
+
+_SIN = (-1.0 / 121645100408832000.0, 1.0 / 355687428096000.0, -1.0 / 1307674368000.0, 1.0 / 6227020800.0,
+        -1.0 / 39916800.0, 1.0 / 362880.0, -1.0 / 5040.0, 1.0 / 120.0, -1.0 / 6.0, 1.0)
+_COS = (1.0 / 6402373705728000.0, -1.0 / 20922789888000.0, 1.0 / 87178291200.0, -1.0 / 479001600.0,
+        1.0 / 3628800.0, -1.0 / 40320.0, 1.0 / 720.0, -1.0 / 24.0, 1.0 / 2.0)
+
+
+def _series_cos2pi(u: np.ndarray) -> np.ndarray:
+    """cos(2 pi u), u in [0, 1): exact reductions to v in [0, 1/8] turns, then
+    the Taylor series of sin or cos at x = 2 pi v (the generators' sequence)."""
+    u = np.where(u >= 0.5, 1.0 - u, u)
+    neg = u > 0.25
+    u = np.where(neg, 0.5 - u, u)
+    sign = np.where(neg, -1.0, 1.0)
+    use_sin = u > 0.125
+    v = np.where(use_sin, 0.25 - u, u)
+    x = 6.28318530717958647692 * v
+    x2 = x * x
+    ps = np.full_like(x, _SIN[0])
+    for c in _SIN[1:]:
+        ps = ps * x2 + c
+    pc = np.full_like(x, _COS[0])
+    for c in _COS[1:]:
+        pc = pc * x2 + c
+    return sign * np.where(use_sin, x * ps, 1.0 - x2 * pc)
+
+
+def jitter_normal(seed: int, stream: int, n: int) -> np.ndarray:
+    """Normal draws 0..n-1 of the scene generators (Box-Muller from streams
+    `stream` and stream + 1000003, in correctly rounded operations only)."""
+    u1 = uniform(seed, stream, n)
+    u2 = uniform(seed, stream + 1000003, n)
+    return np.sqrt(-2.0 * _series_log(1.0 - u1)) * _series_cos2pi(u2)
+
+
+def rectangle_layout(start, end, num, types, seed: int = 0, stream: int = 1000):
+    """Positions and type indices of a DemSo rectangle block
+    (createRectanglesD, src/particles_kernel.cuh:43-74): particle
+    p = x + y*num[0] at start + (end-start)/(num-1) * ((x, y) + (g - 0.5)/100)
+    in FP32, g the normal draws p of streams `stream` (x) and stream + 1 (y);
+    type types[(x + y) % len(types)]."""
+    f32 = np.float32
+    nx_, ny_ = int(num[0]), int(num[1])
+    n = nx_ * ny_
+    p = np.arange(n)
+    x = (p % nx_).astype(np.uint32)
+    y = (p // nx_).astype(np.uint32)
+    gx = jitter_normal(seed, stream, n).astype(f32)
+    gy = jitter_normal(seed, stream + 1, n).astype(f32)
+    pos = np.empty((n, 2), f32)
+    with np.errstate(divide="ignore", invalid="ignore"):  # a single row or column divides by zero, as DemSo does
+        dx = f32(f32(end[0]) - f32(start[0])) / f32(nx_ - 1)
+        dy = f32(f32(end[1]) - f32(start[1])) / f32(ny_ - 1)
+        pos[:, 0] = f32(start[0]) + dx * (x.astype(f32) + (gx - f32(0.5)) / f32(100))
+        pos[:, 1] = f32(start[1]) + dy * (y.astype(f32) + (gy - f32(0.5)) / f32(100))
+    tv = np.asarray(types, np.uint32)
+    return pos, tv[(x + y) % len(tv)]
+
+
 def restitution_ratio(zeta: float) -> float:
     """Coefficient of restitution of a linear spring with a dashpot that acts
     only while the discs approach (src/particles_kernel.cuh:281): the
@@ -335,6 +413,30 @@ def reference_grid_size(box: Tuple[float, float], max_radius: float) -> Tuple[in
     return axis(box[0]), axis(box[1])
 
 
+def triangle_layout(start, N: int, space, height, types, count: Optional[int] = None):
+    """Positions and type indices of a DemSo triangle block
+    (createTrianglesD, src/particles_kernel.cuh:76-111) in its FP32
+    arithmetic: particle `index` sits in row L (from the base) and column C,
+    L1 = int((1 + 2 sqrtf(N^2 + N - 2(index+1) + 1)) / 2), L = N - L1,
+    s = (L1 + N + N^2 - L1^2) / 2, C = index - s + L1, at
+    (start.x + (L + 2C - (N-1)) space / 2, start.y + L height); type
+    types[(C + L) % len(types)]."""
+    f32 = np.float32
+    count = N * (N + 1) // 2 if count is None else count
+    index = np.arange(count, dtype=np.int64)
+    radicand = (N * N + N - 2 * (index + 1) + 1).astype(f32)
+    L1 = ((f32(1) + f32(2) * np.sqrt(radicand)) / f32(2)).astype(np.int64)
+    L = N - L1
+    sidx = (L1 + N + N * N - L1 * L1) // 2
+    C = index - sidx + L1
+    A = N - 1
+    pos = np.empty((count, 2), f32)
+    pos[:, 0] = f32(start[0]) + (L + 2 * C - A).astype(f32) * f32(space) / f32(2)
+    pos[:, 1] = f32(start[1]) + L.astype(f32) * f32(height)
+    tv = np.asarray(types, np.uint32)
+    return pos, tv[(C + L) % len(tv)]
+
+
 def triangle_pile(rows: int = 16, start: Tuple[float, float] = (5.0, 0.125), width: float = 3.75,
                   radius: float = 0.125, mass: float = 1e-2, kn: float = 1000.0, kt: float = 1000.0,
                   cn: float = 0.2, bdamp: float = 0.5, mu: float = 0.6, attract: float = 10.0,
@@ -350,18 +452,10 @@ def triangle_pile(rows: int = 16, start: Tuple[float, float] = (5.0, 0.125), wid
     PrepareSim does."""
     f32 = np.float32
     N = rows
-    count = N * (N + 1) // 2
     space = f32(width) / f32(N - 1)
     height = f32(float(space) * math.sqrt(3.0) / 2.0)
-    pos = np.zeros((count, 2), np.float32)
-    for index in range(count):
-        L1 = int((1 + 2 * float(f32(math.sqrt(float(f32(N * N + N - 2 * (index + 1) + 1)))))) / 2)
-        L = N - L1
-        s = (L1 + N + N * N - L1 * L1) // 2
-        C = index - s + L1
-        A = N - 1
-        pos[index, 0] = f32(start[0]) + f32(L + 2 * C - A) * space / f32(2)
-        pos[index, 1] = f32(start[1]) + f32(L) * height
+    pos, _ = triangle_layout(start, N, space, height, [0])
+    count = pos.shape[0]
     inertia = float(f32(2) * f32(mass) * f32(radius) * f32(radius) / f32(5))
     mat = Material(radius=radius, mass=mass, inertia=inertia, kn=kn, kt=kt, cn=cn, bdamp=bdamp, mu=mu,
                    attract=attract)

@@ -219,7 +219,8 @@ typedef struct dem_sim dem_sim;
  * outlive the sim. */
 int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_radius, dem_sim** out);
 int dem_sim_destroy(dem_sim* sim);
-/* Host arrays -> current device state (radius may be NULL without radius). */
+/* Host or device arrays -> current device state (radius may be NULL without
+ * radius). */
 int dem_sim_upload(dem_sim* sim, const float* pos, const float* vel, const float* theta,
                    const float* omega, const uint32_t* id, const uint32_t* type, const uint32_t* loc,
                    const float* radius);
@@ -288,6 +289,37 @@ int dem_gather_rows(dem_context* ctx, const uint32_t* follow_ids, uint32_t count
                     uint32_t numParticles, uint32_t* rows);
 /* The same rows from the sim's current state. */
 int dem_sim_gather_rows(dem_sim* sim, const uint32_t* follow_ids, uint32_t count, uint32_t* rows);
+/* ---- scene generators (src/functions.cu:198-311) ---------------------------- */
+/* Stream of the x draws of dem_create_rectangles (y: DEM_RECT_STREAM + 1). */
+#define DEM_RECT_STREAM 1000u
+/* createRectangles: side[0] x side[1] particles p = x + y*side[0] at
+ * start + sideLength/(side-1) * ((x, y) + (normal - 0.5)/100), ID = loc =
+ * startID + p, type = particleTypeVec[(x+y) % numParticleTypes] (host array).
+ * The reference's curand_normal draws (seeded with time(NULL)) are replaced
+ * by reproducible normals: draw p of streams DEM_RECT_STREAM (x) and
+ * DEM_RECT_STREAM + 1 (y) under `seed`.  pos/ID/loc/type are device arrays
+ * that start at the rectangle's first particle.  Synchronizes. */
+int dem_create_rectangles(dem_context* ctx, float* pos, uint32_t* ID, uint32_t* loc, uint32_t* type,
+                          const float start[2], const float sideLength[2], const uint32_t side[2], uint32_t startID,
+                          uint32_t numParticleTypes, const uint32_t* particleTypeVec, uint64_t seed);
+/* The same with the x draws from `stream` and the y draws from stream + 1
+ * (the driver gives rectangle r the streams 1000 + 2r, 1001 + 2r). */
+int dem_create_rectangles_ex(dem_context* ctx, float* pos, uint32_t* ID, uint32_t* loc, uint32_t* type,
+                             const float start[2], const float sideLength[2], const uint32_t side[2],
+                             uint32_t startID, uint32_t numParticleTypes, const uint32_t* particleTypeVec,
+                             uint64_t seed, uint64_t stream);
+/* createTriangles: numParticles (at most N(N+1)/2) particles of an N-row
+ * triangle, row spacing `height`, particle spacing `space`; type =
+ * particleTypeVec[(C+L) % numParticleTypes].  Synchronizes. */
+int dem_create_triangles(dem_context* ctx, float* pos, uint32_t* ID, uint32_t* loc, uint32_t* type,
+                         const float start[2], uint32_t N, uint32_t numParticleTypes, const uint32_t* particleTypeVec,
+                         float space, float height, uint32_t startID, uint32_t numParticles);
+/* createSingleParticles: copies the given particles (host or device arrays)
+ * and numbers them startID, startID + 1, ...  Synchronizes. */
+int dem_create_single_particles(dem_context* ctx, float* pos, float* vel, float* theta, float* omega, uint32_t* ID,
+                                uint32_t* loc, uint32_t* type, const float* usrPos, const float* usrVel,
+                                const float* usrTheta, const float* usrOmega, const uint32_t* usrType,
+                                uint32_t numParticles, uint32_t startID);
 /* ---- controlled particle -------------------------------------------------- */
 /* Owned simulations of ctx: with state->enabled, every step adds the pair
  * with the controlled particle to each particle's force, reduces the

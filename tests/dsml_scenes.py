@@ -109,3 +109,20 @@ def python_scene(dump: dict, type_ids, box, dt: float, bkn: float, bkt: float, g
                     type=np.ascontiguousarray(dump["type"]), loc=ids.copy(), radius=None, materials=mats,
                     instances=[sc.Instance(box=box, extent=box, grid=grid)], dt=dt, gravity=gravity, bkn=bkn,
                     bkt=bkt, flags=0)
+
+
+def big_blocks(rect=(300, 200), rows=400) -> str:
+    """MIXED with a large rectangle, a second rectangle (draw streams 1002 and
+    1003) and a large triangle: tens of thousands of generator draws."""
+    text = MIXED.replace("<num><x>10</x><y>6</y></num>", f"<num><x>{rect[0]}</x><y>{rect[1]}</y></num>")
+    text = text.replace("<num>10</num>", f"<num>{rows}</num>")
+    second = """    <rectangle particletype="big,a,b">
+      <start><x>2.0</x><y>1.5</y></start>
+      <end><x>3.5</x><y>3.5</y></end>
+      <num><x>37</x><y>53</y></num>
+    </rectangle>
+"""
+    return text.replace("    <triangle", second + "    <triangle", 1)
+
+
+BIG_BLOCKS_COUNTS = (2, 300 * 200, 37 * 53, 400 * 401 // 2)

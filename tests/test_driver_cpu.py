@@ -97,12 +97,14 @@ def test_mixed_scene_layout(tmp_path):
     assert d["type"][ns:ns + nr].tolist() == [[0, 1][(i + j) % 2] for i, j in zip(x, y)]
     dx = f32(f32(1.6) - f32(0.5)) / f32(9)
     dy = f32(f32(1.2) - f32(0.5)) / f32(5)
-    nx = sc.normal(7, 1000, nr).astype(np.float32)
-    ny = sc.normal(7, 1001, nr).astype(np.float32)
+    nx = sc.jitter_normal(7, 1000, nr).astype(np.float32)
+    ny = sc.jitter_normal(7, 1001, nr).astype(np.float32)
     ex = f32(0.5) + dx * (x.astype(f32) + (nx - f32(0.5)) / f32(100))
     ey = f32(0.5) + dy * (y.astype(f32) + (ny - f32(0.5)) / f32(100))
     got = d["pos"][ns:ns + nr]
     assert np.array_equal(got[:, 0], ex) and np.array_equal(got[:, 1], ey)
+    pos, types = sc.rectangle_layout((0.5, 0.5), (1.6, 1.2), (10, 6), [0, 1], seed=7, stream=1000)
+    assert np.array_equal(pos, got) and np.array_equal(types, d["type"][ns:ns + nr])
     # triangle: 10 rows, base centred on (3, 0.1), rows 1/(N-1)*sqrt(3)/2 apart
     tri = d["pos"][ns + nr:]
     ref = sc.triangle_pile(rows=10, start=(3.0, 0.1), width=1.0)
