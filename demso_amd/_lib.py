@@ -94,6 +94,19 @@ class ControlledProps(ctypes.Structure):
     ]
 
 
+class RenderProps(ctypes.Structure):
+    """dem_render_params."""
+
+    _fields_ = [
+        ("image_dim", ctypes.c_int32 * 2),
+        ("control_dim", ctypes.c_int32 * 2),
+        ("bg_color", ctypes.c_int32),
+        ("view_rotations", c_uint32),
+        ("color_by_pressure", c_uint32),
+        ("max_pressure", c_float),
+    ]
+
+
 P = c_void_p  # every device pointer travels as a plain address
 
 # name -> (restype, argtypes)
@@ -147,6 +160,11 @@ PROTOTYPES = {
         c_int, [c_void_p] + [P] * 4 + [POINTER(c_float), c_uint32, c_uint32, POINTER(c_uint32), c_float, c_float,
                                        c_uint32, c_uint32]),
     "dem_create_single_particles": (c_int, [c_void_p] + [P] * 12 + [c_uint32, c_uint32]),
+    "dem_plot_particles": (c_int, [c_void_p] + [P] * 5 + [c_uint32, POINTER(RenderProps),
+                                                        POINTER(ControlledProps), P]),
+    "dem_sim_plot": (c_int, [c_void_p, P, POINTER(RenderProps)]),
+    "dem_update_pressure_scale": (c_int, [c_void_p, P, c_uint32, POINTER(c_float)]),
+    "dem_sim_pressure_scale": (c_int, [c_void_p, POINTER(c_float)]),
     "dem_set_controlled": (c_int, [c_void_p, POINTER(ControlledProps)]),
     "dem_get_controlled": (c_int, [c_void_p, POINTER(ControlledProps), POINTER(c_float)]),
     "dem_collide_controlled": (

@@ -82,7 +82,7 @@ def particle_props(scene: Scene, single_instance: bool) -> ctypes.Array:
         p.radius, p.mass, p.inertia = m.radius, m.mass, m.inertia
         p.normal_stiffness, p.shear_stiffness, p.normal_damping = m.kn, m.kt, m.cn
         p.boundary_damping, p.friction_coefficient, p.attract_coefficient = m.bdamp, m.mu, m.attract
-        p.color_r = p.color_g = p.color_b = 255.0
+        p.color_r, p.color_g, p.color_b = (float(v) for v in m.color)
         p.areal_density = m.areal
         p.instance = 0 if single_instance else m.instance
     return arr
@@ -194,6 +194,58 @@ class Context:
 
 
 # ---- the five stages, reference names ----------------------------------------
+
+def render_params(scene: Scene, image_height: int = 800, controlled_type: Optional[int] = None, bg_color: int = 0,
+                  view_rotations: bool = True, color_by_pressure: bool = False, max_pressure: float = 1.0) -> dict:
+    """DemSo's RenderParameters for a scene (src/main.cu:172-177, 209-215):
+    imageDIMx = Lx/Ly * imageDIMy and the controlled particle's square of
+    2 ceil(pixels per metre * its radius) (at least 2), in FP32."""
+    f32 = np.float32
+    lx, ly = (f32(v) for v in scene.instances[0].box)
+    dimy = int(image_height)
+    dimx = int(lx / ly * f32(dimy))
+    cdim = [0, 0]
+    if controlled_type is not None:
+        rc = f32(scene.materials[controlled_type].radius)
+        cdim = [max(int(np.ceil(f32(dimx) / lx * rc) * f32(2)), 2), max(int(np.ceil(f32(dimy) / ly * rc) * f32(2)), 2)]
+    return {"image_dim": [dimx, dimy], "control_dim": cdim, "bg_color": int(bg_color),
+            "view_rotations": bool(view_rotations), "color_by_pressure": bool(color_by_pressure),
+            "max_pressure": float(max_pressure)}
+
+
+def _render_struct(rp: dict):
+    r = _lib.RenderProps()
+    r.image_dim[:] = rp["image_dim"]
+    r.control_dim[:] = rp["control_dim"]
+    r.bg_color = rp["bg_color"]
+    r.view_rotations = int(rp["view_rotations"])
+    r.color_by_pressure = int(rp["color_by_pressure"])
+    r.max_pressure = rp["max_pressure"]
+    return r
+
+
+def plotParticles(ctx: Context, image, pos, theta, type, numParticles: int, rp: dict, pressure, radius=None,
+                  control: Optional[dict] = None) -> None:
+    """plotParticles (src/functions.cu:557-600) into a device uint8 tensor of
+    image_dim[1] x image_dim[0] x 4 (RGBA, row 0 at the bottom).  control:
+    the controlled particle {"pos", "type"} or None."""
+    c = None
+    if control is not None:
+        c = _lib.ControlledProps()
+        c.pos[:] = [float(control["pos"][0]), float(control["pos"][1])]
+        c.type, c.enabled = int(control["type"]), 1
+    r = _render_struct(rp)
+    ctx._check(ctx.lib.dem_plot_particles(ctx.handle, _ptr(image), _ptr(pos), _ptr(theta), _ptr(type), _ptr(radius),
+                                          numParticles, byref(r), None if c is None else byref(c), _ptr(pressure)))
+
+
+def updatePressureScale(ctx: Context, pressure, numParticles: int) -> float:
+    """updatePressureScale (src/functions.cu:684-693): the largest pressure,
+    1 when none is positive."""
+    out = c_float()
+    ctx._check(ctx.lib.dem_update_pressure_scale(ctx.handle, _ptr(pressure), numParticles, byref(out)))
+    return float(out.value)
+
 
 def calcHash(ctx: Context, pos, gridParticleIndex, gridParticleHash, numParticles: int, type=None) -> None:
     """calcHash (src/functions.cu:315-327)."""
@@ -464,6 +516,21 @@ class Simulation:
     def set_by_id(self, pos=None, vel=None) -> None:
         """Overwrite positions / velocities from (n, 2) GPU tensors indexed by id."""
         self.ctx._check(self.lib.dem_sim_set_by_id(self.handle, _ptr(pos), _ptr(vel)))
+
+    def pressure_scale(self) -> float:
+        """updatePressureScale over the current pressure."""
+        out = c_float()
+        self.ctx._check(self.lib.dem_sim_pressure_scale(self.handle, byref(out)))
+        return float(out.value)
+
+    def plot(self, rp: dict) -> np.ndarray:
+        """The image of the current state (dem_sim_plot), (dimy, dimx, 4) uint8."""
+        import torch
+
+        img = torch.empty((rp["image_dim"][1], rp["image_dim"][0], 4), dtype=torch.uint8, device="cuda")
+        r = _render_struct(rp)
+        self.ctx._check(self.lib.dem_sim_plot(self.handle, _ptr(img), byref(r)))
+        return img.cpu().numpy()
 
     def synchronize(self) -> None:
         self.ctx.synchronize()

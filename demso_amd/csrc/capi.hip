@@ -160,6 +160,21 @@ __global__ void k_copy2(const uint32_t* __restrict__ a, const uint32_t* __restri
 
 }  // namespace
 
+namespace {
+
+// Particle colours (colorR/G/B of ParticleProperties), kept for images.
+std::vector<float> colors_of(const dem_particle_props* types, uint32_t num_types) {
+  std::vector<float> c(3 * (size_t)num_types);
+  for (uint32_t t = 0; t < num_types; ++t) {
+    c[3 * t] = types[t].color_r;
+    c[3 * t + 1] = types[t].color_g;
+    c[3 * t + 2] = types[t].color_b;
+  }
+  return c;
+}
+
+}  // namespace
+
 extern "C" {
 
 int dem_abi_version(void) { return DEMSO_ABI_VERSION; }
@@ -197,6 +212,9 @@ int dem_destroy(dem_context* ctx) {
   if (ctx->d_control) (void)hipFree(ctx->d_control);
   if (ctx->d_stage_control) (void)hipFree(ctx->d_stage_control);
   if (ctx->d_typevec) (void)hipFree(ctx->d_typevec);
+  if (ctx->d_render_types) (void)hipFree(ctx->d_render_types);
+  if (ctx->d_owner) (void)hipFree(ctx->d_owner);
+  if (ctx->d_scalar) (void)hipFree(ctx->d_scalar);
   free_sort(ctx->sort);
   delete ctx;
   return DEM_OK;
@@ -252,6 +270,7 @@ int dem_set_params(dem_context* ctx, const dem_system_props* sys, const dem_part
   uint32_t total = 0;
   if (int r = validate_instances(ctx, inst, sys->flags, &total)) return r;
   ctx->h_mat = mat;
+  ctx->h_color = colors_of(types, num_types);
   ctx->h_inst = inst;
   ctx->c.dt = sys->time_step;
   ctx->c.gx = sys->gravity[0];
@@ -274,6 +293,7 @@ int dem_set_materials(dem_context* ctx, const dem_particle_props* types, uint32_
   if (int r = convert_materials(ctx, types, num_types, (uint32_t)ctx->h_inst.size(), mat, attract)) return r;
   if (int r = ensure_device(ctx)) return r;
   ctx->h_mat = mat;
+  ctx->h_color = colors_of(types, num_types);
   ctx->c.attract = attract ? 1u : 0u;
   return upload_tables(ctx);
 }

@@ -39,6 +39,15 @@ struct dem_context {
   // particle type vector of the scene generators (grown as needed)
   uint32_t* d_typevec = nullptr;
   uint32_t typevec_cap = 0;
+  // images: particle colours (r, g, b per material row), the per-type
+  // render table, the pixel owner map and a one-word result (grown as needed)
+  std::vector<float> h_color;
+  uint32_t* d_render_types = nullptr;
+  size_t render_types_cap = 0;
+  uint32_t* d_owner = nullptr;
+  size_t owner_cap = 0;
+  uint32_t* d_scalar = nullptr;
+  size_t scalar_cap = 0;
 };
 
 // Buffer of the owned simulation's dynamic fields (pos, vel, theta, omega)

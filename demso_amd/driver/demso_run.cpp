@@ -35,6 +35,12 @@
 //   --device D         HIP device (default 0)
 //   --print-config     print the loaded scene and exit (no GPU)
 //   --dump-initial F   write the prepared initial state to F and exit (no GPU)
+//   --image-every K    write the frame of plotParticles every K steps as a
+//                      binary PPM (P6, top row first) named <prefix><step>.ppm
+//   --image-prefix P   image file prefix (default "frame_")
+//   --color-by-pressure  colour by pressure / the frame's largest pressure
+//                      (updatePressureScale before each frame)
+//   --no-rotations     no rotation lines;  --bg-color B  background byte
 //
 // Dump format (little endian): "DEMSODMP", u32 version 1, u32 n, u64 step,
 // f64 time, then arrays in particle-ID order: pos f32[n][2], vel f32[n][2],
@@ -93,12 +99,18 @@ struct Options {
   bool print_config = false;
   std::string dump_initial;
   bool graph = false;
+  long long image_every = 0;
+  std::string image_prefix = "frame_";
+  bool color_by_pressure = false;
+  bool rotations = true;
+  int bg_color = 0;
 };
 
 [[noreturn]] void usage(const char* msg) {
   std::fprintf(stderr, "demso_run: %s\nusage: demso_run [--steps N] [--output FILE] [--dump-every K] [--dump-final] "
                        "[--dump-prefix P] [--seed S] [--controlled] [--loop stages|graph] [--device D] "
-                       "[--print-config] [--dump-initial FILE] scene.dsml\n",
+                       "[--print-config] [--dump-initial FILE] [--image-every K] [--image-prefix P] [--color-by-pressure] "
+                       "[--no-rotations] [--bg-color B] scene.dsml\n",
                msg);
   std::exit(2);
 }
@@ -128,6 +140,11 @@ Options parse_args(int argc, char** argv) {
     else if (a == "--device") o.device = (int)integer();
     else if (a == "--print-config") o.print_config = true;
     else if (a == "--dump-initial") o.dump_initial = value();
+    else if (a == "--image-every") o.image_every = integer();
+    else if (a == "--image-prefix") o.image_prefix = value();
+    else if (a == "--color-by-pressure") o.color_by_pressure = true;
+    else if (a == "--no-rotations") o.rotations = false;
+    else if (a == "--bg-color") o.bg_color = (int)integer();
     else if (a == "--loop") {
       const std::string v = value();
       if (v == "graph") o.graph = true;
@@ -281,6 +298,33 @@ int create_on_device(dem_context* ctx, const dsml::PreparedScene& ps, float* pos
   return 0;
 }
 
+// One frame: the image of plotParticles (device RGBA, row 0 at the bottom)
+// as a P6 PPM with the top row first.
+struct Frames {
+  uint8_t* d_img = nullptr;
+  std::vector<uint8_t> h_img, rgb;
+  ~Frames() {
+    if (d_img) (void)hipFree(d_img);
+  }
+  bool write(const std::string& path, const dem_render_params& rp) {
+    const size_t w = (size_t)rp.image_dim[0], h = (size_t)rp.image_dim[1];
+    h_img.resize(4 * w * h);
+    if (hipMemcpy(h_img.data(), d_img, h_img.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    rgb.resize(3 * w * h);
+    for (size_t y = 0; y < h; ++y)
+      for (size_t x = 0; x < w; ++x)
+        for (int c = 0; c < 3; ++c) rgb[3 * ((h - 1 - y) * w + x) + c] = h_img[4 * (y * w + x) + c];
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+      std::fprintf(stderr, "demso_run: cannot write %s\n", path.c_str());
+      return false;
+    }
+    std::fprintf(f, "P6\n%zu %zu\n255\n", w, h);
+    const bool ok = std::fwrite(rgb.data(), 1, rgb.size(), f) == rgb.size();
+    return (std::fclose(f) == 0) && ok;
+  }
+};
+
 struct Run {
   const Options& opt;
   const dsml::PreparedScene& ps;
@@ -339,6 +383,15 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
   }
 
   HostState host(n);
+  Frames frames;
+  dem_render_params rp = ps.render;
+  rp.view_rotations = opt.rotations ? 1u : 0u;
+  rp.color_by_pressure = opt.color_by_pressure ? 1u : 0u;
+  rp.bg_color = opt.bg_color;
+  if (opt.image_every > 0)
+    HIP_OK(hipMalloc((void**)&frames.d_img, 4 * (size_t)rp.image_dim[0] * (size_t)rp.image_dim[1]));
+  auto want_image = [&](long long step) { return opt.image_every > 0 && step % opt.image_every == 0; };
+  auto image_name = [&](long long step) { return opt.image_prefix + std::to_string(step) + ".ppm"; };
   auto dump_name = [&](long long step) { return opt.dump_prefix + std::to_string(step) + ".bin"; };
   auto want_dump = [&](long long step) {
     return (opt.dump_every > 0 && step % opt.dump_every == 0) || (opt.dump_final && step == steps);
@@ -442,6 +495,19 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
         if (ctrl_pos.y < -1) ctrl_pos.y = 10.5;
         if (ctrl_pos.x > sis.cubeDimension.x + 0.5) ctrl_pos.x = -0.5;
       }
+      if (want_image(tempo)) {
+        // plotParticles after the step (src/main.cu:538-553)
+        if (rp.color_by_pressure) DEM_OK_OR_FAIL(dem_update_pressure_scale(ctx, b.pressure, n, &rp.max_pressure), ctx);
+        dem_controlled c{};
+        c.pos[0] = ctrl_pos.x;
+        c.pos[1] = ctrl_pos.y;
+        c.type = control ? (uint32_t)ps.controlled_type : 0u;
+        c.enabled = control ? 1u : 0u;
+        DEM_OK_OR_FAIL(dem_plot_particles(ctx, frames.d_img, b.pos[t], b.theta[t], b.type[t], nullptr, n, &rp,
+                                          control ? &c : nullptr, b.pressure),
+                       ctx);
+        if (!frames.write(image_name(tempo), rp)) return 1;
+      }
       cur = t;
       if (run.d_rows) {
         if (run.pending == 0) run.first_pending_step = tempo;
@@ -533,6 +599,7 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
     long long next = steps;
     if (run.d_rows) next = done + 1;
     else if (opt.dump_every > 0) next = std::min(steps, (done / opt.dump_every + 1) * opt.dump_every);
+    if (opt.image_every > 0) next = std::min(next, (done / opt.image_every + 1) * opt.image_every);
     DEM_OK_OR_FAIL(dem_sim_step_graph(sim, (uint32_t)(next - done)), ctx);
     done = next;
     if (run.d_rows) {
@@ -541,6 +608,11 @@ int run_sim(const Options& opt, const dsml::PreparedScene& ps, long long steps) 
                                          run.d_rows + run.pending * count * DEM_ROW_WORDS),
                      ctx);
       if (++run.pending == run.batch && run.flush_rows(s)) return 1;
+    }
+    if (want_image(done)) {
+      if (rp.color_by_pressure) DEM_OK_OR_FAIL(dem_sim_pressure_scale(sim, &rp.max_pressure), ctx);
+      DEM_OK_OR_FAIL(dem_sim_plot(sim, frames.d_img, &rp), ctx);
+      if (!frames.write(image_name(done), rp)) return 1;
     }
     if (want_dump(done)) {
       if (run.flush_rows(s)) return 1;

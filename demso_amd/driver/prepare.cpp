@@ -90,6 +90,22 @@ PreparedScene prepare_sim(const DEMSimulation& sim, uint64_t seed) {
   }
   out.max_radius = max_radius;
   out.controlled_type = P.controlledType;
+  {
+    dem_render_params& rp = out.render;
+    rp.image_dim[1] = (int32_t)sim.parameters.imageDIMy;
+    rp.image_dim[0] = (int32_t)(s.cube_dimension[0] / s.cube_dimension[1] * (float)rp.image_dim[1]);
+    if (P.controlledType >= 0) {
+      const float plot_radius = out.types[P.controlledType].radius;
+      rp.control_dim[0] = (int32_t)(std::ceil((float)rp.image_dim[0] / s.cube_dimension[0] * plot_radius) * 2.0f);
+      if (rp.control_dim[0] < 2) rp.control_dim[0] = 2;
+      rp.control_dim[1] = (int32_t)(std::ceil((float)rp.image_dim[1] / s.cube_dimension[1] * plot_radius) * 2.0f);
+      if (rp.control_dim[1] < 2) rp.control_dim[1] = 2;
+    }
+    rp.bg_color = 0;
+    rp.view_rotations = 1;
+    rp.color_by_pressure = 0;
+    rp.max_pressure = 1.0f;
+  }
   reference_grid(s.cube_dimension[0], s.cube_dimension[1], max_radius, &s.grid_size[0], &s.grid_size[1]);
   s.num_cells = s.grid_size[0] * s.grid_size[1];
 

@@ -46,6 +46,9 @@ struct PreparedScene {
   // num_singles) are the single particles (host arrays above), then the
   // rectangles and triangles
   uint64_t num_singles = 0, seed = 0;
+  // RenderParameters (src/main.cu:172-177, 209-215): image size and the
+  // controlled particle's pixel square
+  dem_render_params render{};
   std::vector<RectBlock> rects;
   std::vector<TriBlock> tris;
 };

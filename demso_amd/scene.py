@@ -192,6 +192,7 @@ class Material:
     attract: float = 0.0
     areal: float = 0.0
     instance: int = 0
+    color: Tuple[float, float, float] = (255.0, 255.0, 255.0)  # colorR/G/B, 0..255
 
 
 @dataclass

@@ -320,6 +320,36 @@ int dem_create_single_particles(dem_context* ctx, float* pos, float* vel, float*
                                 uint32_t* loc, uint32_t* type, const float* usrPos, const float* usrVel,
                                 const float* usrTheta, const float* usrOmega, const uint32_t* usrType,
                                 uint32_t numParticles, uint32_t startID);
+/* ---- images (src/functions.cu:557-600, 684-693) ---------------------------- */
+/* RenderParameters (src/main.cuh:124-135).  DemSo's values: image_dim =
+ * (Lx/Ly * imageheight, imageheight), control_dim = 2 ceil(pixels per metre *
+ * controlled radius) (at least 2), bg_color 0, view_rotations 1,
+ * color_by_pressure 0. */
+typedef struct dem_render_params {
+  int32_t image_dim[2];      /* imageDIMx, imageDIMy */
+  int32_t control_dim[2];    /* dimx, dimy */
+  int32_t bg_color;          /* byte written to every channel of the background */
+  uint32_t view_rotations;   /* a black line from each centre along theta */
+  uint32_t color_by_pressure;
+  float max_pressure;        /* pressure scale (dem_update_pressure_scale) */
+} dem_render_params;
+/* plotParticles: an RGBA image (device, image_dim[0] * image_dim[1] * 4 bytes,
+ * row 0 at the bottom) of the particles in array order, coloured by type or
+ * by pressure / max_pressure, shaded towards the rim, with rotation lines and
+ * the controlled particle (host state, NULL or not enabled: none) on top.
+ * Where discs overlap, the higher array index wins (the reference leaves it
+ * to thread timing).  radius may be NULL (type radii).  Synchronizes. */
+int dem_plot_particles(dem_context* ctx, uint8_t* image, const float* pos, const float* theta, const uint32_t* type,
+                       const float* radius, uint32_t numParticles, const dem_render_params* rp,
+                       const dem_controlled* control, const float* pressure);
+/* The same image of an owned simulation's current state (sorted order, its
+ * controlled particle if enabled). */
+int dem_sim_plot(dem_sim* sim, uint8_t* image, const dem_render_params* rp);
+/* updatePressureScale: the largest pressure of the n values (NaN ignored),
+ * or 1 if none is positive, to the host.  Synchronizes. */
+int dem_update_pressure_scale(dem_context* ctx, const float* pressure, uint32_t numParticles, float* maxPressure);
+/* The same over the owned simulation's current pressure. */
+int dem_sim_pressure_scale(dem_sim* sim, float* maxPressure);
 /* ---- controlled particle -------------------------------------------------- */
 /* Owned simulations of ctx: with state->enabled, every step adds the pair
  * with the controlled particle to each particle's force, reduces the

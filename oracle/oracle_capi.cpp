@@ -3,6 +3,7 @@
 // TEST INFRASTRUCTURE ONLY (see dem_oracle.hpp).  Loaded by the tests, the
 // smoke check and the benchmark's CPU-baseline leg; never by the product.
 #include "adjoint_oracle.hpp"
+#include "render_oracle.hpp"
 
 #include <chrono>
 #include <new>
@@ -320,5 +321,14 @@ int orc_adj_backward(void* p, const double* gpos, const double* gvel, double* gp
   static_cast<orc::AdjointOracle*>(p)->backward(gpos, gvel, gpos0, gvel0, gparams);
   return 0;
 }
+
+// images: plotParticles and updatePressureScale (render_oracle.hpp)
+void orc_plot_particles(uint8_t* img, const float* pos, const float* theta, const uint32_t* type, const float* radius,
+                        uint32_t n, const orc::RenderType* types, uint32_t num_types, const orc::RenderParams* rp,
+                        const orc::Control* ctl, const float* pressure) {
+  orc::plot(img, pos, theta, type, radius, n, types, num_types, *rp, ctl, pressure);
+}
+
+float orc_pressure_scale(const float* p, uint32_t n) { return orc::pressure_scale(p, n); }
 
 }  // extern "C"

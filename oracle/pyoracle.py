@@ -83,6 +83,8 @@ def load():
         "orc_adj_forward": (c_int, [c_void_p, c_uint32]),
         "orc_adj_final_state": (c_int, [c_void_p, P, P]),
         "orc_adj_backward": (c_int, [c_void_p] + [P] * 5),
+        "orc_plot_particles": (None, [P] * 5 + [c_uint32, P, c_uint32, P, c_void_p, P]),
+        "orc_pressure_scale": (ctypes.c_float, [P, c_uint32]),
     }
     for name, (res, args) in proto.items():
         fn = getattr(lib, name)
@@ -339,3 +341,41 @@ class AdjointOracle:
         gpar = np.empty((self.nt, 4), np.float64)
         self.lib.orc_adj_backward(self.h, _p(gpos), _p(gv), _p(gp0), _p(gv0), _p(gpar))
         return gp0, gv0, gpar
+
+
+# ---- images ---------------------------------------------------------------------------
+
+class RenderParams(ctypes.Structure):
+    """orc::RenderParams (render_oracle.hpp)."""
+
+    _fields_ = [("dimx", ctypes.c_int32), ("dimy", ctypes.c_int32), ("cdimx", ctypes.c_int32),
+                ("cdimy", ctypes.c_int32), ("bg", ctypes.c_int32), ("rotations", c_uint32),
+                ("by_pressure", c_uint32), ("max_pressure", c_float), ("box", c_float * 2)]
+
+
+def plot(scene, rp: dict, pos, theta, type_, pressure, radius=None, ctl: Optional["Control"] = None) -> np.ndarray:
+    """The image (dimy, dimx, 4) uint8 of plotParticles for arrays in the
+    given order; rp: image_dim, control_dim, bg_color, view_rotations,
+    color_by_pressure, max_pressure (as demso_amd.api.render_params)."""
+    lib = load()
+    types = np.array([[m.radius, *m.color] for m in scene.materials], np.float32)
+    r = RenderParams()
+    r.dimx, r.dimy = rp["image_dim"]
+    r.cdimx, r.cdimy = rp["control_dim"]
+    r.bg, r.rotations, r.by_pressure = rp["bg_color"], int(rp["view_rotations"]), int(rp["color_by_pressure"])
+    r.max_pressure = rp["max_pressure"]
+    r.box[:] = scene.instances[0].box
+    img = np.empty((r.dimy, r.dimx, 4), np.uint8)
+    n = pos.shape[0]
+    c = lambda a, dt: None if a is None else np.ascontiguousarray(a, dt)
+    arrs = [c(pos, np.float32), c(theta, np.float32), c(type_, np.uint32), c(radius, np.float32),
+            c(pressure, np.float32)]
+    lib.orc_plot_particles(_p(img), _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]), n, _p(types),
+                           len(scene.materials), ctypes.byref(r), None if ctl is None else ctypes.byref(ctl),
+                           _p(arrs[4]))
+    return img
+
+
+def pressure_scale(p) -> float:
+    a = np.ascontiguousarray(p, np.float32)
+    return float(load().orc_pressure_scale(_p(a), a.size))

@@ -122,3 +122,47 @@ def test_reference_pile_scene_matches_oracle(tmp_path, oracle, loop):
     # pilha.dsml follows IDs 500 and 2200 of 136 discs: reported, nothing written
     assert "followed particle 500 does not exist" in r.stderr
     assert (tmp_path / "out.csv").read_text() == ""
+
+
+def read_ppm(path):
+    data = path.read_bytes()
+    parts = data.split(b"\n", 3)
+    assert parts[0] == b"P6" and parts[2] == b"255"
+    w, h = map(int, parts[1].split())
+    return np.frombuffer(parts[3], np.uint8).reshape(h, w, 3)
+
+
+@pytest.mark.parametrize("loop", ["stages", "graph"])
+def test_driver_frames_match_oracle(tmp_path, oracle, loop):
+    """--image-every: pilha.dsml's frames (800 x 800, controlled particle,
+    pressure colours) from both loops equal the oracle's image of its own
+    run, rows flipped to top-first."""
+    from pathlib import Path
+
+    from demso_amd import api
+    from demso_amd import scene as sc
+
+    steps = 1300
+    pile = Path(__file__).resolve().parent / "golden" / "dsml" / "pilha.dsml"
+    print(f"driver frames: pilha.dsml --loop {loop}", flush=True)
+    r = run_driver("--controlled", "--loop", loop, "--steps", steps, "--output", "", "--image-every", 650,
+                   "--color-by-pressure", "--image-prefix", tmp_path / "f_", pile, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_ppm(tmp_path / f"f_{steps}.ppm")
+    assert (tmp_path / "f_650.ppm").exists()
+    s = sc.triangle_pile(controlled=True)
+    s.materials[0].color = (255.0, 255.0, 255.0)   # "ffffff"
+    s.materials[1].color = (58.0, 153.0, 51.0)     # "3a9933"
+    # the frame is drawn from the state in its sorted order
+    ref = oracle.Sim(s, "f32")
+    c = s.notes["controlled"]
+    ref.set_control(oracle.control(c["pos"], c["type"], move=c["move"], wrap_below_y=c["wrap_below_y"],
+                                   wrap_above_x=c["wrap_above_x"], reset_y=c["reset_y"], reset_x=c["reset_x"]))
+    ref.step(steps)
+    st = ref.state()
+    cs, _ = ref.control()
+    ref.close()
+    rp = api.render_params(s, image_height=800, controlled_type=1, color_by_pressure=True,
+                           max_pressure=oracle.pressure_scale(st["pressure"]))
+    want = oracle.plot(s, rp, st["pos"], st["theta"], st["type"], st["pressure"], ctl=cs)
+    assert np.array_equal(got, want[::-1, :, :3])
