@@ -258,15 +258,16 @@ __device__ __forceinline__ void attract_force(const PairCtx& q, float rx, float 
 }
 
 template <bool kRadius>
-__device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, float2 pB, float rB,
-                                             uint32_t tB, const float2* __restrict__ vel,
+__device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, bool self, float2 pB,
+                                             float rB, uint32_t tB, const float2* __restrict__ vel,
                                              const float* __restrict__ omega, const uint32_t* __restrict__ type,
                                              float& cfx, float& cfy, float& moment, float& pres) {
   const float rx = pB.x - q.p.x, ry = pB.y - q.p.y;
   const float d2 = rx * rx + ry * ry;
   const float R = q.rA + rB;
   const bool maybe = !(d2 > (R * R) * 1.00000095367431640625f) || R < 1.0842022e-19f;
-  if (!(maybe || c.attract)) return;
+  // the particle itself is skipped (collideCell's j != index test)
+  if (self || !(maybe || c.attract)) return;
   const float dist = sqrtf(d2);
   if (kRadius) tB = clamp_idx(type[j], c.num_types);
   if (dist < R) {
@@ -332,7 +333,7 @@ __device__ __forceinline__ void candidate_load(const StepConst& c, uint32_t j, c
 }
 
 template <bool kRadius>
-__device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j,
+__device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
                                           const float2* __restrict__ pos, const float2* __restrict__ vel,
                                           const float* __restrict__ omega, const uint32_t* __restrict__ type,
                                           const float* __restrict__ radius, float& cfx, float& cfy,
@@ -341,7 +342,18 @@ __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, 
   float rB;
   uint32_t tB;
   candidate_load<kRadius>(c, j, pos, type, radius, pB, rB, tB);
-  candidate_at<kRadius>(c, q, j, pB, rB, tB, vel, omega, type, cfx, cfy, moment, pres);
+  candidate_at<kRadius>(c, q, j, self, pB, rB, tB, vel, omega, type, cfx, cfy, moment, pres);
+}
+
+// At the first candidate of a new cell, the previous cell's sum joins the
+// particle's total (force += collideCell(...), one cell at a time).  Written
+// without a branch: adding +0 leaves a total unchanged, as a total that
+// starts at +0 is never -0.
+__device__ __forceinline__ void cell_boundary(bool at, float& fx, float& fy, float& cfx, float& cfy) {
+  fx += at ? cfx : 0.0f;
+  fy += at ? cfy : 0.0f;
+  cfx = at ? 0.0f : cfx;
+  cfy = at ? 0.0f : cfy;
 }
 
 // Forces on sorted particle i from its 3x3 cell neighbourhood (the body of
@@ -415,13 +427,19 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
         cy = (uint32_t)iy & ((uint32_t)G1 - 1u);
       }
       const uint32_t row = in.cell_base + cy * (uint32_t)G0;
-      const uint32_t s0 = ok && vx0 ? cell_start[row + x0] : kEmpty;
-      const uint32_t s1 = ok ? cell_start[row + x1] : kEmpty;
-      const uint32_t s2 = ok && vx2 ? cell_start[row + x2] : kEmpty;
+      // All six words are loaded unconditionally from cells inside the grid
+      // (a column or row outside it reads the centre column or row 0) and
+      // then masked: no branch per load.
+      const uint32_t a0 = row + (vx0 ? x0 : x1), a1 = row + x1, a2 = row + (vx2 ? x2 : x1);
+      const uint32_t s0r = cell_start[a0], s1r = cell_start[a1], s2r = cell_start[a2];
+      const uint32_t e0r = cell_end[a0], e1r = cell_end[a1], e2r = cell_end[a2];
+      const uint32_t s0 = ok && vx0 ? s0r : kEmpty;
+      const uint32_t s1 = ok ? s1r : kEmpty;
+      const uint32_t s2 = ok && vx2 ? s2r : kEmpty;
       const bool n0 = s0 != kEmpty, n1 = s1 != kEmpty, n2 = s2 != kEmpty;
-      const uint32_t e0 = n0 ? min(cell_end[row + x0], n) : 0u;
-      const uint32_t e1 = n1 ? min(cell_end[row + x1], n) : 0u;
-      const uint32_t e2 = n2 ? min(cell_end[row + x2], n) : 0u;
+      const uint32_t e0 = n0 ? min(e0r, n) : 0u;
+      const uint32_t e1 = n1 ? min(e1r, n) : 0u;
+      const uint32_t e2 = n2 ? min(e2r, n) : 0u;
       rs[y] = n0 ? s0 : (n1 ? s1 : (n2 ? s2 : 0u));
       re[y] = n2 ? e2 : (n1 ? e1 : e0);
       b0[y] = n0 ? e0 : kNoBoundary;
@@ -444,23 +462,13 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
           const float rB = rn;
           const uint32_t tB = tn;
           if (j + 1u < end) candidate_load<kRadius>(c, j + 1u, pos, type, radius, pn, rn, tn);
-          if (j == b0[y] || j == b1[y]) {
-            fx += cfx;
-            fy += cfy;
-            cfx = 0.0f;
-            cfy = 0.0f;
-          }
-          if (j != i) candidate_at<kRadius>(c, q, j, pB, rB, tB, vel, omega, type, cfx, cfy, moment, pres);
+          cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
+          candidate_at<kRadius>(c, q, j, j == i, pB, rB, tB, vel, omega, type, cfx, cfy, moment, pres);
         }
       } else {
         for (uint32_t j = rs[y]; j < end; ++j) {
-          if (j == b0[y] || j == b1[y]) {
-            fx += cfx;
-            fy += cfy;
-            cfx = 0.0f;
-            cfy = 0.0f;
-          }
-          if (j != i) candidate<kRadius>(c, q, j, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
+          cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
+          candidate<kRadius>(c, q, j, j == i, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
         }
       }
       fx += cfx;
@@ -479,7 +487,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
         const uint32_t en = min(cell_end[row + xs[sg]], n);
         float cfx = 0.0f, cfy = 0.0f;
         for (uint32_t j = st; j < en; ++j)
-          if (j != i) candidate<kRadius>(c, q, j, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
+          candidate<kRadius>(c, q, j, j == i, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
         fx += cfx;
         fy += cfy;
       }

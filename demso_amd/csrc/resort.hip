@@ -41,6 +41,8 @@ constexpr int kSortThreads = 1024;
 constexpr int kSortWaves = kSortThreads / kWave;
 constexpr uint32_t kBitonicMax = 4096;
 constexpr int kPlaceThreads = 256;
+constexpr uint32_t kPlaceItems = 8;  // stayers per thread
+constexpr uint32_t kPlaceTile = kPlaceThreads * kPlaceItems;
 constexpr uint32_t kMoverBlocks = 32;
 constexpr uint32_t kWindowMax = 1024;
 constexpr int kDigitBits = 8;
@@ -315,38 +317,60 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     return;
   }
 
-  // stayers: one particle per lane, a tile of kPlaceThreads per block
-  const uint32_t first = (blockIdx.x - kMoverBlocks) * kPlaceThreads;
+  // stayers: a tile of kPlaceTile particles per block, kPlaceItems per
+  // thread at a stride of kPlaceThreads (coalesced).  The movers that sort
+  // among the tile's stayers are staged in LDS: all of them when there are
+  // at most kWindowMax (no search in global memory at all), else the window
+  // [lo, hi) found by two binary searches.
+  const uint32_t first = (blockIdx.x - kMoverBlocks) * kPlaceTile;
   if (first >= n) return;
-  const uint32_t last = min(first + kPlaceThreads, n) - 1u;
-  if (tid == 0) s_lo = m ? movers_below(sk, sp, 0, m, composite(prev_key[first], first)) : 0u;
-  if (tid == 1) s_hi = m ? movers_below(sk, sp, 0, m, composite(prev_key[last], last)) : 0u;
-  __syncthreads();
-  const uint32_t lo = s_lo, hi = s_hi;
+  const uint32_t last = min(first + kPlaceTile, n) - 1u;
+  uint32_t lo = 0, hi = m;
+  if (m > kWindowMax) {
+    if (tid == 0) s_lo = movers_below(sk, sp, 0, m, composite(prev_key[first], first));
+    if (tid == 1) s_hi = movers_below(sk, sp, 0, m, composite(prev_key[last], last));
+    __syncthreads();
+    lo = s_lo;
+    hi = s_hi;
+  }
   const bool staged = hi - lo <= kWindowMax;
   if (staged)
     for (uint32_t j = tid; j < hi - lo; j += kPlaceThreads) win[j] = composite(sk[lo + j], sp[lo + j]);
   __syncthreads();
-  const uint32_t i = first + tid;
-  if (i > last || ((bits[i >> 5] >> (i & 31u)) & 1u)) return;
-  const uint32_t k = key[i];
-  const uint64_t x = composite(k, i);
-  uint32_t below;
-  if (staged) {
-    uint32_t a = 0, b = hi - lo;
-    while (a < b) {
-      const uint32_t mid = (a + b) >> 1;
-      if (win[mid] < x) a = mid + 1;
-      else b = mid;
-    }
-    below = lo + a;
-  } else {
-    below = movers_below(sk, sp, lo, hi, x);
+  // all loads of the thread's items first, then the placements
+  uint32_t kk[kPlaceItems], bw[kPlaceItems], wp[kPlaceItems];
+#pragma unroll
+  for (uint32_t it = 0; it < kPlaceItems; ++it) {
+    const uint32_t i = first + it * kPlaceThreads + tid;
+    const bool valid = i <= last;
+    kk[it] = valid ? key[i] : 0u;
+    bw[it] = valid ? bits[i >> 5] : 0xFFFFFFFFu;
+    wp[it] = valid ? word_prefix[i >> 5] : 0u;
   }
-  const uint32_t slot = i - movers_before(i, n, m, bits, word_prefix) + below;
-  if (slot < n) {  // always true for a consistent input; guards the store
-    out_key[slot] = k;
-    out_idx[slot] = i;
+#pragma unroll
+  for (uint32_t it = 0; it < kPlaceItems; ++it) {
+    const uint32_t i = first + it * kPlaceThreads + tid;
+    const uint32_t b = i & 31u;
+    if (i > last || ((bw[it] >> b) & 1u)) continue;  // past the end, or a mover
+    const uint64_t x = composite(kk[it], i);
+    uint32_t below;
+    if (staged) {
+      uint32_t a = 0, e = hi - lo;
+      while (a < e) {
+        const uint32_t mid = (a + e) >> 1;
+        if (win[mid] < x) a = mid + 1;
+        else e = mid;
+      }
+      below = lo + a;
+    } else {
+      below = movers_below(sk, sp, lo, hi, x);
+    }
+    // i - movers_before(i) + below
+    const uint32_t slot = i - (wp[it] + (uint32_t)__popc(bw[it] & ((1u << b) - 1u))) + below;
+    if (slot < n) {  // always true for a consistent input; guards the store
+      out_key[slot] = kk[it];
+      out_idx[slot] = i;
+    }
   }
 }
 
@@ -364,7 +388,7 @@ hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* pre
                      r.word_prefix, r.mkey[0], r.mpos[0], n);
   hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.mkey[0], r.mpos[0],
                      r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count);
-  const uint32_t place_tiles = (n + kPlaceThreads - 1u) / kPlaceThreads;
+  const uint32_t place_tiles = (n + kPlaceTile - 1u) / kPlaceTile;
   hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + place_tiles), dim3(kPlaceThreads), 0, s, key, prev_key, r.bits,
                      r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, prev_cell_start,
                      prev_cell_end, num_cells, cells_valid ? 1 : 0, out_key, out_idx, n);
