@@ -538,8 +538,8 @@ hipError_t copy_state(dem_sim* m, AdjCheckpoint& c, bool to_checkpoint, hipStrea
   const size_t n = m->n;
   const int b = to_checkpoint ? m->cur : c.cur;
   struct Item { void* ck; void* st; size_t bytes; } items[] = {
-      {c.pos, m->pos[kHome], 8 * n},   {c.vel, m->vel[kHome], 8 * n}, {c.theta, m->theta[kHome], 4 * n},
-      {c.omega, m->omega[kHome], 4 * n}, {c.id, m->id[b], 4 * n},     {c.type, m->type[b], 4 * n},
+      {c.pos, m->pos[b], 8 * n},   {c.vel, m->vel[b], 8 * n}, {c.theta, m->theta[b], 4 * n},
+      {c.omega, m->omega[b], 4 * n}, {c.id, m->id[b], 4 * n},     {c.type, m->type[b], 4 * n},
       {c.loc, m->loc[b], 4 * n},   {c.radius, m->rad ? m->radius[b] : nullptr, 4 * n}};
   for (const Item& it : items) {
     if (!it.ck || !it.st) continue;
@@ -556,7 +556,7 @@ hipError_t store_slot_hook(void* user, dem_sim* m, hipStream_t s, int t) {
   auto* a = static_cast<dem_adjoint*>(user);
   const size_t off = a->slot_off;
   const uint32_t n = m->n;
-  hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[kSorted], m->vel[kSorted], m->omega[kSorted],
+  hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[kScratch], m->vel[kScratch], m->omega[kScratch],
                      m->rad ? m->radius[t] : nullptr, m->type[t], m->sorted_idx, m->sorted_key, m->acc, m->alpha,
                      a->spos + off, a->svel + off, a->somega + off, a->sradius ? a->sradius + off : nullptr,
                      a->stype + off, a->sindex + off, a->skey + off, a->sacc + off, a->salpha + off, n);

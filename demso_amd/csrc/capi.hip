@@ -483,12 +483,17 @@ void sim_free(dem_sim* m) {
 hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, const StepHook* hook) {
   dem_context* ctx = m->ctx;
   const StepConst& c = ctx->c;
-  // static fields (id, type, loc, radius) ping-pong between buffers o and t;
-  // the dynamic fields (pos, vel, theta, omega) live in buffer kHome between
-  // steps and are sorted into kSorted, from which the fused collide +
-  // integrate writes them back
+  // The state is in buffer o and the step writes buffer t.  Two layouts:
+  //  * gather (the default): reorder sorts only the positions (into t) and
+  //    the per-particle constants; collide and integrate read velocity,
+  //    spin and angle from buffer o through the sorted index, and integrate
+  //    writes the new state into t (positions in place);
+  //  * full (fused collide + integrate, or a hook that records the sorted
+  //    pre-integrate state): reorder sorts every field into kScratch, from
+  //    which collide + integrate write buffer t.
   const int o = m->cur, t = o ^ 1;
-  const int H = kHome, S = kSorted;
+  const bool gather = !ctx->fuse && !(hook && hook->fn);
+  const int S = gather ? t : kScratch;
   const uint32_t n = m->n;
   const uint32_t* type_for_hash = multi(ctx) ? m->type[o] : nullptr;
   hipError_t e;
@@ -501,7 +506,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
                            m->skey[t], m->sidx, m->resort, n);
     if (e != hipSuccess) return e;
   } else {
-    e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[H], type_for_hash, m->key, m->idx, n);
+    e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], type_for_hash, m->key, m->idx, n);
     if (e != hipSuccess) return e;
     // this step's collide + integrate accumulates the next step's mover counts
     const uint32_t tiles = (n + dem::kResortTile - 1u) / dem::kResortTile;
@@ -523,10 +528,11 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   e = dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[3], s);
-  e = dem::launch_reorder(s, ctx->bs_reorder, m->cell_start, m->cell_end, m->pos[S], m->vel[S], m->theta[S],
-                          m->omega[S], m->id[t], m->loc[t], m->type[t], m->rad ? m->radius[t] : nullptr,
-                          m->sorted_key, m->sorted_idx, m->pos[H], m->vel[H], m->theta[H], m->omega[H], m->id[o],
-                          m->type[o], m->rad ? m->radius[o] : nullptr, n, m->cells);
+  e = dem::launch_reorder(s, ctx->bs_reorder, m->cell_start, m->cell_end, m->pos[S], gather ? nullptr : m->vel[S],
+                          gather ? nullptr : m->theta[S], gather ? nullptr : m->omega[S], m->id[t], m->loc[t],
+                          m->type[t], m->rad ? m->radius[t] : nullptr, m->sorted_key, m->sorted_idx, m->pos[o],
+                          m->vel[o], m->theta[o], m->omega[o], m->id[o], m->type[o],
+                          m->rad ? m->radius[o] : nullptr, n, m->cells, !gather);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[4], s);
   dem::NextKeys nk;
@@ -545,10 +551,10 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
     co.react_y = m->react_y;
     co.react_m = m->react_m;
   }
-  if (ctx->fuse) {
+  if (!gather && ctx->fuse) {
     e = dem::launch_collide_integrate(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->omega[S], m->theta[S],
                                       m->type[t], S_radius, m->cell_start, m->cell_end, m->acc, m->alpha, m->pressure,
-                                      m->pos[H], m->vel[H], m->omega[H], m->theta[H], nk, n, ctx->prefetch,
+                                      m->pos[t], m->vel[t], m->omega[t], m->theta[t], nk, n, ctx->prefetch,
                                       control ? &co : nullptr);
     if (e != hipSuccess) return e;
     if (control) {
@@ -558,9 +564,12 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
     }
     if (ev) (void)hipEventRecord(ev[5], s);
   } else {
-    e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->acc, m->omega[S], m->alpha, m->type[t],
+    // with gather, velocity and spin come from buffer o through the index
+    const int D = gather ? o : S;
+    const uint32_t* gat = gather ? m->sorted_idx : nullptr;
+    e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[S], m->vel[D], m->acc, m->omega[D], m->alpha, m->type[t],
                             S_radius, m->cell_start, m->cell_end, m->pressure, n, ctx->prefetch,
-                            control ? &co : nullptr);
+                            control ? &co : nullptr, gat);
     if (e != hipSuccess) return e;
     if (control) {
       e = dem::launch_control_finish(s, co, m->react_bx, m->react_by, m->react_bm, n, ctx->d_reaction, ctx->d_control,
@@ -568,14 +577,15 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
       if (e != hipSuccess) return e;
     }
     if (ev) (void)hipEventRecord(ev[5], s);
-    // integrate from the sorted buffer back to the home buffer
-    e = dem::launch_integrate_to(c, s, ctx->bs_integrate, m->pos[S], m->vel[S], m->theta[S], m->omega[S], m->pos[H],
-                                 m->vel[H], m->theta[H], m->omega[H], m->acc, m->alpha, m->type[t], S_radius,
-                                 m->pressure, n, &nk);
+    // integrate into buffer t (with gather the positions in place, and
+    // velocity, angle and spin through the index from buffer o)
+    e = dem::launch_integrate_to(c, s, ctx->bs_integrate, m->pos[S], m->vel[D], m->theta[D], m->omega[D], m->pos[t],
+                                 m->vel[t], m->theta[t], m->omega[t], m->acc, m->alpha, m->type[t], S_radius,
+                                 m->pressure, n, &nk, gat);
     if (e != hipSuccess) return e;
   }
   if (hook && hook->fn) {
-    // the sorted pre-integrate state is still in kSorted / t
+    // the full layout: the sorted pre-integrate state is still in kScratch / t
     e = hook->fn(hook->user, m, s, t);
     if (e != hipSuccess) return e;
   }
@@ -666,6 +676,10 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
     if (!r) r = sim_alloc(m, &m->loc[b], n);
     if (!r && m->rad) r = sim_alloc(m, &m->radius[b], n);
   }
+  if (!r) r = sim_alloc(m, &m->pos[kScratch], n);
+  if (!r) r = sim_alloc(m, &m->vel[kScratch], n);
+  if (!r) r = sim_alloc(m, &m->theta[kScratch], n);
+  if (!r) r = sim_alloc(m, &m->omega[kScratch], n);
   if (!r) r = sim_alloc(m, &m->acc, n);
   if (!r) r = sim_alloc(m, &m->alpha, n);
   if (!r) r = sim_alloc(m, &m->pressure, n);
@@ -740,10 +754,10 @@ int dem_sim_upload(dem_sim* m, const float* pos, const float* vel, const float* 
   const int c = m->cur;
   const size_t n = m->n;
   hipStream_t s = ctx->stream;
-  DEM_HIP(ctx, hipMemcpyAsync(m->pos[kHome], pos, 8 * n, hipMemcpyDefault, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->vel[kHome], vel, 8 * n, hipMemcpyDefault, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->theta[kHome], theta, 4 * n, hipMemcpyDefault, s), "upload");
-  DEM_HIP(ctx, hipMemcpyAsync(m->omega[kHome], omega, 4 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->pos[c], pos, 8 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->vel[c], vel, 8 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->theta[c], theta, 4 * n, hipMemcpyDefault, s), "upload");
+  DEM_HIP(ctx, hipMemcpyAsync(m->omega[c], omega, 4 * n, hipMemcpyDefault, s), "upload");
   DEM_HIP(ctx, hipMemcpyAsync(m->id[c], id, 4 * n, hipMemcpyDefault, s), "upload");
   DEM_HIP(ctx, hipMemcpyAsync(m->type[c], type, 4 * n, hipMemcpyDefault, s), "upload");
   DEM_HIP(ctx, hipMemcpyAsync(m->loc[c], loc, 4 * n, hipMemcpyDefault, s), "upload");
@@ -764,8 +778,8 @@ int dem_sim_download(dem_sim* m, float* pos, float* vel, float* theta, float* om
   const size_t n = m->n;
   hipStream_t s = ctx->stream;
   struct Item { void* dst; const void* src; size_t bytes; } items[] = {
-      {pos, m->pos[kHome], 8 * n},      {vel, m->vel[kHome], 8 * n},       {theta, m->theta[kHome], 4 * n},
-      {omega, m->omega[kHome], 4 * n},  {id, m->id[c], 4 * n},             {type, m->type[c], 4 * n},
+      {pos, m->pos[c], 8 * n},      {vel, m->vel[c], 8 * n},       {theta, m->theta[c], 4 * n},
+      {omega, m->omega[c], 4 * n},  {id, m->id[c], 4 * n},             {type, m->type[c], 4 * n},
       {loc, m->loc[c], 4 * n},          {radius, m->rad ? m->radius[c] : nullptr, 4 * n},
       {acc, m->acc, 8 * n},             {alpha, m->alpha, 4 * n},          {pressure, m->pressure, 4 * n},
       {hash, m->sorted_key, 4 * n},     {index, m->sorted_idx, 4 * n},
@@ -898,8 +912,8 @@ int dem_sim_get_by_id(dem_sim* m, float* pos_by_id, float* vel_by_id) {
   dem_context* ctx = m->ctx;
   if (int r = ensure_device(ctx)) return r;
   const int c = m->cur;
-  hipLaunchKernelGGL(k_get_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c], m->pos[kHome],
-                     m->vel[kHome],
+  hipLaunchKernelGGL(k_get_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c], m->pos[c],
+                     m->vel[c],
                      (float2*)pos_by_id, (float2*)vel_by_id, m->n);
   DEM_HIP(ctx, hipGetLastError(), "get by id");
   return DEM_OK;
@@ -911,7 +925,7 @@ int dem_sim_set_by_id(dem_sim* m, const float* pos_by_id, const float* vel_by_id
   if (int r = ensure_device(ctx)) return r;
   const int c = m->cur;
   hipLaunchKernelGGL(k_set_by_id, dim3((m->n + 255) / 256), dim3(256), 0, ctx->stream, m->id[c],
-                     (const float2*)pos_by_id, (const float2*)vel_by_id, m->pos[kHome], m->vel[kHome], m->n);
+                     (const float2*)pos_by_id, (const float2*)vel_by_id, m->pos[m->cur], m->vel[m->cur], m->n);
   DEM_HIP(ctx, hipGetLastError(), "set by id");
   // the order still follows skey[cur], but new positions may move many
   // particles to other cells: the next step sorts from scratch
@@ -973,8 +987,8 @@ int dem_gather_rows(dem_context* ctx, const uint32_t* follow_ids, uint32_t count
 int dem_sim_gather_rows(dem_sim* m, const uint32_t* follow_ids, uint32_t count, uint32_t* rows) {
   if (!m) return DEM_ERR_INVALID;
   const int c = m->cur;
-  return dem_gather_rows(m->ctx, follow_ids, count, m->loc[c], (const float*)m->pos[kHome],
-                         (const float*)m->vel[kHome], (const float*)m->acc, m->theta[kHome], m->omega[kHome], m->alpha,
+  return dem_gather_rows(m->ctx, follow_ids, count, m->loc[c], (const float*)m->pos[c],
+                         (const float*)m->vel[c], (const float*)m->acc, m->theta[c], m->omega[c], m->alpha,
                          m->id[c], m->type[c], m->pressure,
                          m->n, rows);
 }
@@ -1102,19 +1116,22 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   // Populate the scratch buffer t (the next step's sorted state) once; the
   // current state in buffer o is never written.
   bool in_alt = false;
-  DEM_HIP(ctx, dem::launch_hash(c, s, ctx->bs_hash, m->pos[kHome], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n), "tune");
+  DEM_HIP(ctx, dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n), "tune");
   DEM_HIP(ctx, dem::launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt), "tune");
   uint32_t* sk = in_alt ? m->sort.keys_alt : m->key;
   uint32_t* si = in_alt ? m->sort.vals_alt : m->idx;
   DEM_HIP(ctx, dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells), "tune");
-  auto reorder = [&](int bs) {
-    return dem::launch_reorder(s, bs, m->cell_start, m->cell_end, m->pos[kSorted], m->vel[kSorted],
-                               m->theta[kSorted], m->omega[kSorted], m->id[t], m->loc[t], m->type[t],
-                               m->rad ? m->radius[t] : nullptr, sk, si, m->pos[kHome], m->vel[kHome],
-                               m->theta[kHome], m->omega[kHome], m->id[o], m->type[o],
-                               m->rad ? m->radius[o] : nullptr, n, m->cells);
+  // the fully sorted copy into kScratch (the fused layout); the gather
+  // layout's reorder writes the sorted positions there too
+  const int K = kScratch;
+  auto reorder_as = [&](int bs, bool full) {
+    return dem::launch_reorder(s, bs, m->cell_start, m->cell_end, m->pos[K], full ? m->vel[K] : nullptr,
+                               full ? m->theta[K] : nullptr, full ? m->omega[K] : nullptr, m->id[t], m->loc[t],
+                               m->type[t], m->rad ? m->radius[t] : nullptr, sk, si, m->pos[o], m->vel[o],
+                               m->theta[o], m->omega[o], m->id[o], m->type[o], m->rad ? m->radius[o] : nullptr, n,
+                               m->cells, full);
   };
-  DEM_HIP(ctx, reorder(ctx->bs_reorder), "tune");
+  DEM_HIP(ctx, reorder_as(ctx->bs_reorder, true), "tune");
   hipEvent_t e0, e1;
   DEM_HIP(ctx, hipEventCreate(&e0), "hipEventCreate");
   DEM_HIP(ctx, hipEventCreate(&e1), "hipEventCreate");
@@ -1141,19 +1158,21 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   nk.key = m->key == sk ? m->idx : m->key;  // any n-word buffer other than sk
   nk.bits = m->resort.bits;
   nk.tile_count = m->resort.tile_count;
-  const float2* S_pos = m->pos[kSorted];
-  const float2* S_vel = m->vel[kSorted];
-  const float* S_theta = m->theta[kSorted];
-  const float* S_omega = m->omega[kSorted];
+  const float2* S_pos = m->pos[K];
+  const float2* S_vel = m->vel[K];
+  const float* S_theta = m->theta[K];
+  const float* S_omega = m->omega[K];
   const float* S_radius = m->rad ? m->radius[t] : nullptr;
   bool pf = false;  // prefetch variant under test
+  // the split layout as the step runs it: velocity, spin and angle from the
+  // live state (buffer o) through the sorted index
   auto collide = [&](int b) {
-    return dem::launch_collide(c, s, b, S_pos, S_vel, m->acc, S_omega, m->alpha, m->type[t], S_radius, m->cell_start,
-                               m->cell_end, m->pressure, n, pf);
+    return dem::launch_collide(c, s, b, S_pos, m->vel[o], m->acc, m->omega[o], m->alpha, m->type[t], S_radius,
+                               m->cell_start, m->cell_end, m->pressure, n, pf, nullptr, si);
   };
   auto integrate = [&](int b) {
-    return dem::launch_integrate_to(c, s, b, S_pos, S_vel, S_theta, S_omega, tpos, tvel, ttheta, tomega, m->acc,
-                                    m->alpha, m->type[t], S_radius, m->pressure, n, &nk);
+    return dem::launch_integrate_to(c, s, b, S_pos, m->vel[o], m->theta[o], m->omega[o], tpos, tvel, ttheta, tomega,
+                                    m->acc, m->alpha, m->type[t], S_radius, m->pressure, n, &nk, si);
   };
   auto fused = [&](int b) {
     return dem::launch_collide_integrate(c, s, b, S_pos, S_vel, S_omega, S_theta, m->type[t], S_radius,
@@ -1162,16 +1181,20 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   };
   int best[4] = {ctx->bs_hash, ctx->bs_reorder, ctx->bs_collide, ctx->bs_integrate};
   float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  float t_full = 1e30f;  // the fused layout's reorder (every field) at the best block size
   // collide and fused block sizes for each prefetch choice
   int best_c[2] = {ctx->bs_collide, ctx->bs_collide}, best_f[2] = {ctx->bs_collide, ctx->bs_collide};
   float t_c[2] = {1e30f, 1e30f}, t_f[2] = {1e30f, 1e30f};
   for (int bs : cand) {
     if (e != hipSuccess) break;
     float tm = 0;
-    e = time_it([&](int b) { return dem::launch_hash(c, s, b, m->pos[kHome], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n); }, bs, &tm);
+    e = time_it([&](int b) { return dem::launch_hash(c, s, b, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n); }, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[0]) { best_t[0] = tm; best[0] = bs; }
-    e = time_it([&](int b) { return reorder(b); }, bs, &tm);
+    e = time_it([&](int b) { return reorder_as(b, true); }, bs, &tm);
+    if (e != hipSuccess) break;
+    t_full = std::min(t_full, tm);
+    e = time_it([&](int b) { return reorder_as(b, false); }, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[1]) { best_t[1] = tm; best[1] = bs; }
     e = time_it(integrate, bs, &tm);
@@ -1197,11 +1220,12 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
   // collide + integrate as one kernel or two, with or without the candidate
   // prefetch: whichever is fastest here
+  // (the split layout's reorder moves fewer bytes than the fused layout's)
   int choice = 0;  // 0/1 split without/with prefetch, 2/3 fused without/with
-  float tbest = t_c[0] + best_t[3];
-  if (t_c[1] + best_t[3] < tbest) { tbest = t_c[1] + best_t[3]; choice = 1; }
-  if (t_f[0] < tbest) { tbest = t_f[0]; choice = 2; }
-  if (t_f[1] < tbest) { tbest = t_f[1]; choice = 3; }
+  float tbest = best_t[1] + t_c[0] + best_t[3];
+  if (best_t[1] + t_c[1] + best_t[3] < tbest) { tbest = best_t[1] + t_c[1] + best_t[3]; choice = 1; }
+  if (t_full + t_f[0] < tbest) { tbest = t_full + t_f[0]; choice = 2; }
+  if (t_full + t_f[1] < tbest) { tbest = t_full + t_f[1]; choice = 3; }
   const bool fuse = choice >= 2, prefetch = (choice & 1) != 0;
   ctx->bs_hash = best[0];
   ctx->bs_reorder = best[1];

@@ -50,17 +50,19 @@ struct dem_context {
   size_t scalar_cap = 0;
 };
 
-// Buffer of the owned simulation's dynamic fields (pos, vel, theta, omega)
-// between steps, and the buffer they are sorted into during a step.
-constexpr int kHome = 0;
-constexpr int kSorted = 1;
+// Every field of the owned simulation's state lives in buffer `cur` (0 or 1)
+// between steps; a step writes the next state into buffer cur ^ 1.  The
+// dynamic fields (pos, vel, theta, omega) have a third buffer, kScratch, for
+// the fully sorted copy that the fused collide + integrate and the adjoint's
+// recorded steps read.
+constexpr int kScratch = 2;
 
 struct dem_sim {
   dem_context* ctx = nullptr;
   uint32_t n = 0, cells = 0;
   bool rad = false;
-  float2 *pos[2] = {}, *vel[2] = {};
-  float *theta[2] = {}, *omega[2] = {}, *radius[2] = {};
+  float2 *pos[3] = {}, *vel[3] = {};
+  float *theta[3] = {}, *omega[3] = {}, *radius[2] = {};
   uint32_t *id[2] = {}, *type[2] = {}, *loc[2] = {};
   float2* acc = nullptr;
   float *alpha = nullptr, *pressure = nullptr;

@@ -89,12 +89,12 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
                           const uint32_t* hash, const uint32_t* index, const float2* o_pos,
                           const float2* o_vel, const float* o_theta, const float* o_omega,
                           const uint32_t* o_id, const uint32_t* o_type, const float* o_radius,
-                          uint32_t n, uint32_t num_cells);
+                          uint32_t n, uint32_t num_cells, bool full = true);
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
                           const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch = false,
-                          const struct ControlOut* ctl = nullptr);
+                          const struct ControlOut* ctl = nullptr, const uint32_t* gat = nullptr);
 // The next step's hash, fused into integrate: key[i] = cell key of the new
 // position, bit i of `bits` set when it differs from prev_key[i] (the keys
 // the state is sorted by), tile_count[i / kResortTile] += movers (the
@@ -114,7 +114,7 @@ hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, con
                                const float2* vel_in, const float* theta_in, const float* omega_in, float2* pos_out,
                                float2* vel_out, float* theta_out, float* omega_out, const float2* acc,
                                const float* alpha, const uint32_t* type, const float* radius, float* pressure,
-                               uint32_t n, const NextKeys* next = nullptr);
+                               uint32_t n, const NextKeys* next = nullptr, const uint32_t* gat = nullptr);
 // collide + integrate + the next step's hash in one kernel: forces from the
 // sorted state *_in, acc/alpha/pressure as collide writes them, the
 // integrated state to *_out (a different buffer).

@@ -120,8 +120,11 @@ __global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint32_t
 // reorderDataAndFindCellStartD (src/particles_kernel.cuh:156-233).  The
 // previous key is read from global memory (it is in the same cache line for
 // all lanes but lane 0).  Keys >= num_cells, indices >= n and IDs >= n are
-// skipped instead of being written out of bounds.
-template <bool kRadius>
+// skipped instead of being written out of bounds.  kFull = false gathers the
+// positions and the per-particle constants only: the owned simulation's step
+// reads velocity, angle and spin through the sorted index instead (collide
+// for its own particle and its contacts, integrate for every particle).
+template <bool kRadius, bool kFull>
 __global__ __launch_bounds__(256) void k_reorder(
     uint32_t* __restrict__ cell_start, uint32_t* __restrict__ cell_end, float2* __restrict__ s_pos,
     float2* __restrict__ s_vel, float* __restrict__ s_theta, float* __restrict__ s_omega,
@@ -149,9 +152,11 @@ __global__ __launch_bounds__(256) void k_reorder(
   const uint32_t k = index[i];
   if (k >= n) return;
   s_pos[i] = o_pos[k];
-  s_vel[i] = o_vel[k];
-  s_theta[i] = o_theta[k];
-  s_omega[i] = o_omega[k];
+  if (kFull) {
+    s_vel[i] = o_vel[k];
+    s_theta[i] = o_theta[k];
+    s_omega[i] = o_omega[k];
+  }
   const uint32_t id = o_id[k];
   s_id[i] = id;
   if (id < n) s_loc[id] = i;
@@ -257,10 +262,19 @@ __device__ __forceinline__ void attract_force(const PairCtx& q, float rx, float 
   pres = add_pressure(pres, sqrtf(afx * afx + afy * afy), q.den);
 }
 
+// Velocity and spin of the sorted particles: vel[j], omega[j], or with a
+// gather index vel[gat[j]], omega[gat[j]] (the previous step's arrays,
+// before the sort).
+struct Dyn {
+  const float2* __restrict__ vel;
+  const float* __restrict__ omega;
+  const uint32_t* __restrict__ gat;
+  __device__ __forceinline__ uint32_t at(uint32_t j) const { return gat ? gat[j] : j; }
+};
+
 template <bool kRadius>
 __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, bool self, float2 pB,
-                                             float rB, uint32_t tB, const float2* __restrict__ vel,
-                                             const float* __restrict__ omega, const uint32_t* __restrict__ type,
+                                             float rB, uint32_t tB, const Dyn& d, const uint32_t* __restrict__ type,
                                              float& cfx, float& cfy, float& moment, float& pres) {
   const float rx = pB.x - q.p.x, ry = pB.y - q.p.y;
   const float d2 = rx * rx + ry * ry;
@@ -272,7 +286,8 @@ __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& 
   if (kRadius) tB = clamp_idx(type[j], c.num_types);
   if (dist < R) {
     float ffx, ffy;
-    contact_force(c, q, rx, ry, dist, R, rB, tB, vel[j], omega[j], ffx, ffy, moment, pres);
+    const uint32_t jv = d.at(j);
+    contact_force(c, q, rx, ry, dist, R, rB, tB, d.vel[jv], d.omega[jv], ffx, ffy, moment, pres);
     cfx += ffx;
     cfy += ffy;
   }
@@ -334,15 +349,14 @@ __device__ __forceinline__ void candidate_load(const StepConst& c, uint32_t j, c
 
 template <bool kRadius>
 __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
-                                          const float2* __restrict__ pos, const float2* __restrict__ vel,
-                                          const float* __restrict__ omega, const uint32_t* __restrict__ type,
-                                          const float* __restrict__ radius, float& cfx, float& cfy,
-                                          float& moment, float& pres) {
+                                          const float2* __restrict__ pos, const Dyn& d,
+                                          const uint32_t* __restrict__ type, const float* __restrict__ radius,
+                                          float& cfx, float& cfy, float& moment, float& pres) {
   float2 pB;
   float rB;
   uint32_t tB;
   candidate_load<kRadius>(c, j, pos, type, radius, pB, rB, tB);
-  candidate_at<kRadius>(c, q, j, self, pB, rB, tB, vel, omega, type, cfx, cfy, moment, pres);
+  candidate_at<kRadius>(c, q, j, self, pB, rB, tB, d, type, cfx, cfy, moment, pres);
 }
 
 // At the first candidate of a new cell, the previous cell's sum joins the
@@ -371,8 +385,8 @@ struct Own {
 
 template <bool kRadius, bool kMulti, bool kPrefetch, bool kControl>
 __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i, const float2* __restrict__ pos,
-                                                 const float2* __restrict__ vel, const float* __restrict__ omega,
-                                                 const uint32_t* __restrict__ type, const float* __restrict__ radius,
+                                                 const Dyn& d, const uint32_t* __restrict__ type,
+                                                 const float* __restrict__ radius,
                                                  const uint32_t* __restrict__ cell_start,
                                                  const uint32_t* __restrict__ cell_end, uint32_t n, Own& own,
                                                  const ControlOut& ctl, float& fx_out, float& fy_out,
@@ -386,8 +400,11 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   body<kRadius>(q.mA, radius, i, q.rA, own.mass, own.inertia);
   own.r = q.rA;
   q.p = pos[i];
-  q.v = vel[i];
-  q.w = omega[i];
+  {
+    const uint32_t iv = d.at(i);
+    q.v = d.vel[iv];
+    q.w = d.omega[iv];
+  }
   own.p = q.p;
   own.v = q.v;
   own.w = q.w;
@@ -463,12 +480,12 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
           const uint32_t tB = tn;
           if (j + 1u < end) candidate_load<kRadius>(c, j + 1u, pos, type, radius, pn, rn, tn);
           cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
-          candidate_at<kRadius>(c, q, j, j == i, pB, rB, tB, vel, omega, type, cfx, cfy, moment, pres);
+          candidate_at<kRadius>(c, q, j, j == i, pB, rB, tB, d, type, cfx, cfy, moment, pres);
         }
       } else {
         for (uint32_t j = rs[y]; j < end; ++j) {
           cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
-          candidate<kRadius>(c, q, j, j == i, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
+          candidate<kRadius>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
         }
       }
       fx += cfx;
@@ -487,7 +504,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
         const uint32_t en = min(cell_end[row + xs[sg]], n);
         float cfx = 0.0f, cfy = 0.0f;
         for (uint32_t j = st; j < en; ++j)
-          candidate<kRadius>(c, q, j, j == i, pos, vel, omega, type, radius, cfx, cfy, moment, pres);
+          candidate<kRadius>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
         fx += cfx;
         fy += cfy;
       }
@@ -512,13 +529,14 @@ __global__ __launch_bounds__(256) void k_collide(
     float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha,
     const uint32_t* __restrict__ type, const float* __restrict__ radius,
     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
-    float* __restrict__ pressure, ControlOut ctl, uint32_t n) {
+    float* __restrict__ pressure, ControlOut ctl, const uint32_t* __restrict__ gat, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
-  collide_particle<kRadius, kMulti, kPrefetch, kControl>(c, i, pos, vel, omega, type, radius, cell_start, cell_end, n,
-                                                         own, ctl, fx, fy, moment, pres);
+  const Dyn d{vel, omega, gat};
+  collide_particle<kRadius, kMulti, kPrefetch, kControl>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
+                                                         ctl, fx, fy, moment, pres);
   acc[i] = make_float2(fx / own.mass, fy / own.mass);
   alpha[i] = moment / own.inertia;
   pressure[i] = pres;
@@ -645,7 +663,7 @@ __global__ __launch_bounds__(256) void k_integrate(
     StepConst c, const float2* pos_in, const float2* vel_in, const float* theta_in, const float* omega_in,
     float2* pos_out, float2* vel_out, float* theta_out, float* omega_out, const float2* __restrict__ acc,
     const float* __restrict__ alpha, const uint32_t* __restrict__ type, const float* __restrict__ radius,
-    float* __restrict__ pressure, NextKeys nk, uint32_t n) {
+    float* __restrict__ pressure, NextKeys nk, const uint32_t* __restrict__ gat, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t t = clamp_idx(type[i], c.num_types);
@@ -653,8 +671,10 @@ __global__ __launch_bounds__(256) void k_integrate(
   const Instance in = kMulti ? c.inst[clamp_idx(m.inst, c.num_instances)] : c.inst0;
   float r, mass, inertia;
   body<kRadius>(m, radius, i, r, mass, inertia);
-  float2 p = pos_in[i], v = vel_in[i];
-  float w = omega_in[i], th = theta_in[i];
+  // velocity, spin and angle through the gather index when one is given
+  const uint32_t g = gat ? gat[i] : i;
+  float2 p = pos_in[i], v = vel_in[g];
+  float w = omega_in[g], th = theta_in[g];
   float pres = 0.0f;
   if (integrate_particle(c, m, in, r, mass, inertia, acc[i], alpha[i], p, v, w, th, pres, pressure + i))
     pressure[i] = pres;
@@ -684,8 +704,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
-  collide_particle<kRadius, kMulti, kPrefetch, kControl>(c, i, pos_in, vel_in, omega_in, type, radius, cell_start,
-                                                         cell_end, n, own, ctl, fx, fy, moment, pres);
+  const Dyn d{vel_in, omega_in, nullptr};
+  collide_particle<kRadius, kMulti, kPrefetch, kControl>(c, i, pos_in, d, type, radius, cell_start, cell_end, n, own,
+                                                         ctl, fx, fy, moment, pres);
   const float2 a = make_float2(fx / own.mass, fy / own.mass);
   const float al = moment / own.inertia;
   acc[i] = a;
@@ -729,16 +750,18 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
                           const uint32_t* hash, const uint32_t* index, const float2* o_pos,
                           const float2* o_vel, const float* o_theta, const float* o_omega,
                           const uint32_t* o_id, const uint32_t* o_type, const float* o_radius,
-                          uint32_t n, uint32_t num_cells) {
+                          uint32_t n, uint32_t num_cells, bool full) {
   if (n == 0) return hipSuccess;
-  if (s_radius)
-    hipLaunchKernelGGL(k_reorder<true>, grid_for(n, block), dim3(block), 0, s, cell_start, cell_end, s_pos,
-                       s_vel, s_theta, s_omega, s_id, s_loc, s_type, s_radius, hash, index, o_pos, o_vel,
-                       o_theta, o_omega, o_id, o_type, o_radius, n, num_cells);
-  else
-    hipLaunchKernelGGL(k_reorder<false>, grid_for(n, block), dim3(block), 0, s, cell_start, cell_end, s_pos,
-                       s_vel, s_theta, s_omega, s_id, s_loc, s_type, s_radius, hash, index, o_pos, o_vel,
-                       o_theta, o_omega, o_id, o_type, o_radius, n, num_cells);
+#define DEM_REORDER(R, F)                                                                                     \
+  hipLaunchKernelGGL((k_reorder<R, F>), grid_for(n, block), dim3(block), 0, s, cell_start, cell_end, s_pos,    \
+                     s_vel, s_theta, s_omega, s_id, s_loc, s_type, s_radius, hash, index, o_pos, o_vel, o_theta, \
+                     o_omega, o_id, o_type, o_radius, n, num_cells)
+  if (s_radius) {
+    if (full) DEM_REORDER(true, true); else DEM_REORDER(true, false);
+  } else {
+    if (full) DEM_REORDER(false, true); else DEM_REORDER(false, false);
+  }
+#undef DEM_REORDER
   return hipGetLastError();
 }
 
@@ -777,13 +800,13 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
                           const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch,
-                          const ControlOut* ctl) {
+                          const ControlOut* ctl, const uint32_t* gat) {
   if (n == 0) return hipSuccess;
   const bool multi = c.num_instances > 1;
   const ControlOut co = ctl ? *ctl : ControlOut{};
 #define DEM_COLLIDE(R, M, P, X)                                                                        \
   hipLaunchKernelGGL((k_collide<R, M, P, X>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
-                     omega, alpha, type, radius, cell_start, cell_end, pressure, co, n)
+                     omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, n)
   DEM_DISPATCH(DEM_COLLIDE);
 #undef DEM_COLLIDE
   return hipGetLastError();
@@ -876,14 +899,14 @@ hipError_t launch_integrate(const StepConst& c, hipStream_t s, int block, float2
                             const uint32_t* type, const float* radius, float* pressure, uint32_t n,
                             const NextKeys* next) {
   return launch_integrate_to(c, s, block, pos, vel, theta, omega, pos, vel, theta, omega, acc, alpha, type, radius,
-                             pressure, n, next);
+                             pressure, n, next, nullptr);
 }
 
 hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, const float2* pos_in,
                                const float2* vel_in, const float* theta_in, const float* omega_in, float2* pos_out,
                                float2* vel_out, float* theta_out, float* omega_out, const float2* acc,
                                const float* alpha, const uint32_t* type, const float* radius, float* pressure,
-                               uint32_t n, const NextKeys* next) {
+                               uint32_t n, const NextKeys* next, const uint32_t* gat) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
   const bool multi = c.num_instances > 1;
@@ -891,7 +914,7 @@ hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, con
 #define DEM_INTEGRATE(R, M, X)                                                                          \
   hipLaunchKernelGGL((k_integrate<R, M, X>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
                      theta_in, omega_in, pos_out, vel_out, theta_out, omega_out, acc, alpha, type, radius,  \
-                     pressure, nk, n)
+                     pressure, nk, gat, n)
 #define DEM_INTEGRATE_X(R, M)            \
   do {                                   \
     if (next) DEM_INTEGRATE(R, M, true); \

@@ -323,7 +323,7 @@ int dem_sim_plot(dem_sim* m, uint8_t* image, const dem_render_params* rp) {
   dem_context* ctx = m->ctx;
   if (int r = ensure_device(ctx)) return r;
   const dem_controlled* d_control = ctx->h_control.enabled ? ctx->d_control : nullptr;
-  return plot(ctx, image, (const float*)m->pos[kHome], m->theta[kHome], m->type[m->cur],
+  return plot(ctx, image, (const float*)m->pos[m->cur], m->theta[m->cur], m->type[m->cur],
               m->rad ? m->radius[m->cur] : nullptr, m->n, rp, d_control, m->pressure);
 }
 
