@@ -39,23 +39,30 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 #   sort      re-sort: R key 4;                W sorted key 4, index 4
 #             (a radix pass would be R key+index 8, W key+index 8)
 #   fill      W cellStart 4 per cell
-#   reorder   R key 4, index 4, state 36;      W state 36, loc 4
-#             (state = pos 8, vel 8, theta 4, omega 4, id 4, type 4, radius 4)
-#   collide   R pos 8, vel 8, omega 4, radius 4, type 4; W acc 8, alpha 4, pressure 4
-#             + R cellStart/cellEnd 8 per cell
-#   integrate (with the next step's hash, sorted buffer -> home buffer)
-#             R pos 8, vel 8, theta 4, omega 4, acc 8, alpha 4, type 4, radius 4,
-#             pressure 4, previous key 4; W pos 8, vel 8, theta 4, omega 4, key 4
-#   When the autotuner fuses collide and integrate into one kernel, that
-#   kernel's bytes are R pos 8, vel 8, omega 4, theta 4, radius 4, type 4,
-#   previous key 4; W acc 8, alpha 4, pressure 4, pos 8, vel 8, theta 4,
-#   omega 4, key 4 (80) + 8 per cell, reported under collide.
-BYTES_PER_PARTICLE = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 84, "collide": 44, "integrate": 80}
+#   reorder   R key 4, index 4, pos 8, id 4, type 4, radius 4;
+#             W pos 8, id 4, loc 4, type 4, radius 4
+#             (velocity, angle and spin stay where they are: collide and
+#             integrate read them through the sorted index)
+#   collide   R pos 8, radius 4, type 4, index 4, vel 8, omega 4;
+#             W acc 8, alpha 4, pressure 4 + R cellStart/cellEnd 8 per cell
+#   integrate (with the next step's hash, into the next state buffer)
+#             R pos 8, index 4, vel 8, theta 4, omega 4, acc 8, alpha 4, type 4,
+#             radius 4, pressure 4, previous key 4; W pos 8, vel 8, theta 4,
+#             omega 4, key 4
+#   When the autotuner fuses collide and integrate into one kernel, reorder
+#   sorts every field (R key 4, index 4, state 36; W state 36, loc 4: 84) and
+#   the fused kernel's bytes are R pos 8, vel 8, omega 4, theta 4, radius 4,
+#   type 4, previous key 4; W acc 8, alpha 4, pressure 4, pos 8, vel 8,
+#   theta 4, omega 4, key 4 (80) + 8 per cell, reported under collide.
+BYTES_PER_PARTICLE = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 52, "collide": 48, "integrate": 84}
 BYTES_PER_CELL = {"hash": 0, "sort": 0, "cell_fill": 4, "reorder": 0, "collide": 8, "integrate": 0}
 FUSED_COLLIDE_BYTES = 80
+FULL_REORDER_BYTES = 84
 
 
 def algorithmic_bytes(stage: str, n: int, cells: int, fused: bool = False) -> int:
+    if fused and stage == "reorder":
+        return FULL_REORDER_BYTES * n
     if fused and stage == "collide":
         return FUSED_COLLIDE_BYTES * n + BYTES_PER_CELL[stage] * cells
     if fused and stage == "integrate":
