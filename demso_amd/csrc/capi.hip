@@ -425,7 +425,7 @@ int dem_collide_ex(dem_context* ctx, const float* pos, const float* vel, float* 
   if (int r = ensure_device(ctx)) return r;
   DEM_HIP(ctx, dem::launch_collide(ctx->c, ctx->stream, ctx->bs_collide, (const float2*)pos, (const float2*)vel,
                                    (float2*)acc, omega, alpha, type, radius, cellStart, cellEnd, pressure,
-                                   numParticles, ctx->prefetch),
+                                   numParticles, ctx->cmode),
           "collide kernel");
   return DEM_OK;
 }
@@ -554,7 +554,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   if (!gather && ctx->fuse) {
     e = dem::launch_collide_integrate(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->omega[S], m->theta[S],
                                       m->type[t], S_radius, m->cell_start, m->cell_end, m->acc, m->alpha, m->pressure,
-                                      m->pos[t], m->vel[t], m->omega[t], m->theta[t], nk, n, ctx->prefetch,
+                                      m->pos[t], m->vel[t], m->omega[t], m->theta[t], nk, n, ctx->cmode,
                                       control ? &co : nullptr);
     if (e != hipSuccess) return e;
     if (control) {
@@ -568,7 +568,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
     const int D = gather ? o : S;
     const uint32_t* gat = gather ? m->sorted_idx : nullptr;
     e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[S], m->vel[D], m->acc, m->omega[D], m->alpha, m->type[t],
-                            S_radius, m->cell_start, m->cell_end, m->pressure, n, ctx->prefetch,
+                            S_radius, m->cell_start, m->cell_end, m->pressure, n, ctx->cmode,
                             control ? &co : nullptr, gat);
     if (e != hipSuccess) return e;
     if (control) {
@@ -831,9 +831,12 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
 
 int dem_set_step_variant(dem_context* ctx, int flags) {
   if (!ctx) return DEM_ERR_INVALID;
-  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH)) return fail(ctx, DEM_ERR_INVALID, "unknown step variant flags");
+  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH))
+    return fail(ctx, DEM_ERR_INVALID, "unknown step variant flags");
+  if ((flags & DEM_STEP_PREFETCH) && (flags & DEM_STEP_BATCH))
+    return fail(ctx, DEM_ERR_INVALID, "DEM_STEP_PREFETCH and DEM_STEP_BATCH exclude each other");
   ctx->fuse = (flags & DEM_STEP_FUSED) != 0;
-  ctx->prefetch = (flags & DEM_STEP_PREFETCH) != 0;
+  ctx->cmode = (flags & DEM_STEP_PREFETCH) ? 1 : ((flags & DEM_STEP_BATCH) ? 2 : 0);
   ctx->generation++;  // captured graphs follow the choice
   return DEM_OK;
 }
@@ -1163,7 +1166,7 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   const float* S_theta = m->theta[K];
   const float* S_omega = m->omega[K];
   const float* S_radius = m->rad ? m->radius[t] : nullptr;
-  bool pf = false;  // prefetch variant under test
+  int pf = 0;  // candidate mode under test: 0 plain, 1 prefetch, 2 batch
   // the split layout as the step runs it: velocity, spin and angle from the
   // live state (buffer o) through the sorted index
   auto collide = [&](int b) {
@@ -1182,9 +1185,10 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   int best[4] = {ctx->bs_hash, ctx->bs_reorder, ctx->bs_collide, ctx->bs_integrate};
   float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
   float t_full = 1e30f;  // the fused layout's reorder (every field) at the best block size
-  // collide and fused block sizes for each prefetch choice
-  int best_c[2] = {ctx->bs_collide, ctx->bs_collide}, best_f[2] = {ctx->bs_collide, ctx->bs_collide};
-  float t_c[2] = {1e30f, 1e30f}, t_f[2] = {1e30f, 1e30f};
+  // collide and fused block sizes for each candidate mode
+  int best_c[3] = {ctx->bs_collide, ctx->bs_collide, ctx->bs_collide};
+  int best_f[3] = {ctx->bs_collide, ctx->bs_collide, ctx->bs_collide};
+  float t_c[3] = {1e30f, 1e30f, 1e30f}, t_f[3] = {1e30f, 1e30f, 1e30f};
   for (int bs : cand) {
     if (e != hipSuccess) break;
     float tm = 0;
@@ -1200,8 +1204,8 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
     e = time_it(integrate, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[3]) { best_t[3] = tm; best[3] = bs; }
-    for (int p = 0; p < 2 && e == hipSuccess; ++p) {
-      pf = p == 1;
+    for (int p = 0; p < 3 && e == hipSuccess; ++p) {
+      pf = p;
       e = time_it(collide, bs, &tm);
       if (e != hipSuccess) break;
       if (tm < t_c[p]) { t_c[p] = tm; best_c[p] = bs; }
@@ -1218,28 +1222,29 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   if (tomega) (void)hipFree(tomega);
   m->resort_ok = false;  // the cell arrays and keys now hold the tuning step's data
   if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
-  // collide + integrate as one kernel or two, with or without the candidate
-  // prefetch: whichever is fastest here
-  // (the split layout's reorder moves fewer bytes than the fused layout's)
-  int choice = 0;  // 0/1 split without/with prefetch, 2/3 fused without/with
-  float tbest = best_t[1] + t_c[0] + best_t[3];
-  if (best_t[1] + t_c[1] + best_t[3] < tbest) { tbest = best_t[1] + t_c[1] + best_t[3]; choice = 1; }
-  if (t_full + t_f[0] < tbest) { tbest = t_full + t_f[0]; choice = 2; }
-  if (t_full + t_f[1] < tbest) { tbest = t_full + t_f[1]; choice = 3; }
-  const bool fuse = choice >= 2, prefetch = (choice & 1) != 0;
+  // collide + integrate as one kernel or two, in whichever candidate mode is
+  // fastest here (the split layout's reorder moves fewer bytes than the
+  // fused layout's)
+  bool fuse = false;
+  int mode = 0;
+  float tbest = 1e30f;
+  for (int p = 0; p < 3; ++p) {
+    if (best_t[1] + t_c[p] + best_t[3] < tbest) { tbest = best_t[1] + t_c[p] + best_t[3]; fuse = false; mode = p; }
+    if (t_full + t_f[p] < tbest) { tbest = t_full + t_f[p]; fuse = true; mode = p; }
+  }
   ctx->bs_hash = best[0];
   ctx->bs_reorder = best[1];
-  ctx->bs_collide = fuse ? best_f[prefetch] : best_c[prefetch];
+  ctx->bs_collide = fuse ? best_f[mode] : best_c[mode];
   ctx->bs_integrate = best[3];
   ctx->fuse = fuse;
-  ctx->prefetch = prefetch;
+  ctx->cmode = mode;
   ctx->generation++;
   if (block) {
     block[0] = (uint32_t)best[0];
     block[1] = (uint32_t)best[1];
     block[2] = (uint32_t)ctx->bs_collide;
     block[3] = (uint32_t)best[3];
-    block[4] = (fuse ? 1u : 0u) | (prefetch ? 2u : 0u);
+    block[4] = (fuse ? 1u : 0u) | (mode == 1 ? 2u : 0u) | (mode == 2 ? 4u : 0u);
   }
   return DEM_OK;
 }

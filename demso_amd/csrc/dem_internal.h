@@ -93,7 +93,7 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
-                          const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch = false,
+                          const uint32_t* cell_end, float* pressure, uint32_t n, int mode = 0,
                           const struct ControlOut* ctl = nullptr, const uint32_t* gat = nullptr);
 // The next step's hash, fused into integrate: key[i] = cell key of the new
 // position, bit i of `bits` set when it differs from prev_key[i] (the keys
@@ -123,7 +123,7 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
                                     const uint32_t* type, const float* radius, const uint32_t* cell_start,
                                     const uint32_t* cell_end, float2* acc, float* alpha, float* pressure,
                                     float2* pos_out, float2* vel_out, float* omega_out, float* theta_out,
-                                    const NextKeys& next, uint32_t n, bool prefetch = false,
+                                    const NextKeys& next, uint32_t n, int mode = 0,
                                     const struct ControlOut* ctl = nullptr);
 // The controlled particle's reaction: per-block tree sums of the three
 // per-particle reaction vectors into bx/by/bm (ceil(n/kReactBlock) each),

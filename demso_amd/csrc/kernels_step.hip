@@ -383,7 +383,15 @@ struct Own {
   float w;
 };
 
-template <bool kRadius, bool kMulti, bool kPrefetch, bool kControl>
+// Candidate modes of the collide kernels: kPlain walks each row's candidates
+// one load at a time; kPrefetch loads each one iteration ahead; kBatch loads
+// the first kBatchN candidates of all three rows at once before walking them
+// (more loads in flight, more registers).  The order of the sums is the same
+// in every mode.
+constexpr int kPlain = 0, kPrefetch = 1, kBatch = 2;
+constexpr int kBatchN = 3;
+
+template <bool kRadius, bool kMulti, int kMode, bool kControl>
 __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i, const float2* __restrict__ pos,
                                                  const Dyn& d, const uint32_t* __restrict__ type,
                                                  const float* __restrict__ radius,
@@ -462,11 +470,24 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       b0[y] = n0 ? e0 : kNoBoundary;
       b1[y] = n1 ? e1 : kNoBoundary;
     }
+    float2 pb[3][kBatchN];
+    float rb[3][kBatchN];
+    uint32_t tb[3][kBatchN];
+    if (kMode == kBatch) {
+      // index 0 stands in for a candidate past the row's end (never used)
+#pragma unroll
+      for (int y = 0; y < 3; ++y)
+#pragma unroll
+        for (int k = 0; k < kBatchN; ++k) {
+          const uint32_t j = rs[y] + (uint32_t)k;
+          candidate_load<kRadius>(c, j < re[y] ? j : 0u, pos, type, radius, pb[y][k], rb[y][k], tb[y][k]);
+        }
+    }
 #pragma unroll
     for (int y = 0; y < 3; ++y) {
       float cfx = 0.0f, cfy = 0.0f;
       const uint32_t end = re[y];
-      if (kPrefetch) {
+      if (kMode == kPrefetch) {
         // each candidate's position and radius fetched one iteration ahead
         // (pays off with many candidates per row, costs registers)
         uint32_t j = rs[y];
@@ -483,7 +504,17 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
           candidate_at<kRadius>(c, q, j, j == i, pB, rB, tB, d, type, cfx, cfy, moment, pres);
         }
       } else {
-        for (uint32_t j = rs[y]; j < end; ++j) {
+        uint32_t j = rs[y];
+        if (kMode == kBatch) {
+#pragma unroll
+          for (int k = 0; k < kBatchN; ++k, ++j)
+            if (j < end) {
+              cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
+              candidate_at<kRadius>(c, q, j, j == i, pb[y][k], rb[y][k], tb[y][k], d, type, cfx, cfy, moment,
+                                    pres);
+            }
+        }
+        for (; j < end; ++j) {
           cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
           candidate<kRadius>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
         }
@@ -523,7 +554,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   pres_out = pres;
 }
 
-template <bool kRadius, bool kMulti, bool kPrefetch, bool kControl>
+template <bool kRadius, bool kMulti, int kMode, bool kControl>
 __global__ __launch_bounds__(256) void k_collide(
     StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,
     float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha,
@@ -535,7 +566,7 @@ __global__ __launch_bounds__(256) void k_collide(
   Own own;
   float fx, fy, moment, pres;
   const Dyn d{vel, omega, gat};
-  collide_particle<kRadius, kMulti, kPrefetch, kControl>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
+  collide_particle<kRadius, kMulti, kMode, kControl>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
                                                          ctl, fx, fy, moment, pres);
   acc[i] = make_float2(fx / own.mass, fy / own.mass);
   alpha[i] = moment / own.inertia;
@@ -692,7 +723,7 @@ __global__ __launch_bounds__(256) void k_integrate(
 // *_out arrays (the unsorted buffer of this step, dead after reorder), so no
 // lane overwrites a neighbour's input.  The arithmetic is that of k_collide
 // followed by k_integrate.
-template <bool kRadius, bool kMulti, bool kPrefetch, bool kControl>
+template <bool kRadius, bool kMulti, int kMode, bool kControl>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_collide_integrate(
     StepConst c, const float2* __restrict__ pos_in, const float2* __restrict__ vel_in,
     const float* __restrict__ omega_in, const float* __restrict__ theta_in, const uint32_t* __restrict__ type,
@@ -705,7 +736,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   Own own;
   float fx, fy, moment, pres;
   const Dyn d{vel_in, omega_in, nullptr};
-  collide_particle<kRadius, kMulti, kPrefetch, kControl>(c, i, pos_in, d, type, radius, cell_start, cell_end, n, own,
+  collide_particle<kRadius, kMulti, kMode, kControl>(c, i, pos_in, d, type, radius, cell_start, cell_end, n, own,
                                                          ctl, fx, fy, moment, pres);
   const float2 a = make_float2(fx / own.mass, fy / own.mass);
   const float al = moment / own.inertia;
@@ -765,41 +796,30 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
   return hipGetLastError();
 }
 
-// Kernel variant dispatch: kRadius x kMulti x (prefetch, control), where
-// the controlled particle always uses the variant without prefetch.
+// Kernel variant dispatch: kRadius x kMulti x (candidate mode, control),
+// where the controlled particle always uses the plain mode.
+#define DEM_DISPATCH_RM(MACRO, MODE, X)                                  \
+  do {                                                                   \
+    if (radius) {                                                        \
+      if (multi) MACRO(true, true, MODE, X);                             \
+      else MACRO(true, false, MODE, X);                                  \
+    } else {                                                             \
+      if (multi) MACRO(false, true, MODE, X);                            \
+      else MACRO(false, false, MODE, X);                                 \
+    }                                                                    \
+  } while (0)
 #define DEM_DISPATCH(MACRO)                                              \
   do {                                                                   \
-    if (ctl) {                                                           \
-      if (radius) {                                                      \
-        if (multi) MACRO(true, true, false, true);                       \
-        else MACRO(true, false, false, true);                            \
-      } else {                                                           \
-        if (multi) MACRO(false, true, false, true);                      \
-        else MACRO(false, false, false, true);                           \
-      }                                                                  \
-    } else if (prefetch) {                                               \
-      if (radius) {                                                      \
-        if (multi) MACRO(true, true, true, false);                       \
-        else MACRO(true, false, true, false);                            \
-      } else {                                                           \
-        if (multi) MACRO(false, true, true, false);                      \
-        else MACRO(false, false, true, false);                           \
-      }                                                                  \
-    } else {                                                             \
-      if (radius) {                                                      \
-        if (multi) MACRO(true, true, false, false);                      \
-        else MACRO(true, false, false, false);                           \
-      } else {                                                           \
-        if (multi) MACRO(false, true, false, false);                     \
-        else MACRO(false, false, false, false);                          \
-      }                                                                  \
-    }                                                                    \
+    if (ctl) DEM_DISPATCH_RM(MACRO, kPlain, true);                       \
+    else if (mode == kPrefetch) DEM_DISPATCH_RM(MACRO, kPrefetch, false);\
+    else if (mode == kBatch) DEM_DISPATCH_RM(MACRO, kBatch, false);      \
+    else DEM_DISPATCH_RM(MACRO, kPlain, false);                          \
   } while (0)
 
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
-                          const uint32_t* cell_end, float* pressure, uint32_t n, bool prefetch,
+                          const uint32_t* cell_end, float* pressure, uint32_t n, int mode,
                           const ControlOut* ctl, const uint32_t* gat) {
   if (n == 0) return hipSuccess;
   const bool multi = c.num_instances > 1;
@@ -817,7 +837,7 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
                                     const uint32_t* type, const float* radius, const uint32_t* cell_start,
                                     const uint32_t* cell_end, float2* acc, float* alpha, float* pressure,
                                     float2* pos_out, float2* vel_out, float* omega_out, float* theta_out,
-                                    const NextKeys& next, uint32_t n, bool prefetch, const ControlOut* ctl) {
+                                    const NextKeys& next, uint32_t n, int mode, const ControlOut* ctl) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
   const bool multi = c.num_instances > 1;
@@ -831,6 +851,7 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
   return hipGetLastError();
 }
 #undef DEM_DISPATCH
+#undef DEM_DISPATCH_RM
 
 // ---- controlled particle: reaction sum and move ------------------------------
 // reduceVecD (src/particles_kernel.cuh:552-578) on the three reaction

@@ -241,11 +241,11 @@ int dem_sim_step_graph(dem_sim* sim, uint32_t steps);
  * integrate as one kernel (timed in ms[4]; ms[5] is then 0). */
 int dem_sim_stage_times(dem_sim* sim, uint32_t reps, float* ms);
 /* Choose block sizes per kernel, whether the sim runs collide and
- * integrate as one kernel or two, and whether collide fetches each
- * neighbour candidate one iteration ahead, by timing candidates on the current state
- * (the state is not modified).  Writes the choice to block[5] (hash,
- * reorder, collide, integrate, flags: 1 if fused, 2 if collide prefetches
- * candidates) if non-NULL. */
+ * integrate as one kernel or two, and how collide loads its neighbour
+ * candidates (DEM_STEP_PREFETCH, DEM_STEP_BATCH or neither), by timing
+ * candidates on the current state (the state is not modified).  Writes the
+ * choice to block[5] (hash, reorder, collide, integrate, flags: the
+ * DEM_STEP_* bits chosen) if non-NULL. */
 int dem_sim_autotune(dem_sim* sim, uint32_t* block);
 /* Sort method of the sim's steps.  After a step the state is in the sorted
  * order of its keys, so the next step can re-sort: only particles whose cell
@@ -262,9 +262,12 @@ int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
 /* Kernel variants of the owned simulations of ctx (dem_sim_autotune sets
  * the fastest): DEM_STEP_FUSED runs collide and integrate as one kernel,
  * DEM_STEP_PREFETCH makes collide fetch each neighbour candidate one
- * iteration ahead.  Results are the same bit for bit. */
+ * iteration ahead, DEM_STEP_BATCH makes it load the first three candidates
+ * of all three neighbour rows at once (not together with PREFETCH).  Results
+ * are the same bit for bit. */
 #define DEM_STEP_FUSED 0x1
 #define DEM_STEP_PREFETCH 0x2
+#define DEM_STEP_BATCH 0x4
 int dem_set_step_variant(dem_context* ctx, int flags);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);

@@ -75,7 +75,9 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
     choice = sim.autotune()
     fused = choice.pop("fused")
     prefetch = choice.pop("prefetch")
-    assert isinstance(fused, bool) and isinstance(prefetch, bool)
+    batch = choice.pop("batch")
+    assert isinstance(fused, bool) and isinstance(prefetch, bool) and isinstance(batch, bool)
+    assert not (prefetch and batch)
     assert set(choice.values()) <= {64, 128, 256}
     sim.step(4)
     ref.step(4)
@@ -83,11 +85,13 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
     sim.close()
 
 
-@pytest.mark.parametrize("variant", [(True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("variant", [(True, False), (False, True), (True, True), (False, False, True),
+                                     (True, False, True)])
 @pytest.mark.parametrize("name", ["pack_1024", "ensemble_5", "pile_wrap_attract"])
 def test_step_variants_equal(gpu_ctx, name, variant):
-    """collide + integrate as one kernel or two, with or without collide's
-    candidate prefetch: the same state, bit for bit."""
+    """collide + integrate as one kernel or two, and each way of loading
+    collide's candidates (one at a time, one ahead, three a row at once):
+    the same state, bit for bit."""
     make, steps = CASES[name]
     s = make()
     ctx2 = api.Context(0)
@@ -145,7 +149,8 @@ def test_dense_wide_pack_matches_oracle(gpu_ctx, oracle):
     """65,536 per-particle-radius discs, contacts throughout, 36 candidates a
     disc: the owned step (re-sorted, velocities read through the sorted
     index) equals the oracle bit for bit after each of 3 steps, and the
-    fused variant equals the split one after 20."""
+    fused variant with batched candidate loads equals the split one after
+    20."""
     s = wide_pack()
     gpu_ctx.load_scene(s)
     gpu_ctx.set_step_variant(False, False)
@@ -160,7 +165,7 @@ def test_dense_wide_pack_matches_oracle(gpu_ctx, oracle):
     assert np.count_nonzero(g["pressure"]) > s.n // 4   # contacts throughout
     ctx2 = api.Context(0)
     ctx2.load_scene(s)
-    ctx2.set_step_variant(True, False)
+    ctx2.set_step_variant(True, False, True)
     fused = api.Simulation(ctx2, s)
     fused.step(3)
     sim.step_graph(17)
