@@ -548,7 +548,10 @@ hipError_t copy_state(dem_sim* m, AdjCheckpoint& c, bool to_checkpoint, hipStrea
     if (e != hipSuccess) return e;
   }
   if (to_checkpoint) c.cur = m->cur;
-  else m->cur = c.cur;
+  else {
+    m->cur = c.cur;
+    m->cell_start = m->cs[m->cur];
+  }
   return hipSuccess;
 }
 

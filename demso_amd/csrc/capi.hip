@@ -502,8 +502,10 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
     // the keys, mover bits and counts were written by the previous step's
     // collide + integrate (the hash stage is fused into it)
     if (ev) (void)hipEventRecord(ev[1], s);
-    e = dem::launch_resort(s, m->key, m->skey[o], m->cell_start, m->cell_end, m->cells, true, ctx->key_bits,
-                           m->skey[t], m->sidx, m->resort, n);
+    // the placement kernel also clears this step's cellStart (cs[t]) while
+    // its movers look up the previous step's (cs[o])
+    e = dem::launch_resort(s, m->key, m->skey[o], m->cs[o], m->cell_end, m->cells, true, ctx->key_bits,
+                           m->skey[t], m->sidx, m->resort, n, m->cs[t], m->cells);
     if (e != hipSuccess) return e;
   } else {
     e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], type_for_hash, m->key, m->idx, n);
@@ -525,10 +527,12 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   m->sorted_key = m->skey[t];
   m->sorted_idx = m->sidx;
   if (ev) (void)hipEventRecord(ev[2], s);
-  e = dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells);
-  if (e != hipSuccess) return e;
+  if (!resort) {
+    e = dem::launch_fill(s, m->cs[t], dem::kEmpty, m->cells);
+    if (e != hipSuccess) return e;
+  }
   if (ev) (void)hipEventRecord(ev[3], s);
-  e = dem::launch_reorder(s, ctx->bs_reorder, m->cell_start, m->cell_end, m->pos[S], gather ? nullptr : m->vel[S],
+  e = dem::launch_reorder(s, ctx->bs_reorder, m->cs[t], m->cell_end, m->pos[S], gather ? nullptr : m->vel[S],
                           gather ? nullptr : m->theta[S], gather ? nullptr : m->omega[S], m->id[t], m->loc[t],
                           m->type[t], m->rad ? m->radius[t] : nullptr, m->sorted_key, m->sorted_idx, m->pos[o],
                           m->vel[o], m->theta[o], m->omega[o], m->id[o], m->type[o],
@@ -553,7 +557,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   }
   if (!gather && ctx->fuse) {
     e = dem::launch_collide_integrate(c, s, ctx->bs_collide, m->pos[S], m->vel[S], m->omega[S], m->theta[S],
-                                      m->type[t], S_radius, m->cell_start, m->cell_end, m->acc, m->alpha, m->pressure,
+                                      m->type[t], S_radius, m->cs[t], m->cell_end, m->acc, m->alpha, m->pressure,
                                       m->pos[t], m->vel[t], m->omega[t], m->theta[t], nk, n, ctx->cmode,
                                       control ? &co : nullptr);
     if (e != hipSuccess) return e;
@@ -568,7 +572,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
     const int D = gather ? o : S;
     const uint32_t* gat = gather ? m->sorted_idx : nullptr;
     e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[S], m->vel[D], m->acc, m->omega[D], m->alpha, m->type[t],
-                            S_radius, m->cell_start, m->cell_end, m->pressure, n, ctx->cmode,
+                            S_radius, m->cs[t], m->cell_end, m->pressure, n, ctx->cmode,
                             control ? &co : nullptr, gat);
     if (e != hipSuccess) return e;
     if (control) {
@@ -591,6 +595,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   }
   if (ev) (void)hipEventRecord(ev[6], s);
   m->cur = t;
+  m->cell_start = m->cs[t];
   return hipSuccess;
 }
 
@@ -631,6 +636,7 @@ int sim_capture(dem_sim* m) {
       hipError_t e2 = e1 == hipSuccess ? sim_step_on(m, m->cap_stream, nullptr, variant == 1) : e1;
       hipError_t e3 = hipStreamEndCapture(m->cap_stream, &g);
       m->cur = saved;
+      m->cell_start = m->cs[saved];
       m->sorted_key = saved_key;
       m->sorted_idx = saved_idx;
       if (e2 != hipSuccess) {
@@ -692,7 +698,8 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   if (!r) r = sim_alloc(m, &m->react_bm, react_blocks);
   if (!r) r = sim_alloc(m, &m->key, n);
   if (!r) r = sim_alloc(m, &m->idx, n);
-  if (!r) r = sim_alloc(m, &m->cell_start, num_cells);
+  if (!r) r = sim_alloc(m, &m->cs[0], num_cells);
+  if (!r) r = sim_alloc(m, &m->cs[1], num_cells);
   if (!r) r = sim_alloc(m, &m->cell_end, num_cells);
   if (!r) r = alloc_sort(ctx, m->sort, n);
   const uint32_t words = (n + 31u) / 32u, tiles = (n + dem::kResortTile - 1u) / dem::kResortTile;
@@ -724,13 +731,15 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   }
   // cellEnd starts at 0 as in allocateVectors (src/functions.cu:140).
   hipError_t e = dem::launch_fill(ctx->stream, m->cell_end, 0u, num_cells);
-  if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->cell_start, dem::kEmpty, num_cells);
+  if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->cs[0], dem::kEmpty, num_cells);
+  if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->cs[1], dem::kEmpty, num_cells);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     sim_free(m);
     return hip_fail(ctx, e, "sim init");
   }
   m->sorted_key = m->key;
+  m->cell_start = m->cs[m->cur];
   m->sorted_idx = m->idx;
   *out = m;
   return DEM_OK;

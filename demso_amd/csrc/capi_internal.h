@@ -69,6 +69,10 @@ struct dem_sim {
   float *alpha = nullptr, *pressure = nullptr;
   uint32_t *key = nullptr, *idx = nullptr;
   uint32_t *sorted_key = nullptr, *sorted_idx = nullptr;
+  // cellStart of the state in buffer b is cs[b] (a step reads the previous
+  // one for the re-sort while it clears and rewrites the next); cell_start
+  // names cs[cur].  cellEnd is one array (only read where cellStart is set).
+  uint32_t* cs[2] = {};
   uint32_t *cell_start = nullptr, *cell_end = nullptr;
   dem::SortScratch sort;
   // Re-sort (dem::launch_resort): skey[b] holds the sorted keys of the last

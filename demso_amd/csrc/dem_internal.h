@@ -78,11 +78,14 @@ struct ResortScratch {
 // prev_key) from the mover bits and counts written with it (NextKeys):
 // writes the sorted keys and the source index of every sorted slot.  Equal
 // to a stable sort of key by position for any number of movers; its cost
-// grows with the number of movers.  prev_cell_start/prev_cell_end are the
+// grows with the number of movers.  With clear, the placement kernel also
+// writes kEmpty to clear[0, clear_n) (the next step's cellStart reset).
+// prev_cell_start/prev_cell_end are the
 // cell ranges of prev_key when cells_valid (a faster lookup), else unused.
 hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* prev_key, const uint32_t* prev_cell_start,
                          const uint32_t* prev_cell_end, uint32_t num_cells, bool cells_valid, uint32_t key_bits,
-                         uint32_t* out_key, uint32_t* out_idx, const ResortScratch& r, uint32_t n);
+                         uint32_t* out_key, uint32_t* out_idx, const ResortScratch& r, uint32_t n,
+                         uint32_t* clear = nullptr, uint32_t clear_n = 0);
 hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32_t* cell_end,
                           float2* s_pos, float2* s_vel, float* s_theta, float* s_omega,
                           uint32_t* s_id, uint32_t* s_loc, uint32_t* s_type, float* s_radius,

@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     const uint32_t* __restrict__ word_prefix, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
     const uint32_t* mpos1, const uint32_t* __restrict__ info, const uint32_t* __restrict__ prev_cell_start,
     const uint32_t* __restrict__ prev_cell_end, uint32_t num_cells, int cells_valid, uint32_t* __restrict__ out_key,
-    uint32_t* __restrict__ out_idx, uint32_t n) {
+    uint32_t* __restrict__ out_idx, uint32_t* __restrict__ clear, uint32_t clear_n, uint32_t n) {
   __shared__ uint64_t win[kWindowMax];
   __shared__ uint32_t s_lo, s_hi;
   const uint32_t m = info[0];
@@ -322,7 +322,15 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
   // among the tile's stayers are staged in LDS: all of them when there are
   // at most kWindowMax (no search in global memory at all), else the window
   // [lo, hi) found by two binary searches.
-  const uint32_t first = (blockIdx.x - kMoverBlocks) * kPlaceTile;
+  const uint32_t tile = blockIdx.x - kMoverBlocks;
+  const uint32_t tiles = gridDim.x - kMoverBlocks;
+  if (clear) {
+    // this tile's share of the next step's cellStart reset
+    const uint32_t per = (clear_n + tiles - 1u) / tiles;
+    const uint32_t c0 = min(tile * per, clear_n), c1 = min(c0 + per, clear_n);
+    for (uint32_t k = c0 + tid; k < c1; k += kPlaceThreads) clear[k] = kEmpty;
+  }
+  const uint32_t first = tile * kPlaceTile;
   if (first >= n) return;
   const uint32_t last = min(first + kPlaceTile, n) - 1u;
   uint32_t lo = 0, hi = m;
@@ -378,7 +386,8 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
 
 hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* prev_key, const uint32_t* prev_cell_start,
                          const uint32_t* prev_cell_end, uint32_t num_cells, bool cells_valid, uint32_t key_bits,
-                         uint32_t* out_key, uint32_t* out_idx, const ResortScratch& r, uint32_t n) {
+                         uint32_t* out_key, uint32_t* out_idx, const ResortScratch& r, uint32_t n, uint32_t* clear,
+                         uint32_t clear_n) {
   if (n == 0) return hipSuccess;
   if (n > r.capacity || !r.bits || !r.word_prefix || !r.tile_count || !r.mkey[0] || !r.mkey[1] || !r.mpos[0] ||
       !r.mpos[1] || !r.info || key_bits > 32)
@@ -391,7 +400,7 @@ hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* pre
   const uint32_t place_tiles = (n + kPlaceTile - 1u) / kPlaceTile;
   hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + place_tiles), dim3(kPlaceThreads), 0, s, key, prev_key, r.bits,
                      r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, prev_cell_start,
-                     prev_cell_end, num_cells, cells_valid ? 1 : 0, out_key, out_idx, n);
+                     prev_cell_end, num_cells, cells_valid ? 1 : 0, out_key, out_idx, clear, clear_n, n);
   return hipGetLastError();
 }
 
