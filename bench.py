@@ -38,7 +38,8 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 #             the previous collide): R pos 8; W key 4, index 4
 #   sort      re-sort: R key 4;                W sorted key 4, index 4
 #             (a radix pass would be R key+index 8, W key+index 8)
-#   fill      W cellStart 4 per cell
+#   fill      W cellStart 4 per cell: on re-sorted steps the placement kernel
+#             clears the next step's cellStart, so these bytes count under sort
 #   reorder   R key 4, index 4, pos 8, id 4, type 4, radius 4;
 #             W pos 8, id 4, loc 4, type 4, radius 4
 #             (velocity, angle and spin stay where they are: collide and
@@ -55,7 +56,7 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 #   type 4, previous key 4; W acc 8, alpha 4, pressure 4, pos 8, vel 8,
 #   theta 4, omega 4, key 4 (80) + 8 per cell, reported under collide.
 BYTES_PER_PARTICLE = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 52, "collide": 48, "integrate": 84}
-BYTES_PER_CELL = {"hash": 0, "sort": 0, "cell_fill": 4, "reorder": 0, "collide": 8, "integrate": 0}
+BYTES_PER_CELL = {"hash": 0, "sort": 4, "cell_fill": 0, "reorder": 0, "collide": 8, "integrate": 0}
 FUSED_COLLIDE_BYTES = 80
 FULL_REORDER_BYTES = 84
 

@@ -1218,6 +1218,7 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
       e = time_it(collide, bs, &tm);
       if (e != hipSuccess) break;
       if (tm < t_c[p]) { t_c[p] = tm; best_c[p] = bs; }
+      if (p == 2) continue;  // the fused kernel has no batched mode
       e = time_it(fused, bs, &tm);
       if (e != hipSuccess) break;
       if (tm < t_f[p]) { t_f[p] = tm; best_f[p] = bs; }

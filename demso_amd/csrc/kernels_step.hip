@@ -846,7 +846,11 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
   hipLaunchKernelGGL((k_collide_integrate<R, M, P, X>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
                      omega_in, theta_in, type, radius, cell_start, cell_end, acc, alpha, pressure, pos_out,        \
                      vel_out, omega_out, theta_out, next, co, n)
-  DEM_DISPATCH(DEM_CI);
+  // the fused kernel has no batched mode: with integrate's registers on top
+  // of the batch it spills heavily; it loads one candidate at a time instead
+  if (ctl) DEM_DISPATCH_RM(DEM_CI, kPlain, true);
+  else if (mode == kPrefetch) DEM_DISPATCH_RM(DEM_CI, kPrefetch, false);
+  else DEM_DISPATCH_RM(DEM_CI, kPlain, false);
 #undef DEM_CI
   return hipGetLastError();
 }
