@@ -551,6 +551,7 @@ hipError_t copy_state(dem_sim* m, AdjCheckpoint& c, bool to_checkpoint, hipStrea
   else {
     m->cur = c.cur;
     m->cell_start = m->cs[m->cur];
+    m->perm_list = false;
   }
   return hipSuccess;
 }

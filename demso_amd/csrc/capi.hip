@@ -480,6 +480,83 @@ void sim_free(dem_sim* m) {
 // events or NULL) brackets each stage for timing.  The sorted keys go to
 // skey[t] and the sorted index to sidx with either sort, so the next step
 // can re-sort from them.
+namespace {
+
+dem::ShiftState shift_state(dem_sim* m) {
+  const int b = m->cur;
+  dem::ShiftState st;
+  st.pos = m->pos[b];
+  st.vel = m->vel[b];
+  st.theta = m->theta[b];
+  st.omega = m->omega[b];
+  st.radius = m->rad ? m->radius[b] : nullptr;
+  st.id = m->id[b];
+  st.type = m->type[b];
+  st.loc = m->loc[b];
+  st.skey = m->skey[b];
+  st.cell_start = m->cs[b];
+  st.cell_end = m->cell_end;
+  return st;
+}
+
+// A re-sorted step that keeps the state in its buffer: the particles whose
+// slot or key changes are moved where they belong, cellStart/cellEnd are
+// fixed around them, and collide + integrate run on the sorted state in
+// place (integrate computes the next step's keys and mover bits as in the
+// other layout).
+hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev) {
+  dem_context* ctx = m->ctx;
+  const StepConst& c = ctx->c;
+  const int b = m->cur;
+  const uint32_t n = m->n;
+  const dem::ShiftState st = shift_state(m);
+  hipError_t e;
+  if (ev) (void)hipEventRecord(ev[0], s);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  e = dem::launch_resort_shift_plan(s, m->key, st, m->shift, m->cells, ctx->key_bits, m->resort, n);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[2], s);
+  if (ev) (void)hipEventRecord(ev[3], s);
+  e = dem::launch_resort_shift_apply(s, st, m->shift, m->cells, n);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[4], s);
+  ControlOut co;
+  const bool control = ctx->h_control.enabled != 0;
+  if (control) {
+    co.state = ctx->d_control;
+    co.react_x = m->react_x;
+    co.react_y = m->react_y;
+    co.react_m = m->react_m;
+  }
+  const float* radius = m->rad ? m->radius[b] : nullptr;
+  e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[b], m->vel[b], m->acc, m->omega[b], m->alpha, m->type[b],
+                          radius, m->cs[b], m->cell_end, m->pressure, n, ctx->cmode, control ? &co : nullptr);
+  if (e != hipSuccess) return e;
+  if (control) {
+    e = dem::launch_control_finish(s, co, m->react_bx, m->react_by, m->react_bm, n, ctx->d_reaction, ctx->d_control,
+                                   true);
+    if (e != hipSuccess) return e;
+  }
+  if (ev) (void)hipEventRecord(ev[5], s);
+  dem::NextKeys nk;
+  nk.prev_key = m->skey[b];
+  nk.key = m->key;
+  nk.bits = m->resort.bits;
+  nk.tile_count = m->resort.tile_count;
+  e = dem::launch_integrate_to(c, s, ctx->bs_integrate, m->pos[b], m->vel[b], m->theta[b], m->omega[b], m->pos[b],
+                               m->vel[b], m->theta[b], m->omega[b], m->acc, m->alpha, m->type[b], radius,
+                               m->pressure, n, &nk);
+  if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[6], s);
+  m->sorted_key = m->skey[b];
+  m->sorted_idx = m->sidx;
+  m->cell_start = m->cs[b];
+  m->perm_list = true;
+  return hipSuccess;
+}
+
+}  // namespace
+
 hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, const StepHook* hook) {
   dem_context* ctx = m->ctx;
   const StepConst& c = ctx->c;
@@ -491,6 +568,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   //  * full (fused collide + integrate, or a hook that records the sorted
   //    pre-integrate state): reorder sorts every field into kScratch, from
   //    which collide + integrate write buffer t.
+  if (resort && sim_inplace(m, hook)) return sim_step_inplace(m, s, ev);
   const int o = m->cur, t = o ^ 1;
   const bool gather = !ctx->fuse && !(hook && hook->fn);
   const int S = gather ? t : kScratch;
@@ -596,7 +674,12 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   if (ev) (void)hipEventRecord(ev[6], s);
   m->cur = t;
   m->cell_start = m->cs[t];
+  m->perm_list = false;
   return hipSuccess;
+}
+
+bool sim_inplace(const dem_sim* m, const StepHook* hook) {
+  return !m->ctx->copy && !m->ctx->fuse && !(hook && hook->fn);
 }
 
 bool sim_use_resort(const dem_sim* m) {
@@ -627,6 +710,7 @@ int sim_capture(dem_sim* m) {
   const int saved = m->cur;
   uint32_t* const saved_key = m->sorted_key;
   uint32_t* const saved_idx = m->sorted_idx;
+  const bool saved_list = m->perm_list;
   for (int variant = 0; variant < 2; ++variant) {
     for (int start = 0; start < 2; ++start) {
       hipGraph_t g = nullptr;
@@ -639,6 +723,7 @@ int sim_capture(dem_sim* m) {
       m->cell_start = m->cs[saved];
       m->sorted_key = saved_key;
       m->sorted_idx = saved_idx;
+      m->perm_list = saved_list;
       if (e2 != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return hip_fail(ctx, e2, "capturing the step");
@@ -713,6 +798,20 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   if (!r) r = sim_alloc(m, &m->resort.word_prefix, words);
   if (!r) r = sim_alloc(m, &m->resort.tile_count, tiles);
   if (!r) r = sim_alloc(m, &m->resort.info, 2);
+  // in-place re-sort: list, staging (dynamic fields in kScratch) and counts
+  if (!r) r = sim_alloc(m, &m->shift.key, n);
+  if (!r) r = sim_alloc(m, &m->shift.id, n);
+  if (!r) r = sim_alloc(m, &m->shift.type, n);
+  if (!r) r = sim_alloc(m, &m->shift.dst, n);
+  if (!r) r = sim_alloc(m, &m->shift.src, n);
+  if (!r) r = sim_alloc(m, &m->shift.emptied, n);
+  if (!r && m->rad) r = sim_alloc(m, &m->shift.radius, n);
+  if (!r) r = sim_alloc(m, &m->shift.count, 2);
+  m->shift.pos = m->pos[kScratch];
+  m->shift.vel = m->vel[kScratch];
+  m->shift.theta = m->theta[kScratch];
+  m->shift.omega = m->omega[kScratch];
+  m->shift.capacity = n;
   if (!r) {
     if (hipHostMalloc((void**)&m->h_count, sizeof(uint32_t), hipHostMallocMapped) != hipSuccess) {
       m->h_count = nullptr;
@@ -731,6 +830,7 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   }
   // cellEnd starts at 0 as in allocateVectors (src/functions.cu:140).
   hipError_t e = dem::launch_fill(ctx->stream, m->cell_end, 0u, num_cells);
+  if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->shift.count, 0u, 2);
   if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->cs[0], dem::kEmpty, num_cells);
   if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->cs[1], dem::kEmpty, num_cells);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -773,6 +873,7 @@ int dem_sim_upload(dem_sim* m, const float* pos, const float* vel, const float* 
   if (m->rad) DEM_HIP(ctx, hipMemcpyAsync(m->radius[c], radius, 4 * n, hipMemcpyDefault, s), "upload");
   DEM_HIP(ctx, hipStreamSynchronize(s), "upload sync");
   m->resort_ok = false;  // an uploaded state is in no known sorted order
+  m->perm_list = false;
   *m->h_count = 0;
   return DEM_OK;
 }
@@ -786,6 +887,12 @@ int dem_sim_download(dem_sim* m, float* pos, float* vel, float* theta, float* om
   const int c = m->cur;
   const size_t n = m->n;
   hipStream_t s = ctx->stream;
+  if (index && m->perm_list) {
+    // the last step re-sorted in place: its index is the identity except
+    // at the slots it listed
+    DEM_HIP(ctx, dem::launch_shift_index(s, m->shift, m->sidx, m->n), "index");
+    m->sorted_idx = m->sidx;
+  }
   struct Item { void* dst; const void* src; size_t bytes; } items[] = {
       {pos, m->pos[c], 8 * n},      {vel, m->vel[c], 8 * n},       {theta, m->theta[c], 4 * n},
       {omega, m->omega[c], 4 * n},  {id, m->id[c], 4 * n},             {type, m->type[c], 4 * n},
@@ -830,6 +937,7 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
     const bool resort = sim_use_resort(m);
     DEM_HIP(ctx, hipGraphLaunch(resort ? m->exec_inc[m->cur] : m->exec[m->cur], ctx->stream), "hipGraphLaunch");
     m->resort_ok = true;
+    m->perm_list = resort && sim_inplace(m);
   }
   // Each replay runs two steps, which leaves the state in the same buffer;
   // the sorted keys are in skey[cur] and the sorted index in sidx.
@@ -840,12 +948,13 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
 
 int dem_set_step_variant(dem_context* ctx, int flags) {
   if (!ctx) return DEM_ERR_INVALID;
-  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH))
+  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH | DEM_STEP_COPY))
     return fail(ctx, DEM_ERR_INVALID, "unknown step variant flags");
   if ((flags & DEM_STEP_PREFETCH) && (flags & DEM_STEP_BATCH))
     return fail(ctx, DEM_ERR_INVALID, "DEM_STEP_PREFETCH and DEM_STEP_BATCH exclude each other");
   ctx->fuse = (flags & DEM_STEP_FUSED) != 0;
   ctx->cmode = (flags & DEM_STEP_PREFETCH) ? 1 : ((flags & DEM_STEP_BATCH) ? 2 : 0);
+  ctx->copy = (flags & DEM_STEP_COPY) != 0;
   ctx->generation++;  // captured graphs follow the choice
   return DEM_OK;
 }
@@ -1176,15 +1285,21 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   const float* S_omega = m->omega[K];
   const float* S_radius = m->rad ? m->radius[t] : nullptr;
   int pf = 0;  // candidate mode under test: 0 plain, 1 prefetch, 2 batch
-  // the split layout as the step runs it: velocity, spin and angle from the
-  // live state (buffer o) through the sorted index
+  // the split layout as its re-sorted steps run it: in place, on the fully
+  // sorted state; else velocity, spin and angle from the live state
+  // (buffer o) through the sorted index
+  const bool inplace = !ctx->copy;
+  const float2* D_vel = inplace ? S_vel : m->vel[o];
+  const float* D_omega = inplace ? S_omega : m->omega[o];
+  const float* D_theta = inplace ? S_theta : m->theta[o];
+  const uint32_t* gat = inplace ? nullptr : si;
   auto collide = [&](int b) {
-    return dem::launch_collide(c, s, b, S_pos, m->vel[o], m->acc, m->omega[o], m->alpha, m->type[t], S_radius,
-                               m->cell_start, m->cell_end, m->pressure, n, pf, nullptr, si);
+    return dem::launch_collide(c, s, b, S_pos, D_vel, m->acc, D_omega, m->alpha, m->type[t], S_radius,
+                               m->cell_start, m->cell_end, m->pressure, n, pf, nullptr, gat);
   };
   auto integrate = [&](int b) {
-    return dem::launch_integrate_to(c, s, b, S_pos, m->vel[o], m->theta[o], m->omega[o], tpos, tvel, ttheta, tomega,
-                                    m->acc, m->alpha, m->type[t], S_radius, m->pressure, n, &nk, si);
+    return dem::launch_integrate_to(c, s, b, S_pos, D_vel, D_theta, D_omega, tpos, tvel, ttheta, tomega, m->acc,
+                                    m->alpha, m->type[t], S_radius, m->pressure, n, &nk, gat);
   };
   auto fused = [&](int b) {
     return dem::launch_collide_integrate(c, s, b, S_pos, S_vel, S_omega, S_theta, m->type[t], S_radius,
@@ -1224,6 +1339,18 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
       if (tm < t_f[p]) { t_f[p] = tm; best_f[p] = bs; }
     }
   }
+  // the in-place layout's apply launches with nothing listed (their cost
+  // when few particles change slot); the last in-place step's index is
+  // materialized first, since its list is cleared for this
+  float t_apply = 0.0f;
+  if (e == hipSuccess && inplace) {
+    if (m->perm_list) e = dem::launch_shift_index(s, m->shift, m->sidx, n);
+    m->perm_list = false;
+    if (e == hipSuccess) e = dem::launch_fill(s, m->shift.count, 0u, 2);
+    const dem::ShiftState st = shift_state(m);
+    if (e == hipSuccess)
+      e = time_it([&](int) { return dem::launch_resort_shift_apply(s, st, m->shift, m->cells, n); }, 0, &t_apply);
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (tpos) (void)hipFree(tpos);
@@ -1238,8 +1365,9 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   bool fuse = false;
   int mode = 0;
   float tbest = 1e30f;
+  const float t_move = inplace ? t_apply : best_t[1];  // the split layout's reorder
   for (int p = 0; p < 3; ++p) {
-    if (best_t[1] + t_c[p] + best_t[3] < tbest) { tbest = best_t[1] + t_c[p] + best_t[3]; fuse = false; mode = p; }
+    if (t_move + t_c[p] + best_t[3] < tbest) { tbest = t_move + t_c[p] + best_t[3]; fuse = false; mode = p; }
     if (t_full + t_f[p] < tbest) { tbest = t_full + t_f[p]; fuse = true; mode = p; }
   }
   ctx->bs_hash = best[0];
@@ -1254,7 +1382,7 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
     block[1] = (uint32_t)best[1];
     block[2] = (uint32_t)ctx->bs_collide;
     block[3] = (uint32_t)best[3];
-    block[4] = (fuse ? 1u : 0u) | (mode == 1 ? 2u : 0u) | (mode == 2 ? 4u : 0u);
+    block[4] = (fuse ? 1u : 0u) | (mode == 1 ? 2u : 0u) | (mode == 2 ? 4u : 0u) | (ctx->copy ? 8u : 0u);
   }
   return DEM_OK;
 }

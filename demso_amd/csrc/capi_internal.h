@@ -27,6 +27,9 @@ struct dem_context {
   bool fuse = false;      // owned simulation: collide + integrate as one kernel (chosen by dem_sim_autotune)
   int cmode = 0;  // collide's candidate loads: 0 one at a time, 1 one ahead, 2 first three of every row
                  // at once (chosen by dem_sim_autotune)
+  // owned simulation: re-sorted steps reorder the whole state into the other
+  // buffer (the layout of radix-sorted steps) instead of re-sorting it in place
+  bool copy = false;
   uint64_t generation = 1;  // bumped by every parameter change
   // Controlled particle of the owned simulations: host copy of the last
   // state set, and the device state (moved by every step) followed by the
@@ -79,6 +82,12 @@ struct dem_sim {
   // step that wrote state buffer b, sidx the sorted index of the last step.
   uint32_t *skey[2] = {}, *sidx = nullptr;
   dem::ResortScratch resort;
+  // In-place re-sort (dem::launch_resort_shift_*): the list and staging of
+  // the particles that change slot; pos/vel/theta/omega stage in kScratch.
+  // After an in-place step (perm_list) the sorted index is the identity
+  // except at the listed slots, and sidx is rebuilt from the list on demand.
+  dem::ShiftScratch shift;
+  bool perm_list = false;
   uint32_t* h_count = nullptr;  // host-mapped mirror of the last step's mover count
   bool resort_ok = false;       // state in the order of skey[cur]; cell arrays describe skey[cur]
   int sort_mode = 0;            // 0 auto, 1 radix sort every step, 2 re-sort whenever resort_ok
@@ -104,7 +113,8 @@ bool multi(const dem_context* ctx);
 int sim_check(dem_sim* m);
 // One step (src/main.cu:431-507) on stream s; ev (7 events) or NULL.
 // resort: sort by dem::launch_resort (needs m->resort_ok) instead of the
-// radix sort.
+// radix sort; without a hook and with the split collide/integrate layout the
+// state is then re-sorted in place (unless ctx->copy).
 // hook: launched after collide (before integrate) with the step's target
 // buffer t, e.g. to store the sorted pre-integrate state (the adjoint).
 struct StepHook {
@@ -115,6 +125,8 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
 // Sort method for the next step of m (auto mode: re-sort while the last
 // observed mover count is small).
 bool sim_use_resort(const dem_sim* m);
+// Whether a re-sorted step of m runs in place (see sim_step_on).
+bool sim_inplace(const dem_sim* m, const StepHook* hook = nullptr);
 
 }  // namespace dem
 

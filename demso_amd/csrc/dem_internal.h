@@ -86,6 +86,42 @@ hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* pre
                          const uint32_t* prev_cell_end, uint32_t num_cells, bool cells_valid, uint32_t key_bits,
                          uint32_t* out_key, uint32_t* out_idx, const ResortScratch& r, uint32_t n,
                          uint32_t* clear = nullptr, uint32_t clear_n = 0);
+// In-place re-sort.  When a state is re-sorted where it lies (instead of
+// being reordered into the other buffer), only the particles whose slot or
+// key changes are listed, staged and written back:
+//   plan  : compact + movers as above, then the placement of the movers and
+//           of the stayers whose slot changes, each appended to the list
+//           (new slot, old slot, key) with its state copied into staging,
+//           and the cells that lose all their particles;
+//   apply : the staged state written to the new slots (sorted keys and loc
+//           too), cellStart cleared for the emptied cells, then the cell
+//           boundaries re-evaluated at every new slot and the slot after it.
+// The result equals reorderDataAndFindCellStart (src/particles_kernel.cuh:
+// 154-233) applied to the stable sort of the new keys, for any number of
+// movers: every other cell boundary is the one the previous step wrote.
+struct ShiftState {
+  float2 *pos = nullptr, *vel = nullptr;
+  float *theta = nullptr, *omega = nullptr, *radius = nullptr;
+  uint32_t *id = nullptr, *type = nullptr, *loc = nullptr;
+  uint32_t* skey = nullptr;  // sorted keys: the previous ones on entry, the new ones after apply
+  uint32_t *cell_start = nullptr, *cell_end = nullptr;
+};
+struct ShiftScratch {
+  float2 *pos = nullptr, *vel = nullptr;
+  float *theta = nullptr, *omega = nullptr, *radius = nullptr;
+  uint32_t *id = nullptr, *type = nullptr, *key = nullptr;
+  uint32_t *dst = nullptr, *src = nullptr;  // new and old slot of each listed particle
+  uint32_t* emptied = nullptr;              // cells left without particles
+  uint32_t* count = nullptr;                // [0] listed particles, [1] emptied cells
+  uint32_t capacity = 0;                    // n
+};
+hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const ShiftState& st, const ShiftScratch& sh,
+                                    uint32_t num_cells, uint32_t key_bits, const ResortScratch& r, uint32_t n);
+hipError_t launch_resort_shift_apply(hipStream_t s, const ShiftState& st, const ShiftScratch& sh, uint32_t num_cells,
+                                     uint32_t n);
+// The sorted index of the last in-place re-sort: identity except at the
+// listed slots (index[new] = old).
+hipError_t launch_shift_index(hipStream_t s, const ShiftScratch& sh, uint32_t* index, uint32_t n);
 hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32_t* cell_end,
                           float2* s_pos, float2* s_vel, float* s_theta, float* s_omega,
                           uint32_t* s_id, uint32_t* s_loc, uint32_t* s_type, float* s_radius,

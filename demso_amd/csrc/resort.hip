@@ -31,6 +31,8 @@
 //             mover's stayer count comes from the previous step's cell
 //             ranges (cellStart/cellEnd of prev_key) or a binary search
 //             over prev_key.
+#include <algorithm>
+
 #include "dem_internal.h"
 
 namespace dem {
@@ -136,7 +138,8 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(uint32_t* __rest
                                                                 uint32_t* mkey0, uint32_t* mpos0, uint32_t* mkey1,
                                                                 uint32_t* mpos1, uint32_t key_bits,
                                                                 uint32_t* __restrict__ info,
-                                                                uint32_t* __restrict__ host_count) {
+                                                                uint32_t* __restrict__ host_count,
+                                                                uint32_t* __restrict__ shift_count) {
   __shared__ uint64_t v[kBitonicMax];
   __shared__ uint32_t wsum[kSortWaves];
   __shared__ uint32_t wcnt[kSortWaves][kDigits];
@@ -144,6 +147,8 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(uint32_t* __rest
   __shared__ uint32_t ctot[kDigits];
   __shared__ uint32_t s_m;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave;
+  // the in-place placement that follows appends to these lists
+  if (shift_count && tid < 2) shift_count[tid] = 0;
 
   uint32_t part = 0;
   for (uint32_t t = tid; t < tiles; t += kSortThreads) part += tile_count[t];
@@ -262,6 +267,34 @@ __device__ __forceinline__ uint32_t movers_below(const uint32_t* __restrict__ sk
   return lo;
 }
 
+// Insertion point of a mover's pair (k, p) among the previous sorted pairs
+// (prev_key[q], q): the number of them that are smaller.  From the cell
+// ranges of prev_key when valid, else a binary search over prev_key.
+__device__ __forceinline__ uint32_t insertion_point(uint32_t k, uint32_t p, const uint32_t* __restrict__ prev_key,
+                                                    const uint32_t* __restrict__ prev_cell_start,
+                                                    const uint32_t* __restrict__ prev_cell_end, uint32_t num_cells,
+                                                    int cells_valid, uint32_t n) {
+  if (cells_valid && k < num_cells) {
+    const uint32_t s0 = prev_cell_start[k];
+    if (s0 != kEmpty) {
+      const uint32_t e0 = prev_cell_end[k];
+      return p < s0 ? s0 : (p >= e0 ? e0 : p);
+    }
+    for (uint32_t kk = k + 1u; kk < num_cells && kk <= k + 8u; ++kk) {
+      const uint32_t s1 = prev_cell_start[kk];
+      if (s1 != kEmpty) return s1;
+    }
+  }
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t q = prev_key[mid];
+    if (q < k || (q == k && mid < p)) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     const uint32_t* __restrict__ key, const uint32_t* __restrict__ prev_key, const uint32_t* __restrict__ bits,
     const uint32_t* __restrict__ word_prefix, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
@@ -279,35 +312,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     // movers: slot = rank + stayers before their (key, pos)
     for (uint32_t r = blockIdx.x * kPlaceThreads + tid; r < m; r += kMoverBlocks * kPlaceThreads) {
       const uint32_t k = sk[r], p = sp[r];
-      uint32_t P = 0;
-      bool found = false;
-      if (cells_valid && k < num_cells) {
-        const uint32_t s0 = prev_cell_start[k];
-        if (s0 != kEmpty) {
-          const uint32_t e0 = prev_cell_end[k];
-          P = p < s0 ? s0 : (p >= e0 ? e0 : p);
-          found = true;
-        } else {
-          for (uint32_t kk = k + 1u; kk < num_cells && kk <= k + 8u; ++kk) {
-            const uint32_t s1 = prev_cell_start[kk];
-            if (s1 != kEmpty) {
-              P = s1;
-              found = true;
-              break;
-            }
-          }
-        }
-      }
-      if (!found) {
-        uint32_t lo = 0, hi = n;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          const uint32_t q = prev_key[mid];
-          if (q < k || (q == k && mid < p)) lo = mid + 1;
-          else hi = mid;
-        }
-        P = lo;
-      }
+      const uint32_t P = insertion_point(k, p, prev_key, prev_cell_start, prev_cell_end, num_cells, cells_valid, n);
       const uint32_t slot = r + P - movers_before(P, n, m, bits, word_prefix);
       if (slot < n) {  // always true for a consistent input; guards the store
         out_key[slot] = k;
@@ -382,7 +387,318 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
   }
 }
 
+// ---- in-place re-sort ------------------------------------------------------
+
+// Wave-aggregated append: one atomic per wave, slots in lane order.  Every
+// lane of the wave must call it (uniform control flow).
+__device__ __forceinline__ uint32_t wave_append(bool emit, uint32_t* counter) {
+  const uint64_t b = __ballot(emit);
+  if (b == 0) return 0;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)b) - 1);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(b));
+  base = __shfl(base, (int)leader, kWave);
+  return base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+}
+
+// Sorted movers whose key is below k (keys are non-decreasing in rank).
+__device__ __forceinline__ uint32_t keys_below(const uint32_t* __restrict__ sk, uint32_t m, uint32_t k) {
+  uint32_t lo = 0, hi = m;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (sk[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool kRadius>
+__device__ __forceinline__ void stage(const ShiftScratch& sh, const ShiftState& st, uint32_t at, uint32_t from,
+                                      uint32_t to, uint32_t key) {
+  sh.dst[at] = to;
+  sh.src[at] = from;
+  sh.key[at] = key;
+  sh.pos[at] = st.pos[from];
+  sh.vel[at] = st.vel[from];
+  sh.theta[at] = st.theta[from];
+  sh.omega[at] = st.omega[from];
+  sh.id[at] = st.id[from];
+  sh.type[at] = st.type[from];
+  if (kRadius) sh.radius[at] = st.radius[from];
+}
+
+// The placement of k_resort_place, listing only what changes: every mover
+// (its key changed) and every stayer whose slot differs from its position,
+// with its state staged, plus the old cells of movers that end up empty.
+// A stayer tile is skipped whole when none of its stayers can move: no
+// mover lies in it, no mover's pair sorts among its pairs, and the movers
+// sorting below it equal the movers before it.
+template <bool kRadius>
+__global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
+    const uint32_t* __restrict__ key, ShiftState st, ShiftScratch sh, const uint32_t* __restrict__ bits,
+    const uint32_t* __restrict__ word_prefix, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
+    const uint32_t* mpos1, const uint32_t* __restrict__ info, uint32_t num_cells, uint32_t n) {
+  __shared__ uint64_t win[kWindowMax];
+  __shared__ uint32_t s_lo, s_hi, s_bf, s_bl;
+  const uint32_t m = info[0];
+  if (m == 0) return;
+  const uint32_t* sk = info[1] ? mkey1 : mkey0;
+  const uint32_t* sp = info[1] ? mpos1 : mpos0;
+  const uint32_t* prev_key = st.skey;
+  const uint32_t tid = threadIdx.x;
+
+  if (blockIdx.x < kMoverBlocks) {
+    for (uint32_t r0 = blockIdx.x * kPlaceThreads; r0 < m; r0 += kMoverBlocks * kPlaceThreads) {
+      const uint32_t r = r0 + tid;
+      const bool ok = r < m;
+      uint32_t k = 0, p = 0, slot = n;
+      if (ok) {
+        k = sk[r];
+        p = sp[r];
+        if (p < n) {
+          const uint32_t P = insertion_point(k, p, prev_key, st.cell_start, st.cell_end, num_cells, 1, n);
+          slot = r + P - movers_before(P, n, m, bits, word_prefix);
+        }
+      }
+      const bool emit = ok && p < n && slot < n;
+      const uint32_t at = wave_append(emit, &sh.count[0]);
+      if (emit && at < sh.capacity) stage<kRadius>(sh, st, at, p, slot, k);
+      // the mover's old cell is left empty when all its particles moved
+      // out and none moved in (checked by the cell's first mover only)
+      bool emptied = false;
+      uint32_t ko = 0;
+      if (emit) {
+        ko = prev_key[p];
+        if (ko < num_cells) {
+          const uint32_t s0 = st.cell_start[ko], e0 = st.cell_end[ko];
+          if (s0 != kEmpty && s0 <= p && p < e0) {
+            const uint32_t mb0 = movers_before(s0, n, m, bits, word_prefix);
+            if (movers_before(p, n, m, bits, word_prefix) == mb0) {
+              const uint32_t leaving = movers_before(e0, n, m, bits, word_prefix) - mb0;
+              const uint32_t entering = keys_below(sk, m, ko + 1u) - keys_below(sk, m, ko);
+              emptied = e0 - s0 == leaving && entering == 0;
+            }
+          }
+        }
+      }
+      const uint32_t ea = wave_append(emptied, &sh.count[1]);
+      if (emptied && ea < sh.capacity) sh.emptied[ea] = ko;
+    }
+    return;
+  }
+
+  const uint32_t tile = blockIdx.x - kMoverBlocks;
+  const uint32_t first = tile * kPlaceTile;
+  if (first >= n) return;
+  const uint32_t last = min(first + kPlaceTile, n) - 1u;
+  uint32_t lo = 0, hi = m;
+  if (m > kWindowMax) {
+    if (tid == 0) s_lo = movers_below(sk, sp, 0, m, composite(prev_key[first], first));
+    if (tid == 1) s_hi = movers_below(sk, sp, 0, m, composite(prev_key[last], last));
+    __syncthreads();
+    lo = s_lo;
+    hi = s_hi;
+  }
+  const bool staged = hi - lo <= kWindowMax;
+  if (staged)
+    for (uint32_t j = tid; j < hi - lo; j += kPlaceThreads) win[j] = composite(sk[lo + j], sp[lo + j]);
+  __syncthreads();
+  if (tid < 2) {
+    // movers sorting below the tile's first and last pairs
+    uint32_t b;
+    if (m > kWindowMax) {
+      b = tid == 0 ? lo : hi;
+    } else {
+      const uint32_t q = tid == 0 ? first : last;
+      const uint64_t x = composite(prev_key[q], q);
+      uint32_t a = 0, e = m;
+      while (a < e) {
+        const uint32_t mid = (a + e) >> 1;
+        if (win[mid] < x) a = mid + 1;
+        else e = mid;
+      }
+      b = a;
+    }
+    if (tid == 0) s_bf = b;
+    else s_bl = b;
+  }
+  __syncthreads();
+  {
+    const uint32_t mb_first = movers_before(first, n, m, bits, word_prefix);
+    const uint32_t mb_end = movers_before(last + 1u, n, m, bits, word_prefix);
+    if (mb_first == mb_end && s_bf == s_bl && s_bf == mb_first) return;  // uniform over the block
+  }
+  uint32_t kk[kPlaceItems], bw[kPlaceItems], wp[kPlaceItems];
+#pragma unroll
+  for (uint32_t it = 0; it < kPlaceItems; ++it) {
+    const uint32_t i = first + it * kPlaceThreads + tid;
+    const bool valid = i <= last;
+    kk[it] = valid ? key[i] : 0u;
+    bw[it] = valid ? bits[i >> 5] : 0xFFFFFFFFu;
+    wp[it] = valid ? word_prefix[i >> 5] : 0u;
+  }
+#pragma unroll
+  for (uint32_t it = 0; it < kPlaceItems; ++it) {
+    const uint32_t i = first + it * kPlaceThreads + tid;
+    const uint32_t b = i & 31u;
+    const bool stayer = i <= last && !((bw[it] >> b) & 1u);
+    uint32_t slot = i;
+    if (stayer) {
+      const uint64_t x = composite(kk[it], i);
+      uint32_t below;
+      if (staged) {
+        uint32_t a = 0, e = hi - lo;
+        while (a < e) {
+          const uint32_t mid = (a + e) >> 1;
+          if (win[mid] < x) a = mid + 1;
+          else e = mid;
+        }
+        below = lo + a;
+      } else {
+        below = movers_below(sk, sp, lo, hi, x);
+      }
+      slot = i - (wp[it] + (uint32_t)__popc(bw[it] & ((1u << b) - 1u))) + below;
+    }
+    const bool emit = stayer && slot != i && slot < n;
+    const uint32_t at = wave_append(emit, &sh.count[0]);
+    if (emit && at < sh.capacity) stage<kRadius>(sh, st, at, i, slot, kk[it]);
+  }
+}
+
+// Staged state -> new slots; cellStart of emptied cells -> EMPTY.
+template <bool kRadius>
+__global__ __launch_bounds__(256) void k_shift_scatter(ShiftState st, ShiftScratch sh, uint32_t num_cells,
+                                                       uint32_t n) {
+  const uint32_t S = min(sh.count[0], sh.capacity), E = min(sh.count[1], sh.capacity);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += stride) {
+    const uint32_t d = sh.dst[k];
+    if (d >= n) continue;
+    st.pos[d] = sh.pos[k];
+    st.vel[d] = sh.vel[k];
+    st.theta[d] = sh.theta[k];
+    st.omega[d] = sh.omega[k];
+    const uint32_t id = sh.id[k];
+    st.id[d] = id;
+    if (id < n) st.loc[id] = d;
+    st.type[d] = sh.type[k];
+    if (kRadius) st.radius[d] = sh.radius[k];
+    st.skey[d] = sh.key[k];
+  }
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride) {
+    const uint32_t c = sh.emptied[k];
+    if (c < num_cells) st.cell_start[c] = kEmpty;
+  }
+}
+
+// The boundary test of reorderDataAndFindCellStartD (src/particles_kernel.
+// cuh:189-212) at slot q of the new sorted keys (q == n: the last cell's end).
+__device__ __forceinline__ void cell_record(const uint32_t* __restrict__ skey, uint32_t q, uint32_t* cell_start,
+                                            uint32_t* cell_end, uint32_t num_cells, uint32_t n) {
+  if (q == n) {
+    const uint32_t h = skey[n - 1u];
+    if (h < num_cells) cell_end[h] = n;
+    return;
+  }
+  const uint32_t h = skey[q];
+  if (q == 0) {
+    if (h < num_cells) cell_start[h] = 0;
+    return;
+  }
+  const uint32_t hp = skey[q - 1u];
+  if (h != hp) {
+    if (h < num_cells) cell_start[h] = q;
+    if (hp < num_cells) cell_end[hp] = q;
+  }
+}
+
+// Cell boundaries at every listed new slot and the slot after it: the only
+// places where the key sequence changed.  Two lanes may write the same
+// record; they write the same value.
+__global__ __launch_bounds__(256) void k_shift_records(const uint32_t* __restrict__ skey,
+                                                       const uint32_t* __restrict__ dst,
+                                                       const uint32_t* __restrict__ count, uint32_t capacity,
+                                                       uint32_t* cell_start, uint32_t* cell_end, uint32_t num_cells,
+                                                       uint32_t n) {
+  const uint32_t S = min(count[0], capacity);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += stride) {
+    const uint32_t d = dst[k];
+    if (d >= n) continue;
+    cell_record(skey, d, cell_start, cell_end, num_cells, n);
+    cell_record(skey, d + 1u, cell_start, cell_end, num_cells, n);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ index, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) index[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_shift_index(const uint32_t* __restrict__ dst,
+                                                     const uint32_t* __restrict__ src,
+                                                     const uint32_t* __restrict__ count, uint32_t capacity,
+                                                     uint32_t* __restrict__ index, uint32_t n) {
+  const uint32_t S = min(count[0], capacity);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x) {
+    const uint32_t d = dst[k];
+    if (d < n) index[d] = src[k];
+  }
+}
+
+// Workgroups of the list kernels (grid-stride over a count known only on
+// the device): enough to cover a large list in a few passes.
+uint32_t list_blocks(uint32_t n) { return std::max(1u, std::min(512u, (n + 2047u) / 2048u)); }
+
 }  // namespace
+
+hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const ShiftState& st, const ShiftScratch& sh,
+                                    uint32_t num_cells, uint32_t key_bits, const ResortScratch& r, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  if (n > r.capacity || !r.bits || !r.word_prefix || !r.tile_count || !r.mkey[0] || !r.mkey[1] || !r.mpos[0] ||
+      !r.mpos[1] || !r.info || key_bits > 32 || n > sh.capacity || !sh.count || !sh.dst || !sh.src || !sh.key ||
+      !sh.emptied || !sh.pos || !sh.vel || !sh.theta || !sh.omega || !sh.id || !sh.type || !st.pos || !st.vel ||
+      !st.theta || !st.omega || !st.id || !st.type || !st.skey || !st.cell_start || !st.cell_end ||
+      (st.radius && !sh.radius))
+    return hipErrorInvalidValue;
+  const uint32_t tiles = (n + kResortTile - 1u) / kResortTile;
+  hipLaunchKernelGGL(k_resort_compact, dim3(tiles), dim3(kWordsPerTile), 0, s, key, r.bits, r.tile_count,
+                     r.word_prefix, r.mkey[0], r.mpos[0], n);
+  hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.mkey[0], r.mpos[0],
+                     r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count, sh.count);
+  const dim3 grid(kMoverBlocks + (n + kPlaceTile - 1u) / kPlaceTile);
+  if (st.radius)
+    hipLaunchKernelGGL(k_shift_place<true>, grid, dim3(kPlaceThreads), 0, s, key, st, sh, r.bits, r.word_prefix,
+                       r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, num_cells, n);
+  else
+    hipLaunchKernelGGL(k_shift_place<false>, grid, dim3(kPlaceThreads), 0, s, key, st, sh, r.bits, r.word_prefix,
+                       r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, num_cells, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_resort_shift_apply(hipStream_t s, const ShiftState& st, const ShiftScratch& sh, uint32_t num_cells,
+                                     uint32_t n) {
+  if (n == 0) return hipSuccess;
+  if (n > sh.capacity || !sh.count || !sh.dst || !st.skey || !st.loc || !st.cell_start || !st.cell_end ||
+      (st.radius && !sh.radius))
+    return hipErrorInvalidValue;
+  const dim3 grid(list_blocks(n));
+  if (st.radius) hipLaunchKernelGGL(k_shift_scatter<true>, grid, dim3(256), 0, s, st, sh, num_cells, n);
+  else hipLaunchKernelGGL(k_shift_scatter<false>, grid, dim3(256), 0, s, st, sh, num_cells, n);
+  hipLaunchKernelGGL(k_shift_records, grid, dim3(256), 0, s, st.skey, sh.dst, sh.count, sh.capacity, st.cell_start,
+                     st.cell_end, num_cells, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_shift_index(hipStream_t s, const ShiftScratch& sh, uint32_t* index, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  if (n > sh.capacity || !sh.count || !sh.dst || !sh.src || !index) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_iota, dim3((n + 255u) / 256u), dim3(256), 0, s, index, n);
+  hipLaunchKernelGGL(k_shift_index, dim3(list_blocks(n)), dim3(256), 0, s, sh.dst, sh.src, sh.count, sh.capacity,
+                     index, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* prev_key, const uint32_t* prev_cell_start,
                          const uint32_t* prev_cell_end, uint32_t num_cells, bool cells_valid, uint32_t key_bits,
@@ -396,7 +712,7 @@ hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* pre
   hipLaunchKernelGGL(k_resort_compact, dim3(tiles), dim3(kWordsPerTile), 0, s, key, r.bits, r.tile_count,
                      r.word_prefix, r.mkey[0], r.mpos[0], n);
   hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.mkey[0], r.mpos[0],
-                     r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count);
+                     r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count, nullptr);
   const uint32_t place_tiles = (n + kPlaceTile - 1u) / kPlaceTile;
   hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + place_tiles), dim3(kPlaceThreads), 0, s, key, prev_key, r.bits,
                      r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, prev_cell_start,

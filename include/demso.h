@@ -268,6 +268,11 @@ int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
 #define DEM_STEP_FUSED 0x1
 #define DEM_STEP_PREFETCH 0x2
 #define DEM_STEP_BATCH 0x4
+/* Re-sorted steps of the split layout normally keep the state in its buffer
+ * and move only the particles whose sorted slot changes (plus the cell
+ * ranges around them); DEM_STEP_COPY makes them reorder the whole state
+ * into the other buffer instead, as radix-sorted steps do.  Same results. */
+#define DEM_STEP_COPY 0x8
 int dem_set_step_variant(dem_context* ctx, int flags);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);

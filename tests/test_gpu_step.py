@@ -76,8 +76,9 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
     fused = choice.pop("fused")
     prefetch = choice.pop("prefetch")
     batch = choice.pop("batch")
+    copy = choice.pop("copy")
     assert isinstance(fused, bool) and isinstance(prefetch, bool) and isinstance(batch, bool)
-    assert not (prefetch and batch)
+    assert not (prefetch and batch) and copy is False  # the tuner keeps the context's re-sort layout
     assert set(choice.values()) <= {64, 128, 256}
     sim.step(4)
     ref.step(4)
@@ -86,12 +87,14 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
 
 
 @pytest.mark.parametrize("variant", [(True, False), (False, True), (True, True), (False, False, True),
+                                     (False, False, False, True), (False, False, True, True),
                                      (True, False, True)])
 @pytest.mark.parametrize("name", ["pack_1024", "ensemble_5", "pile_wrap_attract"])
 def test_step_variants_equal(gpu_ctx, name, variant):
     """collide + integrate as one kernel or two, and each way of loading
-    collide's candidates (one at a time, one ahead, three a row at once):
-    the same state, bit for bit."""
+    collide's candidates (one at a time, one ahead, three a row at once),
+    re-sorted steps in place or into the other buffer (copy): the same
+    state and cell ranges, bit for bit."""
     make, steps = CASES[name]
     s = make()
     ctx2 = api.Context(0)
@@ -105,7 +108,8 @@ def test_step_variants_equal(gpu_ctx, name, variant):
         a.step_graph(chunk)
         b.step_graph(chunk)
         da, db = a.download(), b.download()
-        for k in ("hash", "index", "cellStart", "id", "pos", "vel", "theta", "omega", "acc", "alpha", "pressure"):
+        for k in ("hash", "index", "cellStart", "cellEnd", "id", "loc", "pos", "vel", "theta", "omega", "acc",
+                  "alpha", "pressure"):
             assert bits_equal(da[k], db[k]), k
     a.close()
     b.close()
