@@ -86,10 +86,31 @@ int alloc_sort(dem_context* ctx, SortScratch& s, uint32_t n) {
   return DEM_OK;
 }
 
+// seriesAssociation (src/particles_kernel.cuh:36-41) in float, as the
+// kernels compute it.
+float series_host(float a, float b) {
+  if (a == 0.0f || b == 0.0f) return 0.0f;
+  return (a * b) / (a + b);
+}
+
 int upload_tables(dem_context* ctx) {
   const uint32_t nt = (uint32_t)ctx->h_mat.size(), ni = (uint32_t)ctx->h_inst.size();
   if (int r = grow(ctx, &ctx->d_mat, &ctx->mat_cap, nt)) return r;
   if (int r = grow(ctx, &ctx->d_inst, &ctx->inst_cap, ni)) return r;
+  if (nt > dem::kMaxTypes) return fail(ctx, DEM_ERR_INVALID, "at most 4096 material rows per context");
+  // the contact law's pair parameters for every ordered pair of materials
+  std::vector<float4> pair((size_t)nt * nt);
+  for (uint32_t a = 0; a < nt; ++a)
+    for (uint32_t b = 0; b < nt; ++b) {
+      const Material &A = ctx->h_mat[a], &B = ctx->h_mat[b];
+      pair[(size_t)a * nt + b] = make_float4(series_host(A.kn, B.kn), series_host(A.kt, B.kt),
+                                             series_host(A.cn, B.cn), (A.mu + B.mu) / 2);
+    }
+  if (int r = grow(ctx, &ctx->d_pair, &ctx->pair_cap, (uint32_t)pair.size())) return r;
+  DEM_HIP(ctx, hipMemcpyAsync(ctx->d_pair, pair.data(), sizeof(float4) * pair.size(), hipMemcpyHostToDevice,
+                              ctx->stream),
+          "upload material pairs");
+  ctx->c.pair = ctx->d_pair;
   DEM_HIP(ctx, hipMemcpyAsync(ctx->d_mat, ctx->h_mat.data(), sizeof(Material) * nt, hipMemcpyHostToDevice, ctx->stream),
           "upload materials");
   DEM_HIP(ctx, hipMemcpyAsync(ctx->d_inst, ctx->h_inst.data(), sizeof(Instance) * ni, hipMemcpyHostToDevice, ctx->stream),
@@ -207,6 +228,7 @@ int dem_destroy(dem_context* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->d_mat) (void)hipFree(ctx->d_mat);
+  if (ctx->d_pair) (void)hipFree(ctx->d_pair);
   if (ctx->d_inst) (void)hipFree(ctx->d_inst);
   if (ctx->d_copy_tmp) (void)hipFree(ctx->d_copy_tmp);
   if (ctx->d_control) (void)hipFree(ctx->d_control);

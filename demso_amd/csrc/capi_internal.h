@@ -15,6 +15,8 @@ struct dem_context {
   dem::StepConst c{};
   dem::Material* d_mat = nullptr;
   uint32_t mat_cap = 0;
+  float4* d_pair = nullptr;  // the contact law's per-pair parameters (StepConst::pair)
+  uint32_t pair_cap = 0;
   dem::Instance* d_inst = nullptr;
   uint32_t inst_cap = 0;
   std::vector<dem::Material> h_mat;

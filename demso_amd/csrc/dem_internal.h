@@ -42,8 +42,15 @@ struct StepConst {
   uint32_t pad;
   const Material* mat;
   const Instance* inst;
+  // Per ordered pair of materials (row tA * num_types + tB): the series
+  // combinations kn, kt, cn and the mean mu of collideSpheres
+  // (src/particles_kernel.cuh:262-268), computed once on the host with the
+  // same float operations (seriesAssociation, :36-41).
+  const float4* pair;
   Instance inst0;
 };
+// Material rows per context (the pair table holds up to kMaxTypes^2 entries).
+constexpr uint32_t kMaxTypes = DEM_MAX_TYPES;
 
 // The controlled particle's state (device) and the per-particle reaction
 // vectors the pair with it writes.

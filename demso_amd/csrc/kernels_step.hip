@@ -194,6 +194,7 @@ struct PairCtx {
   float w, rA, dt;
   double den;
   Material mA;
+  uint32_t tA;
 };
 
 // collideSpheres' contact law (src/particles_kernel.cuh:255-303) for a
@@ -203,16 +204,13 @@ struct PairCtx {
 __device__ __forceinline__ void contact_force(const StepConst& c, const PairCtx& q, float rx, float ry, float dist,
                                               float R, float rB, uint32_t tB, float2 vB, float wB, float& ffx,
                                               float& ffy, float& moment, float& pres) {
-  const float kBn = c.mat[tB].kn, kBt = c.mat[tB].kt, cBn = c.mat[tB].cn, muB = c.mat[tB].mu;
+  const float4 pr = c.pair[q.tA * c.num_types + tB];
+  const float kn = pr.x, kt = pr.y, cn = pr.z, mu = pr.w;
   const float nx = rx / dist, ny = ry / dist;
   const float tx = -ny, ty = nx;
   const float rvx = vB.x - q.v.x, rvy = vB.y - q.v.y;
   const float vn = rvx * nx + rvy * ny;
   const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
-  const float kn = series(q.mA.kn, kBn);
-  const float kt = series(q.mA.kt, kBt);
-  const float cn = series(q.mA.cn, cBn);
-  const float mu = (q.mA.mu + muB) / 2;
   const float s = -kn * (R - dist);
   ffx = s * nx;
   ffy = s * ny;
@@ -401,6 +399,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
                                                  float& moment_out, float& pres_out) {
   PairCtx q;
   own.t = clamp_idx(type[i], c.num_types);
+  q.tA = own.t;
   q.mA = c.mat[own.t];
   own.m = q.mA;
   own.in = kMulti ? c.inst[clamp_idx(q.mA.inst, c.num_instances)] : c.inst0;

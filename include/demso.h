@@ -32,7 +32,7 @@ extern "C" {
 
 #define DEMSO_ABI_VERSION 1
 #define DEM_EMPTY_CELL 0xFFFFFFFFu
-#define DEM_MAX_TYPES 65536u
+#define DEM_MAX_TYPES 4096u /* material rows per context (a table of per-pair contact parameters is kept) */
 
 typedef enum dem_status {
   DEM_OK = 0,
