@@ -270,7 +270,7 @@ struct Dyn {
   __device__ __forceinline__ uint32_t at(uint32_t j) const { return gat ? gat[j] : j; }
 };
 
-template <bool kRadius>
+template <bool kRadius, bool kOne>
 __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, bool self, float2 pB,
                                              float rB, uint32_t tB, const Dyn& d, const uint32_t* __restrict__ type,
                                              float& cfx, float& cfy, float& moment, float& pres) {
@@ -281,7 +281,7 @@ __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& 
   // the particle itself is skipped (collideCell's j != index test)
   if (self || !(maybe || c.attract)) return;
   const float dist = sqrtf(d2);
-  if (kRadius) tB = clamp_idx(type[j], c.num_types);
+  if (kRadius && !kOne) tB = clamp_idx(type[j], c.num_types);
   if (dist < R) {
     float ffx, ffy;
     const uint32_t jv = d.at(j);
@@ -331,7 +331,8 @@ __device__ __forceinline__ void control_pair(const StepConst& c, const PairCtx& 
 
 // Candidate j's position and radius (and, without a radius array, its
 // material row), as the pair law reads them.
-template <bool kRadius>
+// (kOne: every particle has material row 0, so no type is read.)
+template <bool kRadius, bool kOne>
 __device__ __forceinline__ void candidate_load(const StepConst& c, uint32_t j, const float2* __restrict__ pos,
                                                const uint32_t* __restrict__ type, const float* __restrict__ radius,
                                                float2& pB, float& rB, uint32_t& tB) {
@@ -340,12 +341,12 @@ __device__ __forceinline__ void candidate_load(const StepConst& c, uint32_t j, c
     rB = radius[j];
     tB = 0;
   } else {
-    tB = clamp_idx(type[j], c.num_types);
+    tB = kOne ? 0u : clamp_idx(type[j], c.num_types);
     rB = c.mat[tB].radius;
   }
 }
 
-template <bool kRadius>
+template <bool kRadius, bool kOne>
 __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
                                           const float2* __restrict__ pos, const Dyn& d,
                                           const uint32_t* __restrict__ type, const float* __restrict__ radius,
@@ -353,8 +354,8 @@ __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, 
   float2 pB;
   float rB;
   uint32_t tB;
-  candidate_load<kRadius>(c, j, pos, type, radius, pB, rB, tB);
-  candidate_at<kRadius>(c, q, j, self, pB, rB, tB, d, type, cfx, cfy, moment, pres);
+  candidate_load<kRadius, kOne>(c, j, pos, type, radius, pB, rB, tB);
+  candidate_at<kRadius, kOne>(c, q, j, self, pB, rB, tB, d, type, cfx, cfy, moment, pres);
 }
 
 // At the first candidate of a new cell, the previous cell's sum joins the
@@ -389,7 +390,11 @@ struct Own {
 constexpr int kPlain = 0, kPrefetch = 1, kBatch = 2;
 constexpr int kBatchN = 3;
 
-template <bool kRadius, bool kMulti, int kMode, bool kControl>
+// Material layouts of the kernels: one material row (no type is read), one
+// instance with several rows, or an ensemble of instances.
+constexpr int kOneMat = 0, kManyMat = 1, kInstances = 2;
+
+template <bool kRadius, int kMat, int kMode, bool kControl>
 __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i, const float2* __restrict__ pos,
                                                  const Dyn& d, const uint32_t* __restrict__ type,
                                                  const float* __restrict__ radius,
@@ -397,8 +402,9 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
                                                  const uint32_t* __restrict__ cell_end, uint32_t n, Own& own,
                                                  const ControlOut& ctl, float& fx_out, float& fy_out,
                                                  float& moment_out, float& pres_out) {
+  constexpr bool kMulti = kMat == kInstances, kOne = kMat == kOneMat;
   PairCtx q;
-  own.t = clamp_idx(type[i], c.num_types);
+  own.t = kOne ? 0u : clamp_idx(type[i], c.num_types);
   q.tA = own.t;
   q.mA = c.mat[own.t];
   own.m = q.mA;
@@ -479,7 +485,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
 #pragma unroll
         for (int k = 0; k < kBatchN; ++k) {
           const uint32_t j = rs[y] + (uint32_t)k;
-          candidate_load<kRadius>(c, j < re[y] ? j : 0u, pos, type, radius, pb[y][k], rb[y][k], tb[y][k]);
+          candidate_load<kRadius, kOne>(c, j < re[y] ? j : 0u, pos, type, radius, pb[y][k], rb[y][k], tb[y][k]);
         }
     }
 #pragma unroll
@@ -493,14 +499,14 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
         float2 pn = make_float2(0.0f, 0.0f);
         float rn = 0.0f;
         uint32_t tn = 0;
-        if (j < end) candidate_load<kRadius>(c, j, pos, type, radius, pn, rn, tn);
+        if (j < end) candidate_load<kRadius, kOne>(c, j, pos, type, radius, pn, rn, tn);
         for (; j < end; ++j) {
           const float2 pB = pn;
           const float rB = rn;
           const uint32_t tB = tn;
-          if (j + 1u < end) candidate_load<kRadius>(c, j + 1u, pos, type, radius, pn, rn, tn);
+          if (j + 1u < end) candidate_load<kRadius, kOne>(c, j + 1u, pos, type, radius, pn, rn, tn);
           cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
-          candidate_at<kRadius>(c, q, j, j == i, pB, rB, tB, d, type, cfx, cfy, moment, pres);
+          candidate_at<kRadius, kOne>(c, q, j, j == i, pB, rB, tB, d, type, cfx, cfy, moment, pres);
         }
       } else {
         uint32_t j = rs[y];
@@ -509,13 +515,13 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
           for (int k = 0; k < kBatchN; ++k, ++j)
             if (j < end) {
               cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
-              candidate_at<kRadius>(c, q, j, j == i, pb[y][k], rb[y][k], tb[y][k], d, type, cfx, cfy, moment,
+              candidate_at<kRadius, kOne>(c, q, j, j == i, pb[y][k], rb[y][k], tb[y][k], d, type, cfx, cfy, moment,
                                     pres);
             }
         }
         for (; j < end; ++j) {
           cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
-          candidate<kRadius>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
+          candidate<kRadius, kOne>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
         }
       }
       fx += cfx;
@@ -534,7 +540,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
         const uint32_t en = min(cell_end[row + xs[sg]], n);
         float cfx = 0.0f, cfy = 0.0f;
         for (uint32_t j = st; j < en; ++j)
-          candidate<kRadius>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
+          candidate<kRadius, kOne>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
         fx += cfx;
         fy += cfy;
       }
@@ -553,7 +559,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   pres_out = pres;
 }
 
-template <bool kRadius, bool kMulti, int kMode, bool kControl>
+template <bool kRadius, int kMat, int kMode, bool kControl>
 __global__ __launch_bounds__(256) void k_collide(
     StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,
     float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha,
@@ -565,7 +571,7 @@ __global__ __launch_bounds__(256) void k_collide(
   Own own;
   float fx, fy, moment, pres;
   const Dyn d{vel, omega, gat};
-  collide_particle<kRadius, kMulti, kMode, kControl>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
+  collide_particle<kRadius, kMat, kMode, kControl>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
                                                          ctl, fx, fy, moment, pres);
   acc[i] = make_float2(fx / own.mass, fy / own.mass);
   alpha[i] = moment / own.inertia;
@@ -688,7 +694,7 @@ __device__ __forceinline__ void next_key(const StepConst& c, const NextKeys& nk,
 // the same (in place, as the reference) or different (the owned
 // simulation writes the integrated state back to its home buffer); each
 // lane reads and writes only its own slot.
-template <bool kRadius, bool kMulti, bool kNext>
+template <bool kRadius, int kMat, bool kNext>
 __global__ __launch_bounds__(256) void k_integrate(
     StepConst c, const float2* pos_in, const float2* vel_in, const float* theta_in, const float* omega_in,
     float2* pos_out, float2* vel_out, float* theta_out, float* omega_out, const float2* __restrict__ acc,
@@ -696,7 +702,8 @@ __global__ __launch_bounds__(256) void k_integrate(
     float* __restrict__ pressure, NextKeys nk, const uint32_t* __restrict__ gat, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t t = clamp_idx(type[i], c.num_types);
+  constexpr bool kMulti = kMat == kInstances;
+  const uint32_t t = kMat == kOneMat ? 0u : clamp_idx(type[i], c.num_types);
   const Material m = c.mat[t];
   const Instance in = kMulti ? c.inst[clamp_idx(m.inst, c.num_instances)] : c.inst0;
   float r, mass, inertia;
@@ -722,7 +729,7 @@ __global__ __launch_bounds__(256) void k_integrate(
 // *_out arrays (the unsorted buffer of this step, dead after reorder), so no
 // lane overwrites a neighbour's input.  The arithmetic is that of k_collide
 // followed by k_integrate.
-template <bool kRadius, bool kMulti, int kMode, bool kControl>
+template <bool kRadius, int kMat, int kMode, bool kControl>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_collide_integrate(
     StepConst c, const float2* __restrict__ pos_in, const float2* __restrict__ vel_in,
     const float* __restrict__ omega_in, const float* __restrict__ theta_in, const uint32_t* __restrict__ type,
@@ -735,7 +742,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   Own own;
   float fx, fy, moment, pres;
   const Dyn d{vel_in, omega_in, nullptr};
-  collide_particle<kRadius, kMulti, kMode, kControl>(c, i, pos_in, d, type, radius, cell_start, cell_end, n, own,
+  collide_particle<kRadius, kMat, kMode, kControl>(c, i, pos_in, d, type, radius, cell_start, cell_end, n, own,
                                                          ctl, fx, fy, moment, pres);
   const float2 a = make_float2(fx / own.mass, fy / own.mass);
   const float al = moment / own.inertia;
@@ -749,7 +756,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   vel_out[i] = v;
   omega_out[i] = w;
   theta_out[i] = th;
-  next_key<kMulti>(c, nk, p, type, i, n);
+  next_key<kMat == kInstances>(c, nk, p, type, i, n);
 }
 
 inline dim3 grid_for(uint32_t n, int block) { return dim3((n + (uint32_t)block - 1u) / (uint32_t)block); }
@@ -795,16 +802,22 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
   return hipGetLastError();
 }
 
-// Kernel variant dispatch: kRadius x kMulti x (candidate mode, control),
-// where the controlled particle always uses the plain mode.
+// Kernel variant dispatch: kRadius x material layout x (candidate mode,
+// control), where the controlled particle always uses the plain mode.
+inline int mat_kind(const StepConst& c) {
+  return c.num_instances > 1 ? kInstances : (c.num_types == 1 ? kOneMat : kManyMat);
+}
 #define DEM_DISPATCH_RM(MACRO, MODE, X)                                  \
   do {                                                                   \
+    const int mk_ = mat_kind(c);                                         \
     if (radius) {                                                        \
-      if (multi) MACRO(true, true, MODE, X);                             \
-      else MACRO(true, false, MODE, X);                                  \
+      if (mk_ == kInstances) MACRO(true, kInstances, MODE, X);           \
+      else if (mk_ == kOneMat) MACRO(true, kOneMat, MODE, X);            \
+      else MACRO(true, kManyMat, MODE, X);                               \
     } else {                                                             \
-      if (multi) MACRO(false, true, MODE, X);                            \
-      else MACRO(false, false, MODE, X);                                 \
+      if (mk_ == kInstances) MACRO(false, kInstances, MODE, X);          \
+      else if (mk_ == kOneMat) MACRO(false, kOneMat, MODE, X);           \
+      else MACRO(false, kManyMat, MODE, X);                              \
     }                                                                    \
   } while (0)
 #define DEM_DISPATCH(MACRO)                                              \
@@ -821,7 +834,6 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
                           const uint32_t* cell_end, float* pressure, uint32_t n, int mode,
                           const ControlOut* ctl, const uint32_t* gat) {
   if (n == 0) return hipSuccess;
-  const bool multi = c.num_instances > 1;
   const ControlOut co = ctl ? *ctl : ControlOut{};
 #define DEM_COLLIDE(R, M, P, X)                                                                        \
   hipLaunchKernelGGL((k_collide<R, M, P, X>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
@@ -839,7 +851,6 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
                                     const NextKeys& next, uint32_t n, int mode, const ControlOut* ctl) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
-  const bool multi = c.num_instances > 1;
   const ControlOut co = ctl ? *ctl : ControlOut{};
 #define DEM_CI(R, M, P, X)                                                                                      \
   hipLaunchKernelGGL((k_collide_integrate<R, M, P, X>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
@@ -933,7 +944,6 @@ hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, con
                                uint32_t n, const NextKeys* next, const uint32_t* gat) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
-  const bool multi = c.num_instances > 1;
   const NextKeys nk = next ? *next : NextKeys{};
 #define DEM_INTEGRATE(R, M, X)                                                                          \
   hipLaunchKernelGGL((k_integrate<R, M, X>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
@@ -944,10 +954,15 @@ hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, con
     if (next) DEM_INTEGRATE(R, M, true); \
     else DEM_INTEGRATE(R, M, false);     \
   } while (0)
+  const int mk = mat_kind(c);
   if (radius) {
-    if (multi) DEM_INTEGRATE_X(true, true); else DEM_INTEGRATE_X(true, false);
+    if (mk == kInstances) DEM_INTEGRATE_X(true, kInstances);
+    else if (mk == kOneMat) DEM_INTEGRATE_X(true, kOneMat);
+    else DEM_INTEGRATE_X(true, kManyMat);
   } else {
-    if (multi) DEM_INTEGRATE_X(false, true); else DEM_INTEGRATE_X(false, false);
+    if (mk == kInstances) DEM_INTEGRATE_X(false, kInstances);
+    else if (mk == kOneMat) DEM_INTEGRATE_X(false, kOneMat);
+    else DEM_INTEGRATE_X(false, kManyMat);
   }
 #undef DEM_INTEGRATE_X
 #undef DEM_INTEGRATE

@@ -37,6 +37,26 @@ def compressed_pack(cols=32, rows=32, seed=7, spin=True) -> sc.Scene:
     return s
 
 
+def mixed_pack(cols=32, rows=32, seed=11) -> sc.Scene:
+    """compressed_pack with three materials of different stiffness, damping,
+    friction and density assigned at random (per-particle radii kept), so the
+    kernels read each particle's material row and every ordered pair of
+    materials meets in contact."""
+    import dataclasses
+
+    s = compressed_pack(cols=cols, rows=rows, seed=seed)
+    base = s.materials[0]
+    s.materials = [
+        base,
+        dataclasses.replace(base, kn=base.kn * 3.0, kt=base.kt * 2.5, cn=base.cn * 0.5, mu=0.2, bdamp=0.3,
+                            areal=base.areal * 1.7),
+        dataclasses.replace(base, kn=base.kn * 0.4, kt=base.kt * 0.7, cn=base.cn * 1.5, mu=0.0, bdamp=0.8,
+                            areal=base.areal * 0.6),
+    ]
+    s.type = np.random.default_rng(seed).integers(0, 3, s.n).astype(np.uint32)
+    return s
+
+
 def state_dict(s: sc.Scene) -> dict:
     return dict(pos=s.pos, vel=s.vel, theta=s.theta, omega=s.omega, id=s.id, type=s.type, loc=s.loc,
                 radius=s.radius)

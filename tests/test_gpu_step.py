@@ -7,7 +7,7 @@ import pytest
 
 from demso_amd import api
 from demso_amd import scene as sc
-from helpers import bits_equal, compressed_pack, rel_err
+from helpers import mixed_pack, bits_equal, compressed_pack, rel_err
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
@@ -31,6 +31,7 @@ def compare(g, o, tol=TOL, exact=True):
 CASES = {
     "column_collapse_4096": (lambda: sc.column_collapse(), 40),
     "pack_1024": (lambda: compressed_pack(), 25),
+    "mixed_pack_1024": (lambda: mixed_pack(), 25),
     "pile_wrap_attract": (lambda: sc.triangle_pile(), 60),
     "ensemble_5": (lambda: sc.ensemble(instances=5, counts=[1500, 800, 2100, 90, 640], seed=4), 30),
 }
