@@ -81,8 +81,8 @@ struct ResortScratch {
   uint32_t* host_count = nullptr;   // device address of a host-mapped mirror of the mover count (or null)
   uint32_t capacity = 0;
 };
-// Stable sort of `key` (the keys of a state whose order is sorted by
-// prev_key) from the mover bits and counts written with it (NextKeys):
+// Stable sort of the new keys of a state whose order is sorted by prev_key,
+// from the mover bits, counts and mover keys written with them (NextKeys):
 // writes the sorted keys and the source index of every sorted slot.  Equal
 // to a stable sort of key by position for any number of movers; its cost
 // grows with the number of movers.  With clear, the placement kernel also
@@ -141,10 +141,11 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
                           const uint32_t* cell_end, float* pressure, uint32_t n, int mode = 0,
                           const struct ControlOut* ctl = nullptr, const uint32_t* gat = nullptr);
-// The next step's hash, fused into integrate: key[i] = cell key of the new
-// position, bit i of `bits` set when it differs from prev_key[i] (the keys
-// the state is sorted by), tile_count[i / kResortTile] += movers (the
-// counts must be zero before).
+// The next step's hash, fused into integrate: bit i of `bits` set when the
+// cell key of the new position differs from prev_key[i] (the keys the state
+// is sorted by), and then key[i] = that key (key[i] is not written for the
+// others: their key is prev_key[i]); tile_count[i / kResortTile] += movers
+// (the counts must be zero before).
 struct NextKeys {
   const uint32_t* prev_key = nullptr;
   uint32_t* key = nullptr;

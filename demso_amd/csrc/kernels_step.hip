@@ -679,8 +679,9 @@ __device__ __forceinline__ void next_key(const StepConst& c, const NextKeys& nk,
   // the new cell key, and whether it left the cell of its sorted slot (the
   // state stays in this order until the next sort)
   const uint32_t k = cell_key<kMulti>(c, p, type, i);
-  nk.key[i] = k;
-  const uint64_t moved = __ballot(k != nk.prev_key[i]);
+  const bool mover = k != nk.prev_key[i];
+  if (mover) nk.key[i] = k;  // a stayer's key is prev_key[i]
+  const uint64_t moved = __ballot(mover);
   if (__lane_id() == 0) {  // lane 0 holds the wave's first index, a multiple of 64
     const uint32_t w0 = i / 32u, nwords = (n + 31u) / 32u;
     nk.bits[w0] = (uint32_t)moved;

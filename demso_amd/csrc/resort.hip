@@ -16,9 +16,9 @@
 // written exactly once, and the result equals the full radix sort for any
 // number of movers; the cost grows with the number of movers.
 //
-// The keys, the mover bit mask (bit i%32 of word i/32) and the movers per
-// tile of kResortTile particles come from the previous step's integrate,
-// which hashes the positions it has just written.  Launches:
+// The movers' keys, the mover bit mask (bit i%32 of word i/32) and the
+// movers per tile of kResortTile particles come from the previous step's
+// integrate, which hashes the positions it has just written.  Launches:
 //   compact : per tile, movers before each 32-particle word and the
 //             compacted (key, position) list of the movers;
 //   movers  : one workgroup sorts the movers: a bitonic sort of the unique
@@ -296,7 +296,7 @@ __device__ __forceinline__ uint32_t insertion_point(uint32_t k, uint32_t p, cons
 }
 
 __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
-    const uint32_t* __restrict__ key, const uint32_t* __restrict__ prev_key, const uint32_t* __restrict__ bits,
+    const uint32_t* __restrict__ prev_key, const uint32_t* __restrict__ bits,
     const uint32_t* __restrict__ word_prefix, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
     const uint32_t* mpos1, const uint32_t* __restrict__ info, const uint32_t* __restrict__ prev_cell_start,
     const uint32_t* __restrict__ prev_cell_end, uint32_t num_cells, int cells_valid, uint32_t* __restrict__ out_key,
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
   for (uint32_t it = 0; it < kPlaceItems; ++it) {
     const uint32_t i = first + it * kPlaceThreads + tid;
     const bool valid = i <= last;
-    kk[it] = valid ? key[i] : 0u;
+    kk[it] = valid ? prev_key[i] : 0u;  // a stayer's key (movers are skipped below)
     bw[it] = valid ? bits[i >> 5] : 0xFFFFFFFFu;
     wp[it] = valid ? word_prefix[i >> 5] : 0u;
   }
@@ -436,7 +436,7 @@ __device__ __forceinline__ void stage(const ShiftScratch& sh, const ShiftState& 
 // sorting below it equal the movers before it.
 template <bool kRadius>
 __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
-    const uint32_t* __restrict__ key, ShiftState st, ShiftScratch sh, const uint32_t* __restrict__ bits,
+    ShiftState st, ShiftScratch sh, const uint32_t* __restrict__ bits,
     const uint32_t* __restrict__ word_prefix, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
     const uint32_t* mpos1, const uint32_t* __restrict__ info, uint32_t num_cells, uint32_t n) {
   __shared__ uint64_t win[kWindowMax];
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
   for (uint32_t it = 0; it < kPlaceItems; ++it) {
     const uint32_t i = first + it * kPlaceThreads + tid;
     const bool valid = i <= last;
-    kk[it] = valid ? key[i] : 0u;
+    kk[it] = valid ? prev_key[i] : 0u;  // a stayer's key (movers are skipped below)
     bw[it] = valid ? bits[i >> 5] : 0xFFFFFFFFu;
     wp[it] = valid ? word_prefix[i >> 5] : 0u;
   }
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(256) void k_shift_index(const uint32_t* __restrict_
 
 // Workgroups of the list kernels (grid-stride over a count known only on
 // the device): enough to cover a large list in a few passes.
-uint32_t list_blocks(uint32_t n) { return std::max(1u, std::min(512u, (n + 2047u) / 2048u)); }
+uint32_t list_blocks(uint32_t n) { return std::max(1u, std::min(128u, (n + 2047u) / 2048u)); }
 
 }  // namespace
 
@@ -669,10 +669,10 @@ hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const Sh
                      r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count, sh.count);
   const dim3 grid(kMoverBlocks + (n + kPlaceTile - 1u) / kPlaceTile);
   if (st.radius)
-    hipLaunchKernelGGL(k_shift_place<true>, grid, dim3(kPlaceThreads), 0, s, key, st, sh, r.bits, r.word_prefix,
+    hipLaunchKernelGGL(k_shift_place<true>, grid, dim3(kPlaceThreads), 0, s, st, sh, r.bits, r.word_prefix,
                        r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, num_cells, n);
   else
-    hipLaunchKernelGGL(k_shift_place<false>, grid, dim3(kPlaceThreads), 0, s, key, st, sh, r.bits, r.word_prefix,
+    hipLaunchKernelGGL(k_shift_place<false>, grid, dim3(kPlaceThreads), 0, s, st, sh, r.bits, r.word_prefix,
                        r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, num_cells, n);
   return hipGetLastError();
 }
@@ -714,7 +714,7 @@ hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* pre
   hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.mkey[0], r.mpos[0],
                      r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count, nullptr);
   const uint32_t place_tiles = (n + kPlaceTile - 1u) / kPlaceTile;
-  hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + place_tiles), dim3(kPlaceThreads), 0, s, key, prev_key, r.bits,
+  hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + place_tiles), dim3(kPlaceThreads), 0, s, prev_key, r.bits,
                      r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, prev_cell_start,
                      prev_cell_end, num_cells, cells_valid ? 1 : 0, out_key, out_idx, clear, clear_n, n);
   return hipGetLastError();
