@@ -32,43 +32,56 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain step"
 
 # Algorithmic (compulsory) HBM bytes per particle and per cell for each
-# stage of the owned simulation's step, FP32 state with per-particle radius
-# and 32-bit keys:
-#   hash      radix-sorted steps only (re-sorted steps get their keys from
-#             the previous collide): R pos 8; W key 4, index 4
-#   sort      re-sort: R key 4;                W sorted key 4, index 4
-#             (a radix pass would be R key+index 8, W key+index 8)
-#   fill      W cellStart 4 per cell: on re-sorted steps the placement kernel
-#             clears the next step's cellStart, so these bytes count under sort
-#   reorder   R key 4, index 4, pos 8, id 4, type 4, radius 4;
-#             W pos 8, id 4, loc 4, type 4, radius 4
-#             (velocity, angle and spin stay where they are: collide and
-#             integrate read them through the sorted index)
-#   collide   R pos 8, radius 4, type 4, index 4, vel 8, omega 4;
-#             W acc 8, alpha 4, pressure 4 + R cellStart/cellEnd 8 per cell
-#   integrate (with the next step's hash, into the next state buffer)
-#             R pos 8, index 4, vel 8, theta 4, omega 4, acc 8, alpha 4, type 4,
-#             radius 4, pressure 4, previous key 4; W pos 8, vel 8, theta 4,
-#             omega 4, key 4
-#   When the autotuner fuses collide and integrate into one kernel, reorder
-#   sorts every field (R key 4, index 4, state 36; W state 36, loc 4: 84) and
-#   the fused kernel's bytes are R pos 8, vel 8, omega 4, theta 4, radius 4,
-#   type 4, previous key 4; W acc 8, alpha 4, pressure 4, pos 8, vel 8,
-#   theta 4, omega 4, key 4 (80) + 8 per cell, reported under collide.
-BYTES_PER_PARTICLE = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 52, "collide": 48, "integrate": 84}
-BYTES_PER_CELL = {"hash": 0, "sort": 4, "cell_fill": 0, "reorder": 0, "collide": 8, "integrate": 0}
-FUSED_COLLIDE_BYTES = 80
-FULL_REORDER_BYTES = 84
+# stage of the owned simulation's re-sorted step, FP32 state and 32-bit keys.
+# `radius`: a per-particle radius array (4 B where noted); `typed`: several
+# material rows, so the kernels read each particle's type (4 B where noted).
+#
+# In place (the default split layout; the state stays in its buffer):
+#   sort      R mover bits, W movers-before-each-word (1/8 + 1/8 per particle),
+#             plus 48 B per particle whose slot changes (not counted: a
+#             handful per step on the settling workload)
+#   reorder   only the particles whose slot changes (not counted)
+#   collide   R pos 8, radius 4, type 4, vel 8, omega 4; W acc 8, alpha 4,
+#             pressure 4; + R cellStart/cellEnd 8 per cell
+#   integrate R pos 8, vel 8, theta 4, omega 4, acc 8, alpha 4, radius 4,
+#             type 4, previous key 4; W pos 8, vel 8, theta 4, omega 4 (the
+#             next key only for the few particles that change cell)
+# Copy (DEM_STEP_COPY): re-sort R previous key 4, W sorted key 4, index 4 and
+#   the next cellStart 4 per cell; reorder R key 4, index 4, pos 8, id 4,
+#   type 4, radius 4, W pos 8, id 4, loc 4, type 4, radius 4; collide and
+#   integrate also read the sorted index (4 each) and velocity, spin and
+#   angle through it.
+# Fused (collide + integrate in one kernel): reorder sorts every field (R key
+#   4, index 4, pos 8, vel 8, theta 4, omega 4, id 4, type 4, radius 4; W the
+#   same state and loc 4) and the fused kernel reads pos 8, vel 8, omega 4,
+#   theta 4, radius 4, type 4, previous key 4 and writes acc 8, alpha 4,
+#   pressure 4, pos 8, vel 8, theta 4, omega 4 (+ 8 per cell), reported
+#   under collide.
+STAGES = ("hash", "sort", "cell_fill", "reorder", "collide", "integrate")
 
 
-def algorithmic_bytes(stage: str, n: int, cells: int, fused: bool = False) -> int:
-    if fused and stage == "reorder":
-        return FULL_REORDER_BYTES * n
-    if fused and stage == "collide":
-        return FUSED_COLLIDE_BYTES * n + BYTES_PER_CELL[stage] * cells
-    if fused and stage == "integrate":
-        return 0
-    return BYTES_PER_PARTICLE[stage] * n + BYTES_PER_CELL[stage] * cells
+def stage_bytes(layout: str, radius: bool, typed: bool) -> tuple[dict, dict]:
+    """Per-particle and per-cell algorithmic bytes of each stage."""
+    r = 4 if radius else 0
+    t = 4 if typed else 0
+    per_cell = {"hash": 0, "sort": 0, "cell_fill": 0, "reorder": 0, "collide": 8, "integrate": 0}
+    if layout == "fused":
+        pp = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 76 + 2 * r, "collide": 68 + r + t, "integrate": 0}
+        per_cell["sort"] = 4
+    elif layout == "copy":
+        pp = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 44 + 2 * r,
+              "collide": 4 + 8 + r + t + 12 + 16, "integrate": 4 + 8 + 16 + 12 + r + t + 4 + 24}
+        per_cell["sort"] = 4
+    else:
+        pp = {"hash": 0, "sort": 0.25, "cell_fill": 0, "reorder": 0,
+              "collide": 8 + r + t + 12 + 16, "integrate": 8 + 16 + 12 + r + t + 4 + 24}
+    return pp, per_cell
+
+
+def algorithmic_bytes(stage: str, n: int, cells: int, layout: str = "inplace", radius: bool = True,
+                      typed: bool = False) -> int:
+    pp, pc = stage_bytes(layout, radius, typed)
+    return int(pp[stage] * n + pc[stage] * cells)
 
 
 def make_scene(name: str):
@@ -241,13 +254,14 @@ def main() -> int:
     # per-kernel durations with HIP events on the launch stream; the sort is
     # a multi-kernel stage (3 kernels per radix pass) and is reported apart
     stages = sim.stage_times(max(5, min(args.steps, 50)))
-    fused = bool(tuned and tuned.get("fused"))
+    layout = "fused" if tuned and tuned.get("fused") else ("copy" if tuned and tuned.get("copy") else "inplace")
+    acct = dict(layout=layout, radius=scene.radius is not None, typed=len(scene.materials) > 1)
     kernel_stages = {k: v for k, v in stages.items()
-                     if k in ("cell_fill", "reorder", "collide", "integrate") and algorithmic_bytes(k, n, cells, fused) > 0}
+                     if k in ("cell_fill", "reorder", "collide", "integrate") and algorithmic_bytes(k, n, cells, **acct) > 0}
     dom = max(kernel_stages, key=kernel_stages.get)
-    dom_bytes = algorithmic_bytes(dom, n, cells, fused)
+    dom_bytes = algorithmic_bytes(dom, n, cells, **acct)
     achieved = dom_bytes / (kernel_stages[dom] * 1e-3) / 1e9
-    step_bytes = sum(algorithmic_bytes(k, n, cells, fused) for k in BYTES_PER_PARTICLE)
+    step_bytes = sum(algorithmic_bytes(k, n, cells, **acct) for k in STAGES)
 
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{args.workload}.json"
@@ -300,6 +314,7 @@ def main() -> int:
             "algorithmic_bytes": dom_bytes,
         },
         "step_roofline": {
+            "layout": layout,
             "algorithmic_bytes_per_step": step_bytes,
             "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
             "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
