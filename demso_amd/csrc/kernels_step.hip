@@ -270,22 +270,71 @@ struct Dyn {
   __device__ __forceinline__ uint32_t at(uint32_t j) const { return gat ? gat[j] : j; }
 };
 
-template <bool kRadius, bool kOne>
-__device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, bool self, float2 pB,
-                                             float rB, uint32_t tB, const Dyn& d, const uint32_t* __restrict__ type,
-                                             float& cfx, float& cfy, float& moment, float& pres) {
-  const float rx = pB.x - q.p.x, ry = pB.y - q.p.y;
+// A neighbour candidate as the pair law reads it.  With kDyn its velocity,
+// spin and material row are loaded together with its position and radius
+// (one dependent load fewer for every contact, three more registers);
+// without, they are loaded only for candidates that touch.
+struct Cand {
+  float2 p;
+  float r;
+  uint32_t t;
+  float2 v;
+  float w;
+};
+
+// The row's per-cell partial sum: collideCell's sums start at +0 for every
+// cell and collideD adds them in cell order.  Candidates that exert no force
+// add +0, which leaves a partial sum unchanged (a sum that starts at +0 is
+// never -0), so the partial sum is flushed into the total only where a
+// candidate that may exert a force lies in a later cell of the row than the
+// sum's (cell index (j >= b0) + (j >= b1), the row's cell ends), and at the
+// row's end: the same additions in the same order as flushing at every cell
+// end.
+struct RowSum {
+  float cfx = 0.0f, cfy = 0.0f;
+  uint32_t cell = 0, b0, b1;
+  __device__ __forceinline__ void at(uint32_t j, float& fx, float& fy) {
+    const uint32_t k = (uint32_t)(j >= b0) + (uint32_t)(j >= b1);
+    if (k != cell) {
+      fx += cfx;
+      fy += cfy;
+      cfx = 0.0f;
+      cfy = 0.0f;
+      cell = k;
+    }
+  }
+};
+
+template <bool kRadius, bool kOne, bool kDyn>
+__device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
+                                             const Cand& b, const Dyn& d, const uint32_t* __restrict__ type,
+                                             RowSum& rsum, float& fx, float& fy, float& moment, float& pres) {
+  const float rx = b.p.x - q.p.x, ry = b.p.y - q.p.y;
   const float d2 = rx * rx + ry * ry;
-  const float R = q.rA + rB;
+  const float R = q.rA + b.r;
   const bool maybe = !(d2 > (R * R) * 1.00000095367431640625f) || R < 1.0842022e-19f;
   // the particle itself is skipped (collideCell's j != index test)
   if (self || !(maybe || c.attract)) return;
+  rsum.at(j, fx, fy);
+  float& cfx = rsum.cfx;
+  float& cfy = rsum.cfy;
   const float dist = sqrtf(d2);
-  if (kRadius && !kOne) tB = clamp_idx(type[j], c.num_types);
+  const float rB = b.r;
+  uint32_t tB = b.t;
+  if (kRadius && !kOne && !kDyn) tB = clamp_idx(type[j], c.num_types);
   if (dist < R) {
     float ffx, ffy;
-    const uint32_t jv = d.at(j);
-    contact_force(c, q, rx, ry, dist, R, rB, tB, d.vel[jv], d.omega[jv], ffx, ffy, moment, pres);
+    float2 vB;
+    float wB;
+    if (kDyn) {
+      vB = b.v;
+      wB = b.w;
+    } else {
+      const uint32_t jv = d.at(j);
+      vB = d.vel[jv];
+      wB = d.omega[jv];
+    }
+    contact_force(c, q, rx, ry, dist, R, rB, tB, vB, wB, ffx, ffy, moment, pres);
     cfx += ffx;
     cfy += ffy;
   }
@@ -330,43 +379,35 @@ __device__ __forceinline__ void control_pair(const StepConst& c, const PairCtx& 
 }
 
 // Candidate j's position and radius (and, without a radius array, its
-// material row), as the pair law reads them.
+// material row), plus with kDyn its velocity, spin and material row.
 // (kOne: every particle has material row 0, so no type is read.)
-template <bool kRadius, bool kOne>
+template <bool kRadius, bool kOne, bool kDyn>
 __device__ __forceinline__ void candidate_load(const StepConst& c, uint32_t j, const float2* __restrict__ pos,
                                                const uint32_t* __restrict__ type, const float* __restrict__ radius,
-                                               float2& pB, float& rB, uint32_t& tB) {
-  pB = pos[j];
+                                               const Dyn& d, Cand& b) {
+  b.p = pos[j];
   if (kRadius) {
-    rB = radius[j];
-    tB = 0;
+    b.r = radius[j];
+    b.t = kDyn && !kOne ? clamp_idx(type[j], c.num_types) : 0u;
   } else {
-    tB = kOne ? 0u : clamp_idx(type[j], c.num_types);
-    rB = c.mat[tB].radius;
+    b.t = kOne ? 0u : clamp_idx(type[j], c.num_types);
+    b.r = c.mat[b.t].radius;
+  }
+  if (kDyn) {
+    const uint32_t jv = d.at(j);
+    b.v = d.vel[jv];
+    b.w = d.omega[jv];
   }
 }
 
-template <bool kRadius, bool kOne>
+template <bool kRadius, bool kOne, bool kDyn>
 __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
                                           const float2* __restrict__ pos, const Dyn& d,
                                           const uint32_t* __restrict__ type, const float* __restrict__ radius,
-                                          float& cfx, float& cfy, float& moment, float& pres) {
-  float2 pB;
-  float rB;
-  uint32_t tB;
-  candidate_load<kRadius, kOne>(c, j, pos, type, radius, pB, rB, tB);
-  candidate_at<kRadius, kOne>(c, q, j, self, pB, rB, tB, d, type, cfx, cfy, moment, pres);
-}
-
-// At the first candidate of a new cell, the previous cell's sum joins the
-// particle's total (force += collideCell(...), one cell at a time).  Written
-// without a branch: adding +0 leaves a total unchanged, as a total that
-// starts at +0 is never -0.
-__device__ __forceinline__ void cell_boundary(bool at, float& fx, float& fy, float& cfx, float& cfy) {
-  fx += at ? cfx : 0.0f;
-  fy += at ? cfy : 0.0f;
-  cfx = at ? 0.0f : cfx;
-  cfy = at ? 0.0f : cfy;
+                                          RowSum& rsum, float& fx, float& fy, float& moment, float& pres) {
+  Cand b;
+  candidate_load<kRadius, kOne, kDyn>(c, j, pos, type, radius, d, b);
+  candidate_at<kRadius, kOne, kDyn>(c, q, j, self, b, d, type, rsum, fx, fy, moment, pres);
 }
 
 // Forces on sorted particle i from its 3x3 cell neighbourhood (the body of
@@ -475,9 +516,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       b0[y] = n0 ? e0 : kNoBoundary;
       b1[y] = n1 ? e1 : kNoBoundary;
     }
-    float2 pb[3][kBatchN];
-    float rb[3][kBatchN];
-    uint32_t tb[3][kBatchN];
+    Cand cb[3][kBatchN];
     if (kMode == kBatch) {
       // index 0 stands in for a candidate past the row's end (never used)
 #pragma unroll
@@ -485,47 +524,40 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
 #pragma unroll
         for (int k = 0; k < kBatchN; ++k) {
           const uint32_t j = rs[y] + (uint32_t)k;
-          candidate_load<kRadius, kOne>(c, j < re[y] ? j : 0u, pos, type, radius, pb[y][k], rb[y][k], tb[y][k]);
+          candidate_load<kRadius, kOne, false>(c, j < re[y] ? j : 0u, pos, type, radius, d, cb[y][k]);
         }
     }
 #pragma unroll
     for (int y = 0; y < 3; ++y) {
-      float cfx = 0.0f, cfy = 0.0f;
+      RowSum rsum;
+      rsum.b0 = b0[y];
+      rsum.b1 = b1[y];
       const uint32_t end = re[y];
       if (kMode == kPrefetch) {
         // each candidate's position and radius fetched one iteration ahead
         // (pays off with many candidates per row, costs registers)
         uint32_t j = rs[y];
-        float2 pn = make_float2(0.0f, 0.0f);
-        float rn = 0.0f;
-        uint32_t tn = 0;
-        if (j < end) candidate_load<kRadius, kOne>(c, j, pos, type, radius, pn, rn, tn);
+        Cand nx{};
+        if (j < end) candidate_load<kRadius, kOne, false>(c, j, pos, type, radius, d, nx);
         for (; j < end; ++j) {
-          const float2 pB = pn;
-          const float rB = rn;
-          const uint32_t tB = tn;
-          if (j + 1u < end) candidate_load<kRadius, kOne>(c, j + 1u, pos, type, radius, pn, rn, tn);
-          cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
-          candidate_at<kRadius, kOne>(c, q, j, j == i, pB, rB, tB, d, type, cfx, cfy, moment, pres);
+          const Cand b = nx;
+          if (j + 1u < end) candidate_load<kRadius, kOne, false>(c, j + 1u, pos, type, radius, d, nx);
+          candidate_at<kRadius, kOne, false>(c, q, j, j == i, b, d, type, rsum, fx, fy, moment, pres);
         }
       } else {
         uint32_t j = rs[y];
         if (kMode == kBatch) {
 #pragma unroll
           for (int k = 0; k < kBatchN; ++k, ++j)
-            if (j < end) {
-              cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
-              candidate_at<kRadius, kOne>(c, q, j, j == i, pb[y][k], rb[y][k], tb[y][k], d, type, cfx, cfy, moment,
-                                    pres);
-            }
+            if (j < end)
+              candidate_at<kRadius, kOne, false>(c, q, j, j == i, cb[y][k], d, type, rsum, fx, fy, moment, pres);
         }
-        for (; j < end; ++j) {
-          cell_boundary(j == b0[y] || j == b1[y], fx, fy, cfx, cfy);
-          candidate<kRadius, kOne>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
-        }
+        for (; j < end; ++j)
+          candidate<kRadius, kOne, kMode == kPlain>(c, q, j, j == i, pos, d, type, radius, rsum, fx, fy, moment,
+                                                     pres);
       }
-      fx += cfx;
-      fy += cfy;
+      fx += rsum.cfx;
+      fy += rsum.cfy;
     }
   } else {
     // wrap mode at the grid's x edge: the three cells one by one, in the
@@ -538,11 +570,13 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
         const uint32_t st = cell_start[row + xs[sg]];
         if (st == kEmpty) continue;
         const uint32_t en = min(cell_end[row + xs[sg]], n);
-        float cfx = 0.0f, cfy = 0.0f;
+        RowSum rsum;  // one cell: no boundary inside
+        rsum.b0 = kNoBoundary;
+        rsum.b1 = kNoBoundary;
         for (uint32_t j = st; j < en; ++j)
-          candidate<kRadius, kOne>(c, q, j, j == i, pos, d, type, radius, cfx, cfy, moment, pres);
-        fx += cfx;
-        fy += cfy;
+          candidate<kRadius, kOne, false>(c, q, j, j == i, pos, d, type, radius, rsum, fx, fy, moment, pres);
+        fx += rsum.cfx;
+        fy += rsum.cfy;
       }
     }
   }
