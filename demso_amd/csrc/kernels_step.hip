@@ -168,15 +168,17 @@ __global__ __launch_bounds__(256) void k_reorder(
 // collideD (src/particles_kernel.cuh:459-549) with collideCell (:345-385),
 // collideSpheres (:235-307) and attractSpheres (:310-342) inlined.
 //
-// The kernel is VALU-bound (it runs ~10^3 vector instructions per wave), so
-// the candidate loop is kept lean:
+// The kernel's cost is per candidate (instructions and the latency of the
+// dependent loads), so the candidate loop is kept lean:
 //  * the three cells of a row are consecutive keys, so their candidates are
 //    one contiguous range of the sorted arrays; collideCell returns one
-//    partial float2 per cell that collideD adds up, so the partial sum is
-//    flushed when the walk crosses a cell end (flushing a zero sum changes
-//    nothing: the sums start at +0 and never become -0).  Where a wrapping
-//    grid makes the row non-contiguous (edge columns), the row's three cells
-//    are walked one by one in the reference order;
+//    partial float2 per cell that collideD adds up, so the running partial
+//    sum is flushed into the total when a candidate that may exert a force
+//    lies in a later cell (RowSum; flushing only there changes nothing, as
+//    candidates without force add +0 and the sums start at +0 and never
+//    become -0).  Where a wrapping grid makes the row non-contiguous (edge
+//    columns), the row's three cells are walked one by one in the reference
+//    order;
 //  * most candidates are not in contact: d2 = rx*rx+ry*ry is the float the
 //    reference takes sqrtf of, and if d2 > fl(R*R)*(1+2^-20) then
 //    sqrtf(d2) >= R for certain (fl(R*R)*(1+2^-20) exceeds the square of
@@ -185,8 +187,9 @@ __global__ __launch_bounds__(256) void k_reorder(
 //    sqrt and the contact law run only for candidates that may touch; the
 //    contact decision itself is the reference's dist < R;
 //  * the neighbour's velocity, spin and material are loaded only for actual
-//    contacts (the reference loads them for every candidate; the values used
-//    are the same).
+//    contacts, except in the plain mode, which loads them with the position
+//    (the reference loads them for every candidate; the values used are the
+//    same).
 constexpr uint32_t kNoBoundary = 0xFFFFFFFFu;
 
 struct PairCtx {
@@ -424,10 +427,11 @@ struct Own {
 };
 
 // Candidate modes of the collide kernels: kPlain walks each row's candidates
-// one load at a time; kPrefetch loads each one iteration ahead; kBatch loads
-// the first kBatchN candidates of all three rows at once before walking them
-// (more loads in flight, more registers).  The order of the sums is the same
-// in every mode.
+// one load at a time (each candidate's velocity and spin loaded with its
+// position); kPrefetch loads each one iteration ahead; kBatch loads the first
+// kBatchN candidates of all three rows at once before walking them (more
+// loads in flight, more registers).  The order of the sums is the same in
+// every mode.
 constexpr int kPlain = 0, kPrefetch = 1, kBatch = 2;
 constexpr int kBatchN = 3;
 
