@@ -13,7 +13,8 @@
 //   k_adj_collide   : reverse of the pair law (:235-307) over the 3x3
 //                     neighbourhood, one lane per particle gathering the
 //                     neighbours' force/moment adjoints (no atomics on state)
-//   k_adj_permute   : adjoint of the sort permutation (scatter)
+//   k_adj_permute   : adjoint of the sort permutation (scatter; in place at
+//                     the moved slots after an in-place re-sort)
 // Switches (contact, approach-only damping, stick/slip, wall hits) are
 // frozen at the forward's decisions.  In exact arithmetic the pair law is
 // F = s n + f_t t, tau = f_t with s = -kn (R - dist) + [vn<0] cn vn,
@@ -49,9 +50,9 @@ __device__ __forceinline__ void series_grad(double a, double b, double& da, doub
     da = db = 0.0;
     return;
   }
-  const double s = (a + b) * (a + b);
-  da = b * b / s;
-  db = a * a / s;
+  const double inv = 1.0 / ((a + b) * (a + b));
+  da = b * b * inv;
+  db = a * a * inv;
 }
 
 __device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
@@ -157,7 +158,7 @@ __global__ void k_store_slot(const float2* __restrict__ pos, const float2* __res
   somega[i] = omega[i];
   if (radius) sradius[i] = radius[i];
   stype[i] = type[i];
-  sindex[i] = index[i];
+  if (index) sindex[i] = index[i];
   skey[i] = key[i];
   sacc[i] = acc[i];
   salpha[i] = alpha[i];
@@ -187,6 +188,35 @@ __global__ void k_adj_permute(const uint32_t* __restrict__ index, const double2*
   lx[src] = lxp[i];
   lv[src] = lvp[i];
   lw[src] = lwp[i];
+}
+
+// The same for a step whose index is the identity except at a few slots (an
+// in-place re-sort), in place on the pre-step adjoint: the listed slots form
+// a permutation among themselves, so their values are first copied aside
+// (into the post-step arrays, no longer needed), then written to their source
+// slots; the caller then swaps the roles of the two arrays.
+__global__ void k_adj_perm_save(const uint32_t* __restrict__ index, const double2* __restrict__ lxp,
+                                const double2* __restrict__ lvp, const double* __restrict__ lwp,
+                                double2* __restrict__ tx, double2* __restrict__ tv, double* __restrict__ tw,
+                                uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || index[i] == i) return;
+  tx[i] = lxp[i];
+  tv[i] = lvp[i];
+  tw[i] = lwp[i];
+}
+
+__global__ void k_adj_perm_put(const uint32_t* __restrict__ index, const double2* __restrict__ tx,
+                               const double2* __restrict__ tv, const double* __restrict__ tw,
+                               double2* __restrict__ lxp, double2* __restrict__ lvp, double* __restrict__ lwp,
+                               uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t src = index[i];
+  if (src == i || src >= n) return;
+  lxp[src] = tx[i];
+  lvp[src] = tv[i];
+  lwp[src] = tw[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -309,7 +339,160 @@ __global__ __launch_bounds__(256) void k_adj_integrate(StepConst c, AdjIn S, con
 }
 
 // ---------------------------------------------------------------------------
-// Reverse of the pair law for particle i over its 3x3 neighbourhood.
+// Reverse of the pair law for one contact of particle A with candidate j:
+// accumulates the adjoints of A's position, velocity and spin and A's share
+// of the parameter gradients.  The contact, approach and stick decisions are
+// the forward's, replayed in FP32 with its operations (the exact distance
+// prefilter of the forward collide skips the candidates that cannot touch);
+// the derivatives are in double.
+struct AdjA {
+  float2 pA, vA;
+  float wA, rA;
+  uint32_t tA;
+  Material A;
+  double2 lFA;
+  double lMA;
+  double2 dx_acc, drv_acc;
+  double dw_acc;
+};
+
+template <bool kRadius>
+__device__ __forceinline__ void adj_candidate(const StepConst& c, const AdjIn& S, const double2* __restrict__ lF,
+                                              const double* __restrict__ lM, double* __restrict__ grad, uint32_t i,
+                                              uint32_t j, AdjA& q, double g[kParams]) {
+  if (j == i) return;
+  uint32_t tB = 0;
+  float rB;
+  if (kRadius) {
+    rB = S.radius[j];
+  } else {
+    tB = clampi(S.type[j], c.num_types);
+    rB = c.mat[tB].radius;
+  }
+  const float2 pB = S.pos[j];
+  // FP32 forward replay of collideSpheres' decisions
+  const float rx = pB.x - q.pA.x, ry = pB.y - q.pA.y;
+  const float d2 = rx * rx + ry * ry;
+  const float R = q.rA + rB;
+  // d2 > fl(R*R)*(1+2^-20) implies sqrtf(d2) >= R (the forward's prefilter)
+  if (d2 > (R * R) * 1.00000095367431640625f && !(R < 1.0842022e-19f)) return;
+  const float dist = sqrtf(d2);
+  if (!(dist < R)) return;
+  if (kRadius) tB = clampi(S.type[j], c.num_types);
+  const float2 vB = S.vel[j];
+  const float wB = S.omega[j];
+  const float nx = rx / dist, ny = ry / dist;
+  const float tx = -ny, ty = nx;
+  const float rvx = vB.x - q.vA.x, rvy = vB.y - q.vA.y;
+  const float vn = rvx * nx + rvy * ny;
+  const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
+  const float4 pr = c.pair[q.tA * c.num_types + tB];  // kn, kt, cn in series and the mean mu
+  const float kn = pr.x, kt = pr.y, cn = pr.z, mu = pr.w;
+  const float dt = c.dt;
+  const bool damp = vn < 0.0f;
+  bool stick = false;
+  const bool fric = mu != 0.0f;
+  if (fric) {
+    const float s = -kn * (R - dist);
+    float ffx = s * nx, ffy = s * ny;
+    if (damp) {
+      const float s2 = cn * vn;
+      ffx += s2 * nx;
+      ffy += s2 * ny;
+    }
+    const float flen = sqrtf(ffx * ffx + ffy * ffy);
+    const float sr = q.rA * q.wA + rB * wB;
+    const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
+    const float ftx = (kt * cvx) * dt, fty = (kt * cvy) * dt;
+    stick = sqrtf(ftx * ftx + fty * fty) < mu * flen;
+  }
+  // closed-form derivatives in double
+  const double dn_x = nx, dn_y = ny, dt_x = tx, dt_y = ty, ddist = dist;
+  const double inv_d = 1.0 / ddist;
+  const double dvn = (double)rvx * dn_x + (double)rvy * dn_y;
+  const double dvt = (double)rvx * dt_x + (double)rvy * dt_y;
+  const double dkn = kn, dkt = kt, dcn = cn, dmu = mu, ddtm = dt;
+  const double delta = (double)R - ddist;
+  const double s = -dkn * delta + (damp ? dcn * dvn : 0.0);
+  const double P = fabs(s);
+  double ct = 0.0, ft = 0.0;
+  if (fric) {
+    ct = dvt - ((double)q.rA * q.wA + (double)rB * wB);
+    ft = stick ? dkt * ddtm * ct : dmu * P * sgn(ct);
+  }
+  const double2 lFB = lF[j];
+  const double gFx = q.lFA.x - lFB.x, gFy = q.lFA.y - lFB.y;
+  const double gtau = (double)q.rA * q.lMA + (double)rB * lM[j];
+  const double a = gFx * dn_x + gFy * dn_y;
+  const double gFt = gFx * dt_x + gFy * dt_y;
+  const double bb = gFt + gtau;
+  const double cdamp = damp ? dcn : 0.0;
+  const double dsx = dn_x * dkn + dt_x * (cdamp * dvt * inv_d);
+  const double dsy = dn_y * dkn + dt_y * (cdamp * dvt * inv_d);
+  const double dsrx = dn_x * cdamp, dsry = dn_y * cdamp;
+  double dftx = 0.0, dfty = 0.0, dftrx = 0.0, dftry = 0.0, dftw = 0.0, slip_k = 0.0;
+  if (fric) {
+    if (stick) {
+      dftx = dt_x * (-dkt * ddtm * dvn * inv_d);
+      dfty = dt_y * (-dkt * ddtm * dvn * inv_d);
+      dftrx = dt_x * (dkt * ddtm);
+      dftry = dt_y * (dkt * ddtm);
+      dftw = -dkt * ddtm;
+    } else {
+      slip_k = dmu * sgn(s) * sgn(ct);
+      dftx = dsx * slip_k;
+      dfty = dsy * slip_k;
+      dftrx = dsrx * slip_k;
+      dftry = dsry * slip_k;
+    }
+  }
+  q.dx_acc.x += dsx * a + dt_x * (s * gFt * inv_d) + dftx * bb - dt_x * (ft * a * inv_d);
+  q.dx_acc.y += dsy * a + dt_y * (s * gFt * inv_d) + dfty * bb - dt_y * (ft * a * inv_d);
+  q.drv_acc.x += dsrx * a + dftrx * bb;
+  q.drv_acc.y += dsry * a + dftry * bb;
+  q.dw_acc += dftw * bb;
+  if (i < j) {
+    const double gkn = a * (-delta) + bb * slip_k * (-delta);
+    const double gcn = damp ? (a * dvn + bb * slip_k * dvn) : 0.0;
+    const double gkt = (fric && stick) ? bb * ddtm * ct : 0.0;
+    const double gmu = (fric && !stick) ? bb * P * sgn(ct) : 0.0;
+    double ka, kb, ta, tb, ca, cb;
+    if (c.dpair) {
+      const double* d = c.dpair + 6 * ((size_t)q.tA * c.num_types + tB);
+      ka = d[0];
+      kb = d[1];
+      ta = d[2];
+      tb = d[3];
+      ca = d[4];
+      cb = d[5];
+    } else {
+      const Material& B = c.mat[tB];
+      series_grad(q.A.kn, B.kn, ka, kb);
+      series_grad(q.A.kt, B.kt, ta, tb);
+      series_grad(q.A.cn, B.cn, ca, cb);
+    }
+    g[0] += gkn * ka;
+    g[1] += gkt * ta;
+    g[2] += gcn * ca;
+    g[3] += gmu * 0.5;
+    if (tB == q.tA) {
+      g[0] += gkn * kb;
+      g[1] += gkt * tb;
+      g[2] += gcn * cb;
+      g[3] += gmu * 0.5;
+    } else {
+      if (gkn * kb != 0.0) atomicAdd(&grad[kParams * tB + 0], gkn * kb);
+      if (gkt * tb != 0.0) atomicAdd(&grad[kParams * tB + 1], gkt * tb);
+      if (gcn * cb != 0.0) atomicAdd(&grad[kParams * tB + 2], gcn * cb);
+      if (gmu != 0.0) atomicAdd(&grad[kParams * tB + 3], gmu * 0.5);
+    }
+  }
+}
+
+// Reverse of the pair law for particle i over its 3x3 neighbourhood, in the
+// forward's candidate order: cells by row (y = -1, 0, 1) and column; the
+// three cells of a row are consecutive keys, walked as one range of sorted
+// candidates (cell by cell where a wrapping grid breaks that at an edge).
 template <bool kRadius, bool kMulti>
 __global__ __launch_bounds__(256) void k_adj_collide(StepConst c, AdjIn S, const uint32_t* __restrict__ cell_start,
                                                      const uint32_t* __restrict__ cell_end,
@@ -322,158 +505,72 @@ __global__ __launch_bounds__(256) void k_adj_collide(StepConst c, AdjIn S, const
   const uint32_t tA = clampi(S.type[ii], c.num_types);
   double g[kParams] = {0.0, 0.0, 0.0, 0.0};
   if (active) {
-    const Material A = c.mat[tA];
-    const Instance in = kMulti ? c.inst[clampi(A.inst, c.num_instances)] : c.inst0;
-    float rA, massA, inertiaA;
-    body_of<kRadius>(A, S.radius, i, rA, massA, inertiaA);
-    const float2 pA = S.pos[i], vA = S.vel[i];
-    const float wA = S.omega[i];
-    const float dt = c.dt;
+    AdjA q;
+    q.tA = tA;
+    q.A = c.mat[tA];
+    const Instance in = kMulti ? c.inst[clampi(q.A.inst, c.num_instances)] : c.inst0;
+    float massA, inertiaA;
+    body_of<kRadius>(q.A, S.radius, i, q.rA, massA, inertiaA);
+    q.pA = S.pos[i];
+    q.vA = S.vel[i];
+    q.wA = S.omega[i];
     const bool clamp = c.flags & DEM_GRID_CLAMP;
-    int gx = to_int_sat(floorf(pA.x * (float)in.grid[0] / in.extent[0]));
-    int gy = to_int_sat(floorf(pA.y * (float)in.grid[1] / in.extent[1]));
+    int gx = to_int_sat(floorf(q.pA.x * (float)in.grid[0] / in.extent[0]));
+    int gy = to_int_sat(floorf(q.pA.y * (float)in.grid[1] / in.extent[1]));
     if (clamp) {
       gx = min(max(gx, 0), (int)in.grid[0] - 1);
       gy = min(max(gy, 0), (int)in.grid[1] - 1);
     }
-    const double2 lFA = lF[i];
-    const double lMA = lM[i];
-    double2 dx_acc = make_double2(0.0, 0.0), drv_acc = make_double2(0.0, 0.0);
-    double dw_acc = 0.0;
+    q.lFA = lF[i];
+    q.lMA = lM[i];
+    q.dx_acc = make_double2(0.0, 0.0);
+    q.drv_acc = make_double2(0.0, 0.0);
+    q.dw_acc = 0.0;
+    const uint32_t G0 = in.grid[0], G1 = in.grid[1];
+    const bool vx0 = !clamp || gx > 0, vx2 = !clamp || gx < (int)G0 - 1;
+    const uint32_t x0 = ((uint32_t)gx - 1u) & (clamp ? 0xFFFFFFFFu : G0 - 1u);
+    const uint32_t x1 = clamp ? (uint32_t)gx : ((uint32_t)gx & (G0 - 1u));
+    const uint32_t x2 = ((uint32_t)gx + 1u) & (clamp ? 0xFFFFFFFFu : G0 - 1u);
+    const bool contiguous = clamp || (x0 + 1u == x1 && x1 + 1u == x2);
     for (int y = -1; y <= 1; ++y) {
-      for (int x = -1; x <= 1; ++x) {
-        uint32_t cxi, cyi;
-        if (clamp) {
-          const int ix = gx + x, iy = gy + y;
-          if (ix < 0 || iy < 0 || ix >= (int)in.grid[0] || iy >= (int)in.grid[1]) continue;
-          cxi = (uint32_t)ix;
-          cyi = (uint32_t)iy;
-        } else {
-          cxi = ((uint32_t)gx + (uint32_t)x) & (in.grid[0] - 1u);
-          cyi = ((uint32_t)gy + (uint32_t)y) & (in.grid[1] - 1u);
-        }
-        const uint32_t key = in.cell_base + cyi * in.grid[0] + cxi;
-        const uint32_t st = cell_start[key];
-        if (st == kEmpty) continue;
-        const uint32_t en = min(cell_end[key], n);
-        for (uint32_t j = st; j < en; ++j) {
-          if (j == i) continue;
-          const uint32_t tB = clampi(S.type[j], c.num_types);
-          const Material B = c.mat[tB];
-          float rB, mB, IB;
-          body_of<kRadius>(B, S.radius, j, rB, mB, IB);
-          const float2 pB = S.pos[j];
-          // FP32 forward replay of collideSpheres' decisions
-          const float rx = pB.x - pA.x, ry = pB.y - pA.y;
-          const float dist = sqrtf(rx * rx + ry * ry);
-          const float R = rA + rB;
-          if (!(dist < R)) continue;
-          const float2 vB = S.vel[j];
-          const float wB = S.omega[j];
-          const float nx = rx / dist, ny = ry / dist;
-          const float tx = -ny, ty = nx;
-          const float rvx = vB.x - vA.x, rvy = vB.y - vA.y;
-          const float vn = rvx * nx + rvy * ny;
-          const float vtx = rvx - vn * nx, vty = rvy - vn * ny;
-          const float kn = series_f(A.kn, B.kn), kt = series_f(A.kt, B.kt), cn = series_f(A.cn, B.cn);
-          const float mu = (A.mu + B.mu) / 2;
-          const bool damp = vn < 0.0f;
-          bool stick = false;
-          const bool fric = mu != 0.0f;
-          if (fric) {
-            const float s = -kn * (R - dist);
-            float ffx = s * nx, ffy = s * ny;
-            if (damp) {
-              const float s2 = cn * vn;
-              ffx += s2 * nx;
-              ffy += s2 * ny;
-            }
-            const float flen = sqrtf(ffx * ffx + ffy * ffy);
-            const float sr = rA * wA + rB * wB;
-            const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
-            const float ftx = (kt * cvx) * dt, fty = (kt * cvy) * dt;
-            stick = sqrtf(ftx * ftx + fty * fty) < mu * flen;
-          }
-          // closed-form derivatives in double
-          const double dn_x = nx, dn_y = ny, dt_x = tx, dt_y = ty, ddist = dist;
-          const double dvn = (double)rvx * dn_x + (double)rvy * dn_y;
-          const double dvt = (double)rvx * dt_x + (double)rvy * dt_y;
-          const double dkn = kn, dkt = kt, dcn = cn, dmu = mu, ddtm = dt;
-          const double delta = (double)R - ddist;
-          const double s = -dkn * delta + (damp ? dcn * dvn : 0.0);
-          const double P = fabs(s);
-          double ct = 0.0, ft = 0.0;
-          if (fric) {
-            ct = dvt - ((double)rA * wA + (double)rB * wB);
-            ft = stick ? dkt * ddtm * ct : dmu * P * sgn(ct);
-          }
-          const double2 lFB = lF[j];
-          const double gFx = lFA.x - lFB.x, gFy = lFA.y - lFB.y;
-          const double gtau = (double)rA * lMA + (double)rB * lM[j];
-          const double a = gFx * dn_x + gFy * dn_y;
-          const double gFt = gFx * dt_x + gFy * dt_y;
-          const double bb = gFt + gtau;
-          const double cdamp = damp ? dcn : 0.0;
-          const double dsx = dn_x * dkn + dt_x * (cdamp * dvt / ddist);
-          const double dsy = dn_y * dkn + dt_y * (cdamp * dvt / ddist);
-          const double dsrx = dn_x * cdamp, dsry = dn_y * cdamp;
-          double dftx = 0.0, dfty = 0.0, dftrx = 0.0, dftry = 0.0, dftw = 0.0, slip_k = 0.0;
-          if (fric) {
-            if (stick) {
-              dftx = dt_x * (-dkt * ddtm * dvn / ddist);
-              dfty = dt_y * (-dkt * ddtm * dvn / ddist);
-              dftrx = dt_x * (dkt * ddtm);
-              dftry = dt_y * (dkt * ddtm);
-              dftw = -dkt * ddtm;
-            } else {
-              slip_k = dmu * sgn(s) * sgn(ct);
-              dftx = dsx * slip_k;
-              dfty = dsy * slip_k;
-              dftrx = dsrx * slip_k;
-              dftry = dsry * slip_k;
-            }
-          }
-          dx_acc.x += dsx * a + dt_x * (s * gFt / ddist) + dftx * bb - dt_x * (ft * a / ddist);
-          dx_acc.y += dsy * a + dt_y * (s * gFt / ddist) + dfty * bb - dt_y * (ft * a / ddist);
-          drv_acc.x += dsrx * a + dftrx * bb;
-          drv_acc.y += dsry * a + dftry * bb;
-          dw_acc += dftw * bb;
-          if (i < j) {
-            const double gkn = a * (-delta) + bb * slip_k * (-delta);
-            const double gcn = damp ? (a * dvn + bb * slip_k * dvn) : 0.0;
-            const double gkt = (fric && stick) ? bb * ddtm * ct : 0.0;
-            const double gmu = (fric && !stick) ? bb * P * sgn(ct) : 0.0;
-            double ka, kb, ta, tb, ca, cb;
-            series_grad(A.kn, B.kn, ka, kb);
-            series_grad(A.kt, B.kt, ta, tb);
-            series_grad(A.cn, B.cn, ca, cb);
-            g[0] += gkn * ka;
-            g[1] += gkt * ta;
-            g[2] += gcn * ca;
-            g[3] += gmu * 0.5;
-            if (tB == tA) {
-              g[0] += gkn * kb;
-              g[1] += gkt * tb;
-              g[2] += gcn * cb;
-              g[3] += gmu * 0.5;
-            } else {
-              if (gkn * kb != 0.0) atomicAdd(&grad[kParams * tB + 0], gkn * kb);
-              if (gkt * tb != 0.0) atomicAdd(&grad[kParams * tB + 1], gkt * tb);
-              if (gcn * cb != 0.0) atomicAdd(&grad[kParams * tB + 2], gcn * cb);
-              if (gmu != 0.0) atomicAdd(&grad[kParams * tB + 3], gmu * 0.5);
-            }
-          }
+      const int iy = gy + y;
+      uint32_t cy;
+      if (clamp) {
+        if (iy < 0 || iy >= (int)G1) continue;
+        cy = (uint32_t)iy;
+      } else {
+        cy = (uint32_t)iy & (G1 - 1u);
+      }
+      const uint32_t row = in.cell_base + cy * G0;
+      if (contiguous) {
+        // the row's candidates: from the first non-empty cell's start to the
+        // last non-empty cell's end
+        const uint32_t a0 = row + (vx0 ? x0 : x1), a1 = row + x1, a2 = row + (vx2 ? x2 : x1);
+        const uint32_t s0 = vx0 ? cell_start[a0] : kEmpty, s1 = cell_start[a1], s2 = vx2 ? cell_start[a2] : kEmpty;
+        const bool n0 = s0 != kEmpty, n1 = s1 != kEmpty, n2 = s2 != kEmpty;
+        if (!(n0 || n1 || n2)) continue;
+        const uint32_t rs = n0 ? s0 : (n1 ? s1 : s2);
+        const uint32_t re = min(n2 ? cell_end[a2] : (n1 ? cell_end[a1] : cell_end[a0]), n);
+        for (uint32_t j = rs; j < re; ++j) adj_candidate<kRadius>(c, S, lF, lM, grad, i, j, q, g);
+      } else {
+        const uint32_t xs[3] = {x0, x1, x2};
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t key = row + xs[k];
+          const uint32_t st = cell_start[key];
+          if (st == kEmpty) continue;
+          const uint32_t en = min(cell_end[key], n);
+          for (uint32_t j = st; j < en; ++j) adj_candidate<kRadius>(c, S, lF, lM, grad, i, j, q, g);
         }
       }
     }
     double2 lxi = lxp[i], lvi = lvp[i];
-    lxi.x -= dx_acc.x;
-    lxi.y -= dx_acc.y;
-    lvi.x -= drv_acc.x;
-    lvi.y -= drv_acc.y;
+    lxi.x -= q.dx_acc.x;
+    lxi.y -= q.dx_acc.y;
+    lvi.x -= q.drv_acc.x;
+    lvi.y -= q.drv_acc.y;
     lxp[i] = lxi;
     lvp[i] = lvi;
-    lwp[i] = lwp[i] + (double)rA * dw_acc;
+    lwp[i] = lwp[i] + (double)q.rA * q.dw_acc;
   }
   flush_grads(g, tA, active, grad);
 }
@@ -504,6 +601,12 @@ struct dem_adjoint {
   float *somega = nullptr, *sradius = nullptr, *salpha = nullptr;
   uint32_t *stype = nullptr, *sindex = nullptr, *skey = nullptr;
   size_t slot_off = 0;  // the slot the step hook fills
+  // per slot of the segment: the step was re-sorted in place (its index is
+  // the identity except at a few slots)
+  std::vector<char> slot_inplace;
+  // the slots hold the last segment as dem_adj_forward recorded it (until a
+  // backward pass recomputes other segments into them)
+  bool last_recorded = false;
   // adjoint state (FP64)
   double2 *lx = nullptr, *lv = nullptr, *lxp = nullptr, *lvp = nullptr, *lF = nullptr;
   double *lw = nullptr, *lwp = nullptr, *lM = nullptr, *grad = nullptr;
@@ -556,12 +659,19 @@ hipError_t copy_state(dem_sim* m, AdjCheckpoint& c, bool to_checkpoint, hipStrea
   return hipSuccess;
 }
 
-hipError_t store_slot_hook(void* user, dem_sim* m, hipStream_t s, int t) {
+hipError_t store_slot_hook(void* user, dem_sim* m, hipStream_t s, int dyn, int stat, bool list) {
   auto* a = static_cast<dem_adjoint*>(user);
   const size_t off = a->slot_off;
   const uint32_t n = m->n;
-  hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[kScratch], m->vel[kScratch], m->omega[kScratch],
-                     m->rad ? m->radius[t] : nullptr, m->type[t], m->sorted_idx, m->sorted_key, m->acc, m->alpha,
+  if (list) {
+    // an in-place step: its index is the identity except at the listed slots
+    const hipError_t e = launch_shift_index(s, m->shift, a->sindex + off, n);
+    if (e != hipSuccess) return e;
+  }
+  a->slot_inplace[off / n] = list ? 1 : 0;
+  hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[dyn], m->vel[dyn], m->omega[dyn],
+                     m->rad ? m->radius[stat] : nullptr, m->type[stat], list ? nullptr : m->sorted_idx,
+                     m->sorted_key, m->acc, m->alpha,
                      a->spos + off, a->svel + off, a->somega + off, a->sradius ? a->sradius + off : nullptr,
                      a->stype + off, a->sindex + off, a->skey + off, a->sacc + off, a->salpha + off, n);
   return hipGetLastError();
@@ -637,13 +747,25 @@ int dem_adj_forward(dem_adjoint* a, uint32_t steps) {
     }
   }
   hipStream_t s = ctx->stream;
+  // the last segment's steps are recorded into the slots as they run, so
+  // that the backward pass, which starts with that segment, need not
+  // recompute it
+  const uint32_t last0 = steps ? ((steps - 1) / a->K) * a->K : 0;
+  a->last_recorded = false;
+  a->slot_inplace.assign(a->K, 0);
+  StepHook hook;
+  hook.fn = store_slot_hook;
+  hook.user = a;
   for (uint32_t k = 0; k < steps; ++k) {
     if (k % a->K == 0) DEM_HIP(ctx, copy_state(m, a->ckpt[k / a->K], true, s), "checkpoint");
-    DEM_HIP(ctx, sim_step_on(m, s, nullptr, sim_use_resort(m)), "forward step");
+    const bool record = k >= last0;
+    if (record) a->slot_off = (size_t)(k - last0) * m->n;
+    DEM_HIP(ctx, sim_step_on(m, s, nullptr, sim_use_resort(m), record ? &hook : nullptr), "forward step");
     m->resort_ok = true;
   }
   DEM_HIP(ctx, copy_state(m, a->final_state, true, s), "final state");
   a->steps = steps;
+  a->last_recorded = steps > 0;
   return DEM_OK;
 }
 
@@ -668,16 +790,22 @@ int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_ve
     // re-run the segment with the owned step (re-sorted after its first
     // step), storing each step's sorted pre-integrate state, keys,
     // permutation and accelerations
-    DEM_HIP(ctx, copy_state(m, a->ckpt[seg], false, s), "restore checkpoint");
-    m->resort_ok = false;
-    StepHook hook;
-    hook.fn = store_slot_hook;
-    hook.user = a;
-    for (uint32_t k = 0; k < len; ++k) {
-      a->slot_off = (size_t)k * n;
-      DEM_HIP(ctx, sim_step_on(m, s, nullptr, sim_use_resort(m), &hook), "segment step");
-      m->resort_ok = true;
+    if (seg == (int)nseg - 1 && a->last_recorded) {
+      // recorded by the forward pass
+    } else {
+      DEM_HIP(ctx, copy_state(m, a->ckpt[seg], false, s), "restore checkpoint");
+      m->resort_ok = false;
+      StepHook hook;
+      hook.fn = store_slot_hook;
+      hook.user = a;
+      a->slot_inplace.assign(a->K, 0);
+      for (uint32_t k = 0; k < len; ++k) {
+        a->slot_off = (size_t)k * n;
+        DEM_HIP(ctx, sim_step_on(m, s, nullptr, sim_use_resort(m), &hook), "segment step");
+        m->resort_ok = true;
+      }
     }
+    a->last_recorded = false;  // the slots are about to be reused
     // reverse through the segment: cell ranges from the stored keys, forces
     // from the stored accelerations
     for (int k = (int)len - 1; k >= 0; --k) {
@@ -702,8 +830,18 @@ int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_ve
       DEM_ADJ(k_adj_collide, c, S, (const uint32_t*)m->cell_start, (const uint32_t*)m->cell_end,
               (const double2*)a->lF, (const double*)a->lM, a->lxp, a->lvp, a->lwp, a->grad, n);
 #undef DEM_ADJ
-      hipLaunchKernelGGL(k_adj_permute, grid_of(n), dim3(256), 0, s, a->sindex + off, a->lxp, a->lvp, a->lwp, a->lx,
-                         a->lv, a->lw, n);
+      if (a->slot_inplace[k]) {
+        hipLaunchKernelGGL(k_adj_perm_save, grid_of(n), dim3(256), 0, s, a->sindex + off, a->lxp, a->lvp, a->lwp,
+                           a->lx, a->lv, a->lw, n);
+        hipLaunchKernelGGL(k_adj_perm_put, grid_of(n), dim3(256), 0, s, a->sindex + off, a->lx, a->lv, a->lw,
+                           a->lxp, a->lvp, a->lwp, n);
+        std::swap(a->lx, a->lxp);
+        std::swap(a->lv, a->lvp);
+        std::swap(a->lw, a->lwp);
+      } else {
+        hipLaunchKernelGGL(k_adj_permute, grid_of(n), dim3(256), 0, s, a->sindex + off, a->lxp, a->lvp, a->lwp,
+                           a->lx, a->lv, a->lw, n);
+      }
       DEM_HIP(ctx, hipGetLastError(), "adjoint kernels");
     }
   }

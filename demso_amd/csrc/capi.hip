@@ -93,6 +93,18 @@ float series_host(float a, float b) {
   return (a * b) / (a + b);
 }
 
+// d(a b/(a+b))/da = b^2/(a+b)^2 and d/db = a^2/(a+b)^2 in double, 0 where
+// seriesAssociation is 0 (the adjoint's formula).
+void series_grad_host(double a, double b, double& da, double& db) {
+  if (a == 0.0 || b == 0.0) {
+    da = db = 0.0;
+    return;
+  }
+  const double inv = 1.0 / ((a + b) * (a + b));
+  da = b * b * inv;
+  db = a * a * inv;
+}
+
 int upload_tables(dem_context* ctx) {
   const uint32_t nt = (uint32_t)ctx->h_mat.size(), ni = (uint32_t)ctx->h_inst.size();
   if (int r = grow(ctx, &ctx->d_mat, &ctx->mat_cap, nt)) return r;
@@ -111,6 +123,24 @@ int upload_tables(dem_context* ctx) {
                               ctx->stream),
           "upload material pairs");
   ctx->c.pair = ctx->d_pair;
+  ctx->c.dpair = nullptr;
+  if (nt <= dem::kDPairMaxTypes) {
+    // d(series)/d(a), d(series)/d(b) of kn, kt, cn for the adjoint, in double
+    std::vector<double> dp(6 * (size_t)nt * nt);
+    for (uint32_t a = 0; a < nt; ++a)
+      for (uint32_t b = 0; b < nt; ++b) {
+        const Material &A = ctx->h_mat[a], &B = ctx->h_mat[b];
+        double* d = dp.data() + 6 * ((size_t)a * nt + b);
+        series_grad_host(A.kn, B.kn, d[0], d[1]);
+        series_grad_host(A.kt, B.kt, d[2], d[3]);
+        series_grad_host(A.cn, B.cn, d[4], d[5]);
+      }
+    if (int r = grow(ctx, &ctx->d_dpair, &ctx->dpair_cap, (uint32_t)dp.size())) return r;
+    DEM_HIP(ctx, hipMemcpyAsync(ctx->d_dpair, dp.data(), sizeof(double) * dp.size(), hipMemcpyHostToDevice,
+                                ctx->stream),
+            "upload material pair derivatives");
+    ctx->c.dpair = ctx->d_dpair;
+  }
   DEM_HIP(ctx, hipMemcpyAsync(ctx->d_mat, ctx->h_mat.data(), sizeof(Material) * nt, hipMemcpyHostToDevice, ctx->stream),
           "upload materials");
   DEM_HIP(ctx, hipMemcpyAsync(ctx->d_inst, ctx->h_inst.data(), sizeof(Instance) * ni, hipMemcpyHostToDevice, ctx->stream),
@@ -229,6 +259,7 @@ int dem_destroy(dem_context* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->d_mat) (void)hipFree(ctx->d_mat);
   if (ctx->d_pair) (void)hipFree(ctx->d_pair);
+  if (ctx->d_dpair) (void)hipFree(ctx->d_dpair);
   if (ctx->d_inst) (void)hipFree(ctx->d_inst);
   if (ctx->d_copy_tmp) (void)hipFree(ctx->d_copy_tmp);
   if (ctx->d_control) (void)hipFree(ctx->d_control);
@@ -526,7 +557,7 @@ dem::ShiftState shift_state(dem_sim* m) {
 // fixed around them, and collide + integrate run on the sorted state in
 // place (integrate computes the next step's keys and mover bits as in the
 // other layout).
-hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev) {
+hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev, const StepHook* hook) {
   dem_context* ctx = m->ctx;
   const StepConst& c = ctx->c;
   const int b = m->cur;
@@ -557,6 +588,12 @@ hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev) {
   if (control) {
     e = dem::launch_control_finish(s, co, m->react_bx, m->react_by, m->react_bm, n, ctx->d_reaction, ctx->d_control,
                                    true);
+    if (e != hipSuccess) return e;
+  }
+  m->sorted_key = m->skey[b];
+  if (hook && hook->fn) {
+    // the sorted pre-integrate state is buffer b itself
+    e = hook->fn(hook->user, m, s, b, b, true);
     if (e != hipSuccess) return e;
   }
   if (ev) (void)hipEventRecord(ev[5], s);
@@ -590,7 +627,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   //  * full (fused collide + integrate, or a hook that records the sorted
   //    pre-integrate state): reorder sorts every field into kScratch, from
   //    which collide + integrate write buffer t.
-  if (resort && sim_inplace(m, hook)) return sim_step_inplace(m, s, ev);
+  if (resort && sim_inplace(m, hook)) return sim_step_inplace(m, s, ev, hook);
   const int o = m->cur, t = o ^ 1;
   const bool gather = !ctx->fuse && !(hook && hook->fn);
   const int S = gather ? t : kScratch;
@@ -690,7 +727,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   }
   if (hook && hook->fn) {
     // the full layout: the sorted pre-integrate state is still in kScratch / t
-    e = hook->fn(hook->user, m, s, t);
+    e = hook->fn(hook->user, m, s, kScratch, t, false);
     if (e != hipSuccess) return e;
   }
   if (ev) (void)hipEventRecord(ev[6], s);
@@ -700,9 +737,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   return hipSuccess;
 }
 
-bool sim_inplace(const dem_sim* m, const StepHook* hook) {
-  return !m->ctx->copy && !m->ctx->fuse && !(hook && hook->fn);
-}
+bool sim_inplace(const dem_sim* m, const StepHook*) { return !m->ctx->copy && !m->ctx->fuse; }
 
 bool sim_use_resort(const dem_sim* m) {
   if (!m->resort_ok || m->sort_mode == 1) return false;

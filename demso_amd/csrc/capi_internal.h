@@ -17,6 +17,8 @@ struct dem_context {
   uint32_t mat_cap = 0;
   float4* d_pair = nullptr;  // the contact law's per-pair parameters (StepConst::pair)
   uint32_t pair_cap = 0;
+  double* d_dpair = nullptr;  // their derivatives for the adjoint (StepConst::dpair)
+  uint32_t dpair_cap = 0;
   dem::Instance* d_inst = nullptr;
   uint32_t inst_cap = 0;
   std::vector<dem::Material> h_mat;
@@ -117,10 +119,15 @@ int sim_check(dem_sim* m);
 // resort: sort by dem::launch_resort (needs m->resort_ok) instead of the
 // radix sort; without a hook and with the split collide/integrate layout the
 // state is then re-sorted in place (unless ctx->copy).
-// hook: launched after collide (before integrate) with the step's target
-// buffer t, e.g. to store the sorted pre-integrate state (the adjoint).
+// hook: given the step's sorted pre-integrate state, e.g. to store it (the
+// adjoint): dyn is the buffer holding the sorted pos/vel/omega (kScratch in
+// the full layout, where the hook runs after integrate, or the state buffer
+// itself in place, where it runs between collide and integrate), stat the
+// buffer of the sorted radius/type; list: the step's sorted index is the
+// identity except at the slots listed in m->shift (in place), else it is
+// m->sorted_idx.
 struct StepHook {
-  hipError_t (*fn)(void* user, dem_sim* m, hipStream_t s, int t) = nullptr;
+  hipError_t (*fn)(void* user, dem_sim* m, hipStream_t s, int dyn, int stat, bool list) = nullptr;
   void* user = nullptr;
 };
 hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, const StepHook* hook = nullptr);

@@ -47,8 +47,13 @@ struct StepConst {
   // (src/particles_kernel.cuh:262-268), computed once on the host with the
   // same float operations (seriesAssociation, :36-41).
   const float4* pair;
+  // For the adjoint, per ordered pair (6 doubles at 6 * (tA * num_types +
+  // tB)): d(series)/d(own value) and d(series)/d(partner's value) of kn, kt
+  // and cn, in double; null above kDPairMaxTypes material rows.
+  const double* dpair;
   Instance inst0;
 };
+constexpr uint32_t kDPairMaxTypes = 1024;
 // Material rows per context (the pair table holds up to kMaxTypes^2 entries).
 constexpr uint32_t kMaxTypes = DEM_MAX_TYPES;
 

@@ -75,6 +75,27 @@ def test_checkpoint_interval_is_transparent(gpu_ctx, weights):
     assert np.allclose(a[2], b[2], rtol=1e-12, atol=0)
 
 
+def test_second_backward_recomputes_last_segment(gpu_ctx, weights):
+    """The forward pass records its last segment for the first backward pass;
+    a second backward pass after the same forward recomputes that segment from
+    its checkpoint (the slots were reused), with the same gradients."""
+    import torch
+
+    s, W, V = weights
+    sim = api.Simulation(gpu_ctx, s)
+    adj = api.Adjoint(sim, checkpoint_every=7)
+    adj.forward(STEPS)
+    gp = torch.from_numpy(W.astype(np.float32)).cuda()
+    gv = torch.from_numpy(V.astype(np.float32)).cuda()
+    first = [t.cpu().numpy() for t in adj.backward(gp, gv, num_types=len(s.materials))]
+    second = [t.cpu().numpy() for t in adj.backward(gp, gv, num_types=len(s.materials))]
+    adj.close()
+    sim.close()
+    assert np.array_equal(first[0], second[0]) and np.array_equal(first[1], second[1])
+    # parameter gradients are summed with FP64 atomics, in no fixed order
+    assert np.allclose(first[2], second[2], rtol=1e-12, atol=0)
+
+
 def test_ensemble_adjoint(gpu_ctx, oracle):
     s = sc.ensemble(instances=3, counts=[300, 500, 200], seed=6)
     for k in range(3):
