@@ -863,7 +863,8 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   if (!r) r = sim_alloc(m, &m->shift.src, n);
   if (!r) r = sim_alloc(m, &m->shift.emptied, n);
   if (!r && m->rad) r = sim_alloc(m, &m->shift.radius, n);
-  if (!r) r = sim_alloc(m, &m->shift.count, 2);
+  if (!r) r = sim_alloc(m, &m->shift.count, 4);
+  if (!r) r = sim_alloc(m, &m->shift.tag, n);
   m->shift.pos = m->pos[kScratch];
   m->shift.vel = m->vel[kScratch];
   m->shift.theta = m->theta[kScratch];
@@ -887,7 +888,8 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   }
   // cellEnd starts at 0 as in allocateVectors (src/functions.cu:140).
   hipError_t e = dem::launch_fill(ctx->stream, m->cell_end, 0u, num_cells);
-  if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->shift.count, 0u, 2);
+  if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->shift.count, 0u, 4);
+  if (e == hipSuccess) e = dem::launch_fill(ctx->stream, (uint32_t*)m->shift.tag, 0u, 2 * n);
   if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->cs[0], dem::kEmpty, num_cells);
   if (e == hipSuccess) e = dem::launch_fill(ctx->stream, m->cs[1], dem::kEmpty, num_cells);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

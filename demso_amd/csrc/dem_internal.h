@@ -106,8 +106,10 @@ hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* pre
 //           (new slot, old slot, key) with its state copied into staging,
 //           and the cells that lose all their particles;
 //   apply : the staged state written to the new slots (sorted keys and loc
-//           too), cellStart cleared for the emptied cells, then the cell
-//           boundaries re-evaluated at every new slot and the slot after it.
+//           too), cellStart cleared for the emptied cells, and the cell
+//           boundaries re-evaluated at every new slot and the slot after it
+//           (one kernel: the new key of a listed slot is read from its tag,
+//           that of any other slot is its unchanged sorted key).
 // The result equals reorderDataAndFindCellStart (src/particles_kernel.cuh:
 // 154-233) applied to the stable sort of the new keys, for any number of
 // movers: every other cell boundary is the one the previous step wrote.
@@ -124,7 +126,10 @@ struct ShiftScratch {
   uint32_t *id = nullptr, *type = nullptr, *key = nullptr;
   uint32_t *dst = nullptr, *src = nullptr;  // new and old slot of each listed particle
   uint32_t* emptied = nullptr;              // cells left without particles
-  uint32_t* count = nullptr;                // [0] listed particles, [1] emptied cells
+  uint32_t* count = nullptr;                // [0] listed particles, [1] emptied cells, [2] step epoch
+  // per slot: (epoch << 32 | new key) of the last step that listed it, so
+  // that the new key of any slot is known while the state is written back
+  uint64_t* tag = nullptr;
   uint32_t capacity = 0;                    // n
 };
 hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const ShiftState& st, const ShiftScratch& sh,

@@ -147,8 +147,12 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(uint32_t* __rest
   __shared__ uint32_t ctot[kDigits];
   __shared__ uint32_t s_m;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave;
-  // the in-place placement that follows appends to these lists
-  if (shift_count && tid < 2) shift_count[tid] = 0;
+  // the in-place placement that follows appends to these lists, under a
+  // new epoch
+  if (shift_count) {
+    if (tid < 2) shift_count[tid] = 0;
+    if (tid == 2) shift_count[2] = shift_count[2] + 1u;
+  }
 
   uint32_t part = 0;
   for (uint32_t t = tid; t < tiles; t += kSortThreads) part += tile_count[t];
@@ -415,7 +419,8 @@ __device__ __forceinline__ uint32_t keys_below(const uint32_t* __restrict__ sk, 
 
 template <bool kRadius>
 __device__ __forceinline__ void stage(const ShiftScratch& sh, const ShiftState& st, uint32_t at, uint32_t from,
-                                      uint32_t to, uint32_t key) {
+                                      uint32_t to, uint32_t key, uint32_t epoch) {
+  sh.tag[to] = ((uint64_t)epoch << 32) | key;
   sh.dst[at] = to;
   sh.src[at] = from;
   sh.key[at] = key;
@@ -446,6 +451,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
   const uint32_t* sk = info[1] ? mkey1 : mkey0;
   const uint32_t* sp = info[1] ? mpos1 : mpos0;
   const uint32_t* prev_key = st.skey;
+  const uint32_t epoch = sh.count[2];
   const uint32_t tid = threadIdx.x;
 
   if (blockIdx.x < kMoverBlocks) {
@@ -463,7 +469,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
       }
       const bool emit = ok && p < n && slot < n;
       const uint32_t at = wave_append(emit, &sh.count[0]);
-      if (emit && at < sh.capacity) stage<kRadius>(sh, st, at, p, slot, k);
+      if (emit && at < sh.capacity) stage<kRadius>(sh, st, at, p, slot, k, epoch);
       // the mover's old cell is left empty when all its particles moved
       // out and none moved in (checked by the cell's first mover only)
       bool emptied = false;
@@ -562,15 +568,47 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
     }
     const bool emit = stayer && slot != i && slot < n;
     const uint32_t at = wave_append(emit, &sh.count[0]);
-    if (emit && at < sh.capacity) stage<kRadius>(sh, st, at, i, slot, kk[it]);
+    if (emit && at < sh.capacity) stage<kRadius>(sh, st, at, i, slot, kk[it], epoch);
   }
 }
 
-// Staged state -> new slots; cellStart of emptied cells -> EMPTY.
+// The new sorted key of slot q: from its tag if this step listed it, else
+// its unchanged key (which no lane writes).
+__device__ __forceinline__ uint32_t new_key(const uint32_t* skey, const uint64_t* __restrict__ tag, uint32_t q,
+                                            uint32_t epoch) {
+  const uint64_t t = tag[q];
+  return (uint32_t)(t >> 32) == epoch ? (uint32_t)t : skey[q];
+}
+
+// The boundary test of reorderDataAndFindCellStartD (src/particles_kernel.
+// cuh:189-212) at slot q of the new sorted keys (q == n: the last cell's end).
+__device__ __forceinline__ void cell_record(const uint32_t* skey, const uint64_t* __restrict__ tag, uint32_t epoch,
+                                            uint32_t q, uint32_t* cell_start, uint32_t* cell_end, uint32_t num_cells,
+                                            uint32_t n) {
+  if (q == n) {
+    const uint32_t h = new_key(skey, tag, n - 1u, epoch);
+    if (h < num_cells) cell_end[h] = n;
+    return;
+  }
+  const uint32_t h = new_key(skey, tag, q, epoch);
+  if (q == 0) {
+    if (h < num_cells) cell_start[h] = 0;
+    return;
+  }
+  const uint32_t hp = new_key(skey, tag, q - 1u, epoch);
+  if (h != hp) {
+    if (h < num_cells) cell_start[h] = q;
+    if (hp < num_cells) cell_end[hp] = q;
+  }
+}
+
+// Staged state -> new slots; cellStart of emptied cells -> EMPTY; the cell
+// boundaries at every listed new slot and the slot after it (the only places
+// where the key sequence changed; two lanes may write the same record, with
+// the same value).
 template <bool kRadius>
-__global__ __launch_bounds__(256) void k_shift_scatter(ShiftState st, ShiftScratch sh, uint32_t num_cells,
-                                                       uint32_t n) {
-  const uint32_t S = min(sh.count[0], sh.capacity), E = min(sh.count[1], sh.capacity);
+__global__ __launch_bounds__(256) void k_shift_apply(ShiftState st, ShiftScratch sh, uint32_t num_cells, uint32_t n) {
+  const uint32_t S = min(sh.count[0], sh.capacity), E = min(sh.count[1], sh.capacity), epoch = sh.count[2];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += stride) {
     const uint32_t d = sh.dst[k];
@@ -585,49 +623,12 @@ __global__ __launch_bounds__(256) void k_shift_scatter(ShiftState st, ShiftScrat
     st.type[d] = sh.type[k];
     if (kRadius) st.radius[d] = sh.radius[k];
     st.skey[d] = sh.key[k];
+    cell_record(st.skey, sh.tag, epoch, d, st.cell_start, st.cell_end, num_cells, n);
+    cell_record(st.skey, sh.tag, epoch, d + 1u, st.cell_start, st.cell_end, num_cells, n);
   }
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride) {
     const uint32_t c = sh.emptied[k];
     if (c < num_cells) st.cell_start[c] = kEmpty;
-  }
-}
-
-// The boundary test of reorderDataAndFindCellStartD (src/particles_kernel.
-// cuh:189-212) at slot q of the new sorted keys (q == n: the last cell's end).
-__device__ __forceinline__ void cell_record(const uint32_t* __restrict__ skey, uint32_t q, uint32_t* cell_start,
-                                            uint32_t* cell_end, uint32_t num_cells, uint32_t n) {
-  if (q == n) {
-    const uint32_t h = skey[n - 1u];
-    if (h < num_cells) cell_end[h] = n;
-    return;
-  }
-  const uint32_t h = skey[q];
-  if (q == 0) {
-    if (h < num_cells) cell_start[h] = 0;
-    return;
-  }
-  const uint32_t hp = skey[q - 1u];
-  if (h != hp) {
-    if (h < num_cells) cell_start[h] = q;
-    if (hp < num_cells) cell_end[hp] = q;
-  }
-}
-
-// Cell boundaries at every listed new slot and the slot after it: the only
-// places where the key sequence changed.  Two lanes may write the same
-// record; they write the same value.
-__global__ __launch_bounds__(256) void k_shift_records(const uint32_t* __restrict__ skey,
-                                                       const uint32_t* __restrict__ dst,
-                                                       const uint32_t* __restrict__ count, uint32_t capacity,
-                                                       uint32_t* cell_start, uint32_t* cell_end, uint32_t num_cells,
-                                                       uint32_t n) {
-  const uint32_t S = min(count[0], capacity);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += stride) {
-    const uint32_t d = dst[k];
-    if (d >= n) continue;
-    cell_record(skey, d, cell_start, cell_end, num_cells, n);
-    cell_record(skey, d + 1u, cell_start, cell_end, num_cells, n);
   }
 }
 
@@ -658,7 +659,7 @@ hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const Sh
   if (n == 0) return hipSuccess;
   if (n > r.capacity || !r.bits || !r.word_prefix || !r.tile_count || !r.mkey[0] || !r.mkey[1] || !r.mpos[0] ||
       !r.mpos[1] || !r.info || key_bits > 32 || n > sh.capacity || !sh.count || !sh.dst || !sh.src || !sh.key ||
-      !sh.emptied || !sh.pos || !sh.vel || !sh.theta || !sh.omega || !sh.id || !sh.type || !st.pos || !st.vel ||
+      !sh.emptied || !sh.tag || !sh.pos || !sh.vel || !sh.theta || !sh.omega || !sh.id || !sh.type || !st.pos || !st.vel ||
       !st.theta || !st.omega || !st.id || !st.type || !st.skey || !st.cell_start || !st.cell_end ||
       (st.radius && !sh.radius))
     return hipErrorInvalidValue;
@@ -680,14 +681,12 @@ hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const Sh
 hipError_t launch_resort_shift_apply(hipStream_t s, const ShiftState& st, const ShiftScratch& sh, uint32_t num_cells,
                                      uint32_t n) {
   if (n == 0) return hipSuccess;
-  if (n > sh.capacity || !sh.count || !sh.dst || !st.skey || !st.loc || !st.cell_start || !st.cell_end ||
-      (st.radius && !sh.radius))
+  if (n > sh.capacity || !sh.count || !sh.dst || !sh.tag || !st.skey || !st.loc || !st.cell_start ||
+      !st.cell_end || (st.radius && !sh.radius))
     return hipErrorInvalidValue;
   const dim3 grid(list_blocks(n));
-  if (st.radius) hipLaunchKernelGGL(k_shift_scatter<true>, grid, dim3(256), 0, s, st, sh, num_cells, n);
-  else hipLaunchKernelGGL(k_shift_scatter<false>, grid, dim3(256), 0, s, st, sh, num_cells, n);
-  hipLaunchKernelGGL(k_shift_records, grid, dim3(256), 0, s, st.skey, sh.dst, sh.count, sh.capacity, st.cell_start,
-                     st.cell_end, num_cells, n);
+  if (st.radius) hipLaunchKernelGGL(k_shift_apply<true>, grid, dim3(256), 0, s, st, sh, num_cells, n);
+  else hipLaunchKernelGGL(k_shift_apply<false>, grid, dim3(256), 0, s, st, sh, num_cells, n);
   return hipGetLastError();
 }
 
