@@ -598,7 +598,6 @@ hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev, const Ste
   }
   if (ev) (void)hipEventRecord(ev[5], s);
   dem::NextKeys nk;
-  nk.prev_key = m->skey[b];
   nk.key = m->key;
   nk.bits = m->resort.bits;
   nk.tile_count = m->resort.tile_count;
@@ -677,7 +676,6 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[4], s);
   dem::NextKeys nk;
-  nk.prev_key = m->skey[t];
   nk.key = m->key;
   nk.bits = m->resort.bits;
   nk.tile_count = m->resort.tile_count;
@@ -1334,7 +1332,6 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   if (e == hipSuccess) e = hipMalloc((void**)&ttheta, 4 * (size_t)n);
   if (e == hipSuccess) e = hipMalloc((void**)&tomega, 4 * (size_t)n);
   dem::NextKeys nk;
-  nk.prev_key = sk;
   nk.key = m->key == sk ? m->idx : m->key;  // any n-word buffer other than sk
   nk.bits = m->resort.bits;
   nk.tile_count = m->resort.tile_count;

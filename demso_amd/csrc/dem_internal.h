@@ -152,12 +152,11 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
                           const uint32_t* cell_end, float* pressure, uint32_t n, int mode = 0,
                           const struct ControlOut* ctl = nullptr, const uint32_t* gat = nullptr);
 // The next step's hash, fused into integrate: bit i of `bits` set when the
-// cell key of the new position differs from prev_key[i] (the keys the state
-// is sorted by), and then key[i] = that key (key[i] is not written for the
-// others: their key is prev_key[i]); tile_count[i / kResortTile] += movers
-// (the counts must be zero before).
+// cell key of the new position differs from the key the particle is sorted
+// by (the hash of its position before integrate), and then key[i] = the new
+// key (not written for the others: their key stays the sorted key);
+// tile_count[i / kResortTile] += movers (the counts must be zero before).
 struct NextKeys {
-  const uint32_t* prev_key = nullptr;
   uint32_t* key = nullptr;
   uint32_t* bits = nullptr;
   uint32_t* tile_count = nullptr;

@@ -710,15 +710,17 @@ __device__ __forceinline__ bool integrate_particle(const StepConst& c, const Mat
   return false;
 }
 
-// The next step's hash of an integrated particle (NextKeys).
+// The next step's hash of an integrated particle (NextKeys): the cell key of
+// its new position p, and whether it left the cell of its sorted slot (the
+// state stays in this order until the next sort).  The key the particle is
+// sorted by is the hash of its position before integrate, p_old, so it is
+// recomputed here rather than read.
 template <bool kMulti>
-__device__ __forceinline__ void next_key(const StepConst& c, const NextKeys& nk, float2 p,
+__device__ __forceinline__ void next_key(const StepConst& c, const NextKeys& nk, float2 p, float2 p_old,
                                          const uint32_t* __restrict__ type, uint32_t i, uint32_t n) {
-  // the new cell key, and whether it left the cell of its sorted slot (the
-  // state stays in this order until the next sort)
   const uint32_t k = cell_key<kMulti>(c, p, type, i);
-  const bool mover = k != nk.prev_key[i];
-  if (mover) nk.key[i] = k;  // a stayer's key is prev_key[i]
+  const bool mover = k != cell_key<kMulti>(c, p_old, type, i);
+  if (mover) nk.key[i] = k;  // a stayer's key is its sorted key
   const uint64_t moved = __ballot(mover);
   if (__lane_id() == 0) {  // lane 0 holds the wave's first index, a multiple of 64
     const uint32_t w0 = i / 32u, nwords = (n + 31u) / 32u;
@@ -749,7 +751,8 @@ __global__ __launch_bounds__(256) void k_integrate(
   body<kRadius>(m, radius, i, r, mass, inertia);
   // velocity, spin and angle through the gather index when one is given
   const uint32_t g = gat ? gat[i] : i;
-  float2 p = pos_in[i], v = vel_in[g];
+  const float2 p_old = pos_in[i];
+  float2 p = p_old, v = vel_in[g];
   float w = omega_in[g], th = theta_in[g];
   float pres = 0.0f;
   if (integrate_particle(c, m, in, r, mass, inertia, acc[i], alpha[i], p, v, w, th, pres, pressure + i))
@@ -758,7 +761,7 @@ __global__ __launch_bounds__(256) void k_integrate(
   vel_out[i] = v;
   omega_out[i] = w;
   theta_out[i] = th;
-  if (kNext) next_key<kMulti>(c, nk, p, type, i, n);
+  if (kNext) next_key<kMulti>(c, nk, p, p_old, type, i, n);
 }
 
 // collide + integrate + the next step's hash in one pass over the sorted
@@ -795,7 +798,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   vel_out[i] = v;
   omega_out[i] = w;
   theta_out[i] = th;
-  next_key<kMat == kInstances>(c, nk, p, type, i, n);
+  next_key<kMat == kInstances>(c, nk, p, own.p, type, i, n);
 }
 
 inline dim3 grid_for(uint32_t n, int block) { return dim3((n + (uint32_t)block - 1u) / (uint32_t)block); }
