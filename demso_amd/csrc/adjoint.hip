@@ -499,7 +499,7 @@ __global__ __launch_bounds__(256) void k_adj_collide(StepConst c, AdjIn S, const
                                                      const double2* __restrict__ lF, const double* __restrict__ lM,
                                                      double2* __restrict__ lxp, double2* __restrict__ lvp,
                                                      double* __restrict__ lwp, double* __restrict__ grad, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   const bool active = i < n;
   const uint32_t ii = active ? i : 0u;
   const uint32_t tA = clampi(S.type[ii], c.num_types);

@@ -67,6 +67,19 @@ struct ControlOut {
 };
 constexpr uint32_t kReactBlock = 512;  // the reference's reduction block (src/functions.cu:475)
 
+// Workgroups are dispatched to the 8 XCDs round-robin (XCD = workgroup id
+// mod 8), each XCD with its own L2.  The collide kernels gather the
+// neighbour rows of their particles, which lie a few workgroups away in
+// sorted order; giving each XCD one contiguous eighth of the workgroups (a
+// bijection of the ids) keeps those rows in the L2 of the XCD that reads
+// them.  Returns the workgroup's position in that order.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t x = b % kXcds, local = b / kXcds, q = nb / kXcds, r = nb % kXcds;
+  return x * q + min(x, r) + local;
+}
+
 // Launch helpers (defined in the .hip files).
 hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float2* pos,
                        const uint32_t* type, uint32_t* hash, uint32_t* index, uint32_t n);

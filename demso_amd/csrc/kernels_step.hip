@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256) void k_collide(
     const uint32_t* __restrict__ type, const float* __restrict__ radius,
     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
     float* __restrict__ pressure, ControlOut ctl, const uint32_t* __restrict__ gat, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
@@ -779,7 +779,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint32_t* __restrict__ cell_end, float2* __restrict__ acc, float* __restrict__ alpha,
     float* __restrict__ pressure, float2* __restrict__ pos_out, float2* __restrict__ vel_out,
     float* __restrict__ omega_out, float* __restrict__ theta_out, NextKeys nk, ControlOut ctl, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
