@@ -852,6 +852,8 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   if (!r) r = sim_alloc(m, &m->resort.bits, words);
   if (!r) r = sim_alloc(m, &m->resort.word_prefix, words);
   if (!r) r = sim_alloc(m, &m->resort.tile_count, tiles);
+  if (!r) r = sim_alloc(m, &m->resort.tile_off, tiles + 1);
+  if (!r) r = sim_alloc(m, &m->resort.tile_list, tiles);
   if (!r) r = sim_alloc(m, &m->resort.info, 2);
   // in-place re-sort: list, staging (dynamic fields in kScratch) and counts
   if (!r) r = sim_alloc(m, &m->shift.key, n);

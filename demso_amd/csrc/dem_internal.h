@@ -90,9 +90,10 @@ hipError_t launch_fill(hipStream_t s, uint32_t* p, uint32_t value, uint32_t coun
 constexpr uint32_t kResortTile = 4096;
 struct ResortScratch {
   uint32_t* bits = nullptr;         // ceil(n/32) mover bits
-  uint32_t* word_prefix = nullptr;  // ceil(n/32) movers before each word
+  uint32_t* word_prefix = nullptr;  // ceil(n/32) movers before each word (written for tiles with movers)
   uint32_t* tile_count = nullptr;   // tiles (accumulated by integrate, cleared by the movers kernel)
-  uint32_t* tile_off = nullptr;     // tiles
+  uint32_t* tile_off = nullptr;     // tiles + 1: movers before each tile
+  uint32_t* tile_list = nullptr;    // tiles: the tiles that have movers
   uint32_t* mkey[2] = {};           // n each: mover keys (compacted, then sorted)
   uint32_t* mpos[2] = {};           // n each: mover positions
   uint32_t* info = nullptr;         // [0] mover count, [1] buffer holding the sorted movers

@@ -19,11 +19,12 @@
 // The movers' keys, the mover bit mask (bit i%32 of word i/32) and the
 // movers per tile of kResortTile particles come from the previous step's
 // integrate, which hashes the positions it has just written.  Launches:
-//   compact : per tile, movers before each 32-particle word and the
-//             compacted (key, position) list of the movers;
-//   movers  : one workgroup sorts the movers: a bitonic sort of the unique
-//             64-bit (key << 32 | position) values in LDS up to 4096 movers,
-//             a stable chunked LSD radix sort through global memory beyond;
+//   movers  : one workgroup: per-tile offsets of the movers from the tile
+//             counts; for the tiles that have movers, the movers before each
+//             32-particle word and the compacted (key, position) list; then
+//             the sort of the movers: a bitonic sort of the unique 64-bit
+//             (key << 32 | position) values in LDS up to 4096 movers, a
+//             stable chunked LSD radix sort through global memory beyond;
 //   place   : the two slot formulas above, one particle per lane.  A
 //             stayer's mover count is a binary search in the window of
 //             sorted movers that falls between the first and last keys of
@@ -52,10 +53,16 @@ constexpr int kDigits = 1 << kDigitBits;
 
 __device__ __forceinline__ uint64_t composite(uint32_t key, uint32_t pos) { return ((uint64_t)key << 32) | pos; }
 
-// Movers among positions [0, p).
+// Movers among positions [0, p): the tile's offset where p's tile has no
+// movers (its word prefixes are not written), else the word prefix plus the
+// movers before p in its word.
 __device__ __forceinline__ uint32_t movers_before(uint32_t p, uint32_t n, uint32_t m, const uint32_t* __restrict__ bits,
-                                                  const uint32_t* __restrict__ word_prefix) {
+                                                  const uint32_t* __restrict__ word_prefix,
+                                                  const uint32_t* __restrict__ tile_off) {
   if (p >= n) return m;
+  const uint32_t t = p / kResortTile;
+  const uint32_t o = tile_off[t];
+  if (tile_off[t + 1u] == o) return o;
   const uint32_t w = p >> 5, b = p & 31u;
   return word_prefix[w] + (uint32_t)__popc(bits[w] & ((1u << b) - 1u));
 }
@@ -88,37 +95,6 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
   return peers;
 }
 
-// Per tile: the movers before each word (word_prefix) and the movers of the
-// tile appended to (mkey, mpos) in position order.  128 threads, one word
-// each; the tile's offset is the sum of the counts of the tiles before it.
-__global__ __launch_bounds__(kWordsPerTile) void k_resort_compact(
-    const uint32_t* __restrict__ key, const uint32_t* __restrict__ bits, const uint32_t* __restrict__ tile_count,
-    uint32_t* __restrict__ word_prefix, uint32_t* __restrict__ mkey, uint32_t* __restrict__ mpos, uint32_t n) {
-  __shared__ uint32_t red[2];
-  const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave, tile = blockIdx.x;
-  uint32_t before = 0;
-  for (uint32_t t = tid; t < tile; t += kWordsPerTile) before += tile_count[t];
-  before = wave_sum(before);
-  if (lane == 0) red[w] = before;
-  const uint32_t nwords = (n + 31u) / 32u;
-  const uint32_t word = tile * kWordsPerTile + tid;
-  const uint32_t b = word < nwords ? bits[word] : 0u;
-  const uint32_t c = (uint32_t)__popc(b);
-  const uint32_t inc = wave_inclusive_scan(c);
-  __shared__ uint32_t wtot[2];
-  if (lane == kWave - 1) wtot[w] = inc;
-  __syncthreads();
-  const uint32_t tile_off = red[0] + red[1];
-  uint32_t r = tile_off + inc - c + (w == 1 ? wtot[0] : 0u);
-  if (word < nwords) word_prefix[word] = r;
-  for (uint32_t v = b; v; v &= v - 1u) {
-    const uint32_t p = word * 32u + (uint32_t)(__ffs(v) - 1);
-    mkey[r] = key[p];
-    mpos[r] = p;
-    ++r;
-  }
-}
-
 // Exclusive scan of one value per thread over a 1024-thread block.
 __device__ __forceinline__ uint32_t block_scan_1024(uint32_t x, uint32_t* wsum) {
   const uint32_t lane = __lane_id(), w = threadIdx.x / kWave;
@@ -131,21 +107,31 @@ __device__ __forceinline__ uint32_t block_scan_1024(uint32_t x, uint32_t* wsum) 
   return off + inc - x;
 }
 
-// One workgroup: the mover count (sum of the tile counts) and the stable
-// sort of (mkey[0], mpos[0]) by key.  info[0] = count, info[1] = the buffer
-// (0 or 1) that holds the sorted movers.
-__global__ __launch_bounds__(kSortThreads) void k_resort_movers(uint32_t* __restrict__ tile_count, uint32_t tiles,
-                                                                uint32_t* mkey0, uint32_t* mpos0, uint32_t* mkey1,
-                                                                uint32_t* mpos1, uint32_t key_bits,
-                                                                uint32_t* __restrict__ info,
-                                                                uint32_t* __restrict__ host_count,
-                                                                uint32_t* __restrict__ shift_count) {
+// One workgroup: the movers, compacted in position order, and their stable
+// sort by key.
+//  * tile_off[t] = movers in tiles before t (tile_off[tiles] = the count),
+//    from the per-tile counts, which are then cleared for the next step's
+//    integrate (it accumulates into them);
+//  * for every tile that has movers (one wave per tile): the movers before
+//    each of its 32-particle words (word_prefix; not written for the other
+//    tiles, where tile_off gives the answer) and its movers appended to
+//    (mkey[0], mpos[0]) in position order;
+//  * the sort: a bitonic sort of the unique 64-bit (key << 32 | position)
+//    values in LDS up to 4096 movers, a stable chunked LSD radix sort through
+//    global memory beyond.
+// info[0] = count, info[1] = the buffer (0 or 1) that holds the sorted movers.
+__global__ __launch_bounds__(kSortThreads) void k_resort_movers(
+    uint32_t* __restrict__ tile_count, uint32_t tiles, const uint32_t* __restrict__ bits,
+    const uint32_t* __restrict__ key, uint32_t n, uint32_t* __restrict__ tile_off, uint32_t* __restrict__ tile_list,
+    uint32_t* __restrict__ word_prefix, uint32_t* mkey0, uint32_t* mpos0, uint32_t* mkey1, uint32_t* mpos1,
+    uint32_t key_bits, uint32_t* __restrict__ info, uint32_t* __restrict__ host_count,
+    uint32_t* __restrict__ shift_count) {
   __shared__ uint64_t v[kBitonicMax];
   __shared__ uint32_t wsum[kSortWaves];
   __shared__ uint32_t wcnt[kSortWaves][kDigits];
   __shared__ uint32_t base[kDigits];
   __shared__ uint32_t ctot[kDigits];
-  __shared__ uint32_t s_m;
+  __shared__ uint32_t s_carry, s_nz;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave;
   // the in-place placement that follows appends to these lists, under a
   // new epoch
@@ -153,24 +139,57 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(uint32_t* __rest
     if (tid < 2) shift_count[tid] = 0;
     if (tid == 2) shift_count[2] = shift_count[2] + 1u;
   }
-
-  uint32_t part = 0;
-  for (uint32_t t = tid; t < tiles; t += kSortThreads) part += tile_count[t];
-  part = wave_sum(part);
-  if (lane == 0) wsum[w] = part;
-  __syncthreads();
   if (tid == 0) {
-    uint32_t m = 0;
-    for (int k = 0; k < kSortWaves; ++k) m += wsum[k];
-    s_m = m;
+    s_carry = 0;
+    s_nz = 0;
+  }
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < tiles; c0 += kSortThreads) {
+    const uint32_t t = c0 + tid;
+    const uint32_t cnt = t < tiles ? tile_count[t] : 0u;
+    const uint32_t ex = block_scan_1024(cnt, wsum);
+    const uint32_t carry = s_carry;
+    if (t < tiles) {
+      tile_off[t] = carry + ex;
+      tile_count[t] = 0;
+      if (cnt) tile_list[atomicAdd(&s_nz, 1u)] = t;
+    }
+    __syncthreads();
+    if (tid == kSortThreads - 1u) s_carry = carry + ex + cnt;
+    __syncthreads();
+  }
+  const uint32_t m = s_carry, nz = s_nz;
+  if (tid == 0) {
+    tile_off[tiles] = m;
     info[0] = m;
     if (host_count) *host_count = m;
   }
-  __syncthreads();
-  // the counts are consumed (compact ran before): clear them for the next
-  // step's integrate, which accumulates into them
-  for (uint32_t t = tid; t < tiles; t += kSortThreads) tile_count[t] = 0;
-  const uint32_t m = s_m;
+  const uint32_t nwords = (n + 31u) / 32u;
+  for (uint32_t k = w; k < nz; k += kSortWaves) {
+    const uint32_t t = tile_list[k];
+    const uint32_t off = tile_off[t];
+    const uint32_t wa = t * kWordsPerTile + lane, wb = wa + (uint32_t)kWave;
+    const uint32_t ba = wa < nwords ? bits[wa] : 0u, bb = wb < nwords ? bits[wb] : 0u;
+    const uint32_t ca = (uint32_t)__popc(ba), cb = (uint32_t)__popc(bb);
+    const uint32_t ia = wave_inclusive_scan(ca), ib = wave_inclusive_scan(cb);
+    const uint32_t tot_a = __shfl(ia, kWave - 1, kWave);
+    uint32_t ra = off + ia - ca, rb = off + tot_a + ib - cb;
+    if (wa < nwords) word_prefix[wa] = ra;
+    if (wb < nwords) word_prefix[wb] = rb;
+    for (uint32_t x = ba; x; x &= x - 1u) {
+      const uint32_t p = wa * 32u + (uint32_t)(__ffs(x) - 1);
+      mkey0[ra] = key[p];
+      mpos0[ra] = p;
+      ++ra;
+    }
+    for (uint32_t x = bb; x; x &= x - 1u) {
+      const uint32_t p = wb * 32u + (uint32_t)(__ffs(x) - 1);
+      mkey0[rb] = key[p];
+      mpos0[rb] = p;
+      ++rb;
+    }
+  }
+  __syncthreads();  // the compacted movers are read by the whole workgroup below
   if (m <= 1u) {
     if (tid == 0) info[1] = 0;
     return;
@@ -301,7 +320,7 @@ __device__ __forceinline__ uint32_t insertion_point(uint32_t k, uint32_t p, cons
 
 __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     const uint32_t* __restrict__ prev_key, const uint32_t* __restrict__ bits,
-    const uint32_t* __restrict__ word_prefix, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
+    const uint32_t* __restrict__ word_prefix, const uint32_t* __restrict__ tile_off, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
     const uint32_t* mpos1, const uint32_t* __restrict__ info, const uint32_t* __restrict__ prev_cell_start,
     const uint32_t* __restrict__ prev_cell_end, uint32_t num_cells, int cells_valid, uint32_t* __restrict__ out_key,
     uint32_t* __restrict__ out_idx, uint32_t* __restrict__ clear, uint32_t clear_n, uint32_t n) {
@@ -317,7 +336,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     for (uint32_t r = blockIdx.x * kPlaceThreads + tid; r < m; r += kMoverBlocks * kPlaceThreads) {
       const uint32_t k = sk[r], p = sp[r];
       const uint32_t P = insertion_point(k, p, prev_key, prev_cell_start, prev_cell_end, num_cells, cells_valid, n);
-      const uint32_t slot = r + P - movers_before(P, n, m, bits, word_prefix);
+      const uint32_t slot = r + P - movers_before(P, n, m, bits, word_prefix, tile_off);
       if (slot < n) {  // always true for a consistent input; guards the store
         out_key[slot] = k;
         out_idx[slot] = p;
@@ -355,6 +374,10 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     for (uint32_t j = tid; j < hi - lo; j += kPlaceThreads) win[j] = composite(sk[lo + j], sp[lo + j]);
   __syncthreads();
   // all loads of the thread's items first, then the placements
+  // the block's particles lie in one tile of kResortTile: without movers
+  // there, its word prefixes are not written and tile_off gives them
+  const uint32_t rt = first / kResortTile, tile_base = tile_off[rt];
+  const bool tile_movers = tile_off[rt + 1u] != tile_base;
   uint32_t kk[kPlaceItems], bw[kPlaceItems], wp[kPlaceItems];
 #pragma unroll
   for (uint32_t it = 0; it < kPlaceItems; ++it) {
@@ -362,7 +385,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
     const bool valid = i <= last;
     kk[it] = valid ? prev_key[i] : 0u;  // a stayer's key (movers are skipped below)
     bw[it] = valid ? bits[i >> 5] : 0xFFFFFFFFu;
-    wp[it] = valid ? word_prefix[i >> 5] : 0u;
+    wp[it] = valid ? (tile_movers ? word_prefix[i >> 5] : tile_base) : 0u;
   }
 #pragma unroll
   for (uint32_t it = 0; it < kPlaceItems; ++it) {
@@ -442,7 +465,7 @@ __device__ __forceinline__ void stage(const ShiftScratch& sh, const ShiftState& 
 template <bool kRadius>
 __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
     ShiftState st, ShiftScratch sh, const uint32_t* __restrict__ bits,
-    const uint32_t* __restrict__ word_prefix, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
+    const uint32_t* __restrict__ word_prefix, const uint32_t* __restrict__ tile_off, const uint32_t* mkey0, const uint32_t* mpos0, const uint32_t* mkey1,
     const uint32_t* mpos1, const uint32_t* __restrict__ info, uint32_t num_cells, uint32_t n) {
   __shared__ uint64_t win[kWindowMax];
   __shared__ uint32_t s_lo, s_hi, s_bf, s_bl;
@@ -464,7 +487,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
         p = sp[r];
         if (p < n) {
           const uint32_t P = insertion_point(k, p, prev_key, st.cell_start, st.cell_end, num_cells, 1, n);
-          slot = r + P - movers_before(P, n, m, bits, word_prefix);
+          slot = r + P - movers_before(P, n, m, bits, word_prefix, tile_off);
         }
       }
       const bool emit = ok && p < n && slot < n;
@@ -479,9 +502,9 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
         if (ko < num_cells) {
           const uint32_t s0 = st.cell_start[ko], e0 = st.cell_end[ko];
           if (s0 != kEmpty && s0 <= p && p < e0) {
-            const uint32_t mb0 = movers_before(s0, n, m, bits, word_prefix);
-            if (movers_before(p, n, m, bits, word_prefix) == mb0) {
-              const uint32_t leaving = movers_before(e0, n, m, bits, word_prefix) - mb0;
+            const uint32_t mb0 = movers_before(s0, n, m, bits, word_prefix, tile_off);
+            if (movers_before(p, n, m, bits, word_prefix, tile_off) == mb0) {
+              const uint32_t leaving = movers_before(e0, n, m, bits, word_prefix, tile_off) - mb0;
               const uint32_t entering = keys_below(sk, m, ko + 1u) - keys_below(sk, m, ko);
               emptied = e0 - s0 == leaving && entering == 0;
             }
@@ -531,10 +554,14 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
   }
   __syncthreads();
   {
-    const uint32_t mb_first = movers_before(first, n, m, bits, word_prefix);
-    const uint32_t mb_end = movers_before(last + 1u, n, m, bits, word_prefix);
+    const uint32_t mb_first = movers_before(first, n, m, bits, word_prefix, tile_off);
+    const uint32_t mb_end = movers_before(last + 1u, n, m, bits, word_prefix, tile_off);
     if (mb_first == mb_end && s_bf == s_bl && s_bf == mb_first) return;  // uniform over the block
   }
+  // the block's particles lie in one tile of kResortTile: without movers
+  // there, its word prefixes are not written and tile_off gives them
+  const uint32_t rt = first / kResortTile, tile_base = tile_off[rt];
+  const bool tile_movers = tile_off[rt + 1u] != tile_base;
   uint32_t kk[kPlaceItems], bw[kPlaceItems], wp[kPlaceItems];
 #pragma unroll
   for (uint32_t it = 0; it < kPlaceItems; ++it) {
@@ -542,7 +569,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
     const bool valid = i <= last;
     kk[it] = valid ? prev_key[i] : 0u;  // a stayer's key (movers are skipped below)
     bw[it] = valid ? bits[i >> 5] : 0xFFFFFFFFu;
-    wp[it] = valid ? word_prefix[i >> 5] : 0u;
+    wp[it] = valid ? (tile_movers ? word_prefix[i >> 5] : tile_base) : 0u;
   }
 #pragma unroll
   for (uint32_t it = 0; it < kPlaceItems; ++it) {
@@ -657,24 +684,23 @@ uint32_t list_blocks(uint32_t n) { return std::max(1u, std::min(128u, (n + 2047u
 hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const ShiftState& st, const ShiftScratch& sh,
                                     uint32_t num_cells, uint32_t key_bits, const ResortScratch& r, uint32_t n) {
   if (n == 0) return hipSuccess;
-  if (n > r.capacity || !r.bits || !r.word_prefix || !r.tile_count || !r.mkey[0] || !r.mkey[1] || !r.mpos[0] ||
-      !r.mpos[1] || !r.info || key_bits > 32 || n > sh.capacity || !sh.count || !sh.dst || !sh.src || !sh.key ||
+  if (n > r.capacity || !r.bits || !r.word_prefix || !r.tile_count || !r.tile_off || !r.tile_list || !r.mkey[0] ||
+      !r.mkey[1] || !r.mpos[0] || !r.mpos[1] || !r.info || key_bits > 32 || n > sh.capacity || !sh.count || !sh.dst || !sh.src || !sh.key ||
       !sh.emptied || !sh.tag || !sh.pos || !sh.vel || !sh.theta || !sh.omega || !sh.id || !sh.type || !st.pos || !st.vel ||
       !st.theta || !st.omega || !st.id || !st.type || !st.skey || !st.cell_start || !st.cell_end ||
       (st.radius && !sh.radius))
     return hipErrorInvalidValue;
   const uint32_t tiles = (n + kResortTile - 1u) / kResortTile;
-  hipLaunchKernelGGL(k_resort_compact, dim3(tiles), dim3(kWordsPerTile), 0, s, key, r.bits, r.tile_count,
-                     r.word_prefix, r.mkey[0], r.mpos[0], n);
-  hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.mkey[0], r.mpos[0],
-                     r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count, sh.count);
+  hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.bits, key, n,
+                     r.tile_off, r.tile_list, r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], key_bits,
+                     r.info, r.host_count, sh.count);
   const dim3 grid(kMoverBlocks + (n + kPlaceTile - 1u) / kPlaceTile);
   if (st.radius)
     hipLaunchKernelGGL(k_shift_place<true>, grid, dim3(kPlaceThreads), 0, s, st, sh, r.bits, r.word_prefix,
-                       r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, num_cells, n);
+                       r.tile_off, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, num_cells, n);
   else
     hipLaunchKernelGGL(k_shift_place<false>, grid, dim3(kPlaceThreads), 0, s, st, sh, r.bits, r.word_prefix,
-                       r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, num_cells, n);
+                       r.tile_off, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, num_cells, n);
   return hipGetLastError();
 }
 
@@ -704,17 +730,16 @@ hipError_t launch_resort(hipStream_t s, const uint32_t* key, const uint32_t* pre
                          uint32_t* out_key, uint32_t* out_idx, const ResortScratch& r, uint32_t n, uint32_t* clear,
                          uint32_t clear_n) {
   if (n == 0) return hipSuccess;
-  if (n > r.capacity || !r.bits || !r.word_prefix || !r.tile_count || !r.mkey[0] || !r.mkey[1] || !r.mpos[0] ||
-      !r.mpos[1] || !r.info || key_bits > 32)
+  if (n > r.capacity || !r.bits || !r.word_prefix || !r.tile_count || !r.tile_off || !r.tile_list || !r.mkey[0] ||
+      !r.mkey[1] || !r.mpos[0] || !r.mpos[1] || !r.info || key_bits > 32)
     return hipErrorInvalidValue;
   const uint32_t tiles = (n + kResortTile - 1u) / kResortTile;
-  hipLaunchKernelGGL(k_resort_compact, dim3(tiles), dim3(kWordsPerTile), 0, s, key, r.bits, r.tile_count,
-                     r.word_prefix, r.mkey[0], r.mpos[0], n);
-  hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.mkey[0], r.mpos[0],
-                     r.mkey[1], r.mpos[1], key_bits, r.info, r.host_count, nullptr);
+  hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(kSortThreads), 0, s, r.tile_count, tiles, r.bits, key, n,
+                     r.tile_off, r.tile_list, r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], key_bits,
+                     r.info, r.host_count, nullptr);
   const uint32_t place_tiles = (n + kPlaceTile - 1u) / kPlaceTile;
   hipLaunchKernelGGL(k_resort_place, dim3(kMoverBlocks + place_tiles), dim3(kPlaceThreads), 0, s, prev_key, r.bits,
-                     r.word_prefix, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, prev_cell_start,
+                     r.word_prefix, r.tile_off, r.mkey[0], r.mpos[0], r.mkey[1], r.mpos[1], r.info, prev_cell_start,
                      prev_cell_end, num_cells, cells_valid ? 1 : 0, out_key, out_idx, clear, clear_n, n);
   return hipGetLastError();
 }
