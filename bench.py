@@ -37,15 +37,17 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 # material rows, so the kernels read each particle's type (4 B where noted).
 #
 # In place (the default split layout; the state stays in its buffer):
-#   sort      R mover bits, W movers-before-each-word (1/8 + 1/8 per particle),
-#             plus 48 B per particle whose slot changes (not counted: a
-#             handful per step on the settling workload)
+#   sort      the per-tile mover counts (4 B per 4,096 particles) and the
+#             mover words of the tiles that have movers, plus 48 B per
+#             particle whose slot changes (a handful per step on the settling
+#             workload): not counted
 #   reorder   only the particles whose slot changes (not counted)
 #   collide   R pos 8, radius 4, type 4, vel 8, omega 4; W acc 8, alpha 4,
 #             pressure 4; + R cellStart/cellEnd 8 per cell
 #   integrate R pos 8, vel 8, theta 4, omega 4, acc 8, alpha 4, radius 4,
-#             type 4, previous key 4; W pos 8, vel 8, theta 4, omega 4 (the
-#             next key only for the few particles that change cell)
+#             type 4; W pos 8, vel 8, theta 4, omega 4 (the cell of the
+#             position read is recomputed, not loaded; the next key is
+#             written only for the few particles that change cell)
 # Copy (DEM_STEP_COPY): re-sort R previous key 4, W sorted key 4, index 4 and
 #   the next cellStart 4 per cell; reorder R key 4, index 4, pos 8, id 4,
 #   type 4, radius 4, W pos 8, id 4, loc 4, type 4, radius 4; collide and
@@ -54,9 +56,8 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 # Fused (collide + integrate in one kernel): reorder sorts every field (R key
 #   4, index 4, pos 8, vel 8, theta 4, omega 4, id 4, type 4, radius 4; W the
 #   same state and loc 4) and the fused kernel reads pos 8, vel 8, omega 4,
-#   theta 4, radius 4, type 4, previous key 4 and writes acc 8, alpha 4,
-#   pressure 4, pos 8, vel 8, theta 4, omega 4 (+ 8 per cell), reported
-#   under collide.
+#   theta 4, radius 4, type 4 and writes acc 8, alpha 4, pressure 4, pos 8,
+#   vel 8, theta 4, omega 4 (+ 8 per cell), reported under collide.
 STAGES = ("hash", "sort", "cell_fill", "reorder", "collide", "integrate")
 
 
@@ -66,15 +67,15 @@ def stage_bytes(layout: str, radius: bool, typed: bool) -> tuple[dict, dict]:
     t = 4 if typed else 0
     per_cell = {"hash": 0, "sort": 0, "cell_fill": 0, "reorder": 0, "collide": 8, "integrate": 0}
     if layout == "fused":
-        pp = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 76 + 2 * r, "collide": 68 + r + t, "integrate": 0}
+        pp = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 76 + 2 * r, "collide": 64 + r + t, "integrate": 0}
         per_cell["sort"] = 4
     elif layout == "copy":
         pp = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 44 + 2 * r,
-              "collide": 4 + 8 + r + t + 12 + 16, "integrate": 4 + 8 + 16 + 12 + r + t + 4 + 24}
+              "collide": 4 + 8 + r + t + 12 + 16, "integrate": 4 + 8 + 16 + 12 + r + t + 24}
         per_cell["sort"] = 4
     else:
-        pp = {"hash": 0, "sort": 0.25, "cell_fill": 0, "reorder": 0,
-              "collide": 8 + r + t + 12 + 16, "integrate": 8 + 16 + 12 + r + t + 4 + 24}
+        pp = {"hash": 0, "sort": 0, "cell_fill": 0, "reorder": 0,
+              "collide": 8 + r + t + 12 + 16, "integrate": 8 + 16 + 12 + r + t + 24}
     return pp, per_cell
 
 
