@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -182,14 +183,22 @@ def calibration_run(ctx, scene, steps: int, iters: int, K: int, settle_steps: in
     }
 
 
-def cpu_baseline(scene, steps: int, threads: int) -> dict:
-    """The oracle (FP32 port of the reference kernels) on this host."""
+def cpu_baseline(scene, steps: int, threads: int, seconds: float) -> dict:
+    """The oracle (FP32 port of the reference kernels) on this host: `steps`
+    steps to gauge the rate, then as many more as make about `seconds` of
+    CPU work in all."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
 
     used = pyoracle.set_threads(threads)
     sim = pyoracle.Sim(scene, "f32")
     secs = sim.step(steps)
+    for _ in range(4):  # top up: the first steps run slower than the rest
+        if secs >= 0.9 * seconds:
+            break
+        more = max(1, int(math.ceil((seconds - secs) * steps / secs)))
+        secs += sim.step(more)
+        steps += more
     sim.close()
     return {
         "value": scene.n * steps / secs,
@@ -209,7 +218,8 @@ def main() -> int:
     ap.add_argument("--workload", default="settling", choices=["settling", "column_collapse", "ensemble", "adjoint", "calibration"])
     ap.add_argument("--adj-steps", type=int, default=0, help="steps of the adjoint / calibration run (0 = workload default)")
     ap.add_argument("--iters", type=int, default=5, help="Adam iterations of the calibration workload")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps that gauge the CPU baseline's rate (0 = skip it)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work of the whole CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
@@ -272,7 +282,7 @@ def main() -> int:
 
     cpu = None
     if args.cpu_steps > 0:
-        cpu = cpu_baseline(scene, args.cpu_steps, args.cpu_threads)
+        cpu = cpu_baseline(scene, args.cpu_steps, args.cpu_threads, args.cpu_seconds)
 
     extra = {}
     if args.workload == "adjoint":
