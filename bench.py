@@ -222,6 +222,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work of the whole CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--collide-mode", choices=["plain", "prefetch", "batch"], default=None,
+                    help="with --no-autotune: collide's candidate mode (default: one candidate at a time)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="finish with a plain device copy of this many MiB (reference for HBM counters)")
     args = ap.parse_args()
@@ -243,6 +245,8 @@ def main() -> int:
 
         cell_tuning = tuning.autotune(ctx, scene, multipliers=(1.0, 1.25, 1.5, 2.0), steps=10)
         scene = tuning.with_cell_multiplier(scene, cell_tuning["multiplier"])
+    if args.collide_mode:
+        ctx.set_step_variant(prefetch=args.collide_mode == "prefetch", batch=args.collide_mode == "batch")
     sim = api.Simulation(ctx, scene)
     n, cells = scene.n, scene.total_cells
 

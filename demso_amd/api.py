@@ -145,7 +145,7 @@ class Context:
                          copy: bool = False) -> None:
         """Owned simulations: collide + integrate as one kernel or two, and
         how collide loads its candidates: one at a time, one ahead
-        (prefetch) or the first three of every row at once (batch); copy
+        (prefetch) or the first of every row at once (batch); copy
         makes re-sorted steps reorder the whole state into the other buffer
         instead of moving only the particles whose slot changes.  The
         results are the same in every combination."""

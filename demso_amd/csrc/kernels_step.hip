@@ -429,11 +429,12 @@ struct Own {
 // Candidate modes of the collide kernels: kPlain walks each row's candidates
 // one load at a time (each candidate's velocity and spin loaded with its
 // position); kPrefetch loads each one iteration ahead; kBatch loads the first
-// kBatchN candidates of all three rows at once before walking them (more
-// loads in flight, more registers).  The order of the sums is the same in
-// every mode.
+// candidate of all three rows at once before walking them (three loads in
+// flight).  The order of the sums is the same in every mode.  Preloading two
+// or three candidates a row needs 73 or 81 VGPRs (6 waves per SIMD instead of
+// 8) and was slower.
 constexpr int kPlain = 0, kPrefetch = 1, kBatch = 2;
-constexpr int kBatchN = 3;
+constexpr int kBatchN = 1;
 
 // Material layouts of the kernels: one material row (no type is read), one
 // instance with several rows, or an ensemble of instances.
@@ -597,13 +598,15 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   pres_out = pres;
 }
 
+#define DEM_COLLIDE_PARAMS                                                                    \
+  StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,              \
+      float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha, \
+      const uint32_t* __restrict__ type, const float* __restrict__ radius,                  \
+      const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,       \
+      float* __restrict__ pressure, ControlOut ctl, const uint32_t* __restrict__ gat, uint32_t n
+
 template <bool kRadius, int kMat, int kMode, bool kControl>
-__global__ __launch_bounds__(256) void k_collide(
-    StepConst c, const float2* __restrict__ pos, const float2* __restrict__ vel,
-    float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha,
-    const uint32_t* __restrict__ type, const float* __restrict__ radius,
-    const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
-    float* __restrict__ pressure, ControlOut ctl, const uint32_t* __restrict__ gat, uint32_t n) {
+__device__ __forceinline__ void collide_kernel_body(DEM_COLLIDE_PARAMS) {
   const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Own own;
@@ -614,6 +617,30 @@ __global__ __launch_bounds__(256) void k_collide(
   acc[i] = make_float2(fx / own.mass, fy / own.mass);
   alpha[i] = moment / own.inertia;
   pressure[i] = pres;
+}
+
+template <bool kRadius, int kMat, int kMode, bool kControl>
+__global__ __launch_bounds__(256) void k_collide(DEM_COLLIDE_PARAMS) {
+  collide_kernel_body<kRadius, kMat, kMode, kControl>(c, pos, vel, acc, omega, alpha, type, radius, cell_start,
+                                                      cell_end, pressure, ctl, gat, n);
+}
+
+// The one-material batched mode needs 65 VGPRs on its own; held to 64 (8
+// waves per SIMD) it does not spill.  The other batched layouts would spill
+// 12-19 registers under the same bound, so they keep k_collide.
+template <bool kRadius>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_collide_batch(DEM_COLLIDE_PARAMS) {
+  collide_kernel_body<kRadius, kOneMat, kBatch, false>(c, pos, vel, acc, omega, alpha, type, radius, cell_start,
+                                                       cell_end, pressure, ctl, gat, n);
+}
+#undef DEM_COLLIDE_PARAMS
+
+template <bool kRadius, int kMat, int kMode, bool kControl>
+constexpr auto collide_kernel() {
+  if constexpr (kMat == kOneMat && kMode == kBatch && !kControl)
+    return &k_collide_batch<kRadius>;
+  else
+    return &k_collide<kRadius, kMat, kMode, kControl>;
 }
 
 // ------------------------------------------------------------- integrate --
@@ -878,7 +905,7 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
   if (n == 0) return hipSuccess;
   const ControlOut co = ctl ? *ctl : ControlOut{};
 #define DEM_COLLIDE(R, M, P, X)                                                                        \
-  hipLaunchKernelGGL((k_collide<R, M, P, X>), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
+  hipLaunchKernelGGL((collide_kernel<R, M, P, X>()), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
                      omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, n)
   DEM_DISPATCH(DEM_COLLIDE);
 #undef DEM_COLLIDE

@@ -262,8 +262,8 @@ int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
 /* Kernel variants of the owned simulations of ctx (dem_sim_autotune sets
  * the fastest): DEM_STEP_FUSED runs collide and integrate as one kernel,
  * DEM_STEP_PREFETCH makes collide fetch each neighbour candidate one
- * iteration ahead, DEM_STEP_BATCH makes it load the first three candidates
- * of all three neighbour rows at once (not together with PREFETCH; the fused
+ * iteration ahead, DEM_STEP_BATCH makes it load the first candidate of all
+ * three neighbour rows at once (not together with PREFETCH; the fused
  * kernel loads one at a time instead).  Results are the same bit for bit. */
 #define DEM_STEP_FUSED 0x1
 #define DEM_STEP_PREFETCH 0x2
