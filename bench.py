@@ -183,14 +183,9 @@ def calibration_run(ctx, scene, steps: int, iters: int, K: int, settle_steps: in
     }
 
 
-def cpu_baseline(scene, steps: int, threads: int, seconds: float) -> dict:
-    """The oracle (FP32 port of the reference kernels) on this host: `steps`
-    steps to gauge the rate, then as many more as make about `seconds` of
-    CPU work in all."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import pyoracle
-
-    used = pyoracle.set_threads(threads)
+def _oracle_sample(pyoracle, scene, steps: int, seconds: float):
+    """Steps of the oracle from the scene's initial state: `steps` to gauge
+    the rate, then as many more as make about `seconds` in all."""
     sim = pyoracle.Sim(scene, "f32")
     secs = sim.step(steps)
     for _ in range(4):  # top up: the first steps run slower than the rest
@@ -200,13 +195,42 @@ def cpu_baseline(scene, steps: int, threads: int, seconds: float) -> dict:
         secs += sim.step(more)
         steps += more
     sim.close()
+    return steps, secs
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, steps: int, threads: int, seconds: float) -> dict:
+    """The oracle (FP32 port of the reference kernels) on this host, over
+    `threads` OpenMP threads for about `seconds` of work, and on one thread
+    for about half that."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+
+    used = pyoracle.set_threads(threads)
+    steps_m, secs_m = _oracle_sample(pyoracle, scene, steps, seconds)
+    pyoracle.set_threads(1)
+    steps_1, secs_1 = _oracle_sample(pyoracle, scene, 1, seconds / 2)
+    pyoracle.set_threads(used)
     return {
-        "value": scene.n * steps / secs,
+        "value": scene.n * steps_m / secs_m,
         "unit": "particle-steps/s",
         "cores": used,
         "kind": "port",
-        "sample": f"{steps} full steps of the same {scene.n}-disc workload from its initial state "
-                  f"({secs:.1f} s, OpenMP over {used} threads)",
+        "sample": f"{steps_m} full steps of the same {scene.n}-disc workload from its initial state "
+                  f"({secs_m:.1f} s, OpenMP over {used} threads)",
+        "single_thread": {"value": scene.n * steps_1 / secs_1, "cores": 1,
+                          "sample": f"{steps_1} steps of the same workload ({secs_1:.1f} s)"},
+        "cpu_model": _cpu_model(),
     }
 
 
