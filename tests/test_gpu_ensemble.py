@@ -25,9 +25,7 @@ def squeezed_ensemble():
     return s
 
 
-def test_ensemble_equals_instances_run_alone(gpu_ctx):
-    s = squeezed_ensemble()
-    steps = 12
+def check_equals_alone(gpu_ctx, s, steps):
     ens = api.Simulation(gpu_ctx, s)
     ens.step_graph(steps)
     E = ens.download()
@@ -49,6 +47,21 @@ def test_ensemble_equals_instances_run_alone(gpu_ctx):
         occ = O["cellStart"] != 0xFFFFFFFF
         assert np.array_equal(cs[occ] - a, O["cellStart"][occ])
         assert np.all(cs[~occ] == 0xFFFFFFFF)
+
+
+def test_ensemble_equals_instances_run_alone(gpu_ctx):
+    check_equals_alone(gpu_ctx, squeezed_ensemble(), 12)
+
+
+def test_256_instances_equal_instances_run_alone(gpu_ctx):
+    """256 instances (the ensemble workload's instance count and parameter
+    draws, with 50-1,500 discs each): one batched step per stage equals the
+    256 plain single-system steps, instance by instance."""
+    counts = np.random.default_rng(12).integers(50, 1500, size=256).tolist()
+    s = sc.ensemble(instances=256, counts=counts, seed=2)
+    assert len(s.instances) == 256
+    print(f"256 instances, {s.n} discs", flush=True)
+    check_equals_alone(gpu_ctx, s, 6)
 
 
 @pytest.mark.parametrize("mult", [1.0, 1.5, 2.0])
