@@ -142,14 +142,16 @@ class Context:
             self.set_materials(particle_props(scene, single_instance=False))
 
     def set_step_variant(self, fused: bool = False, prefetch: bool = False, batch: bool = False,
-                         copy: bool = False) -> None:
+                         copy: bool = False, wide: bool = False) -> None:
         """Owned simulations: collide + integrate as one kernel or two, and
         how collide loads its candidates: one at a time, one ahead
         (prefetch) or the first of every row at once (batch); copy
         makes re-sorted steps reorder the whole state into the other buffer
-        instead of moving only the particles whose slot changes.  The
-        results are the same in every combination."""
-        flags = int(bool(fused)) | (2 if prefetch else 0) | (4 if batch else 0) | (8 if copy else 0)
+        instead of moving only the particles whose slot changes; wide
+        makes collide use 64-bit addresses (as states over 2^29 particles
+        or 2^30 cells do).  The results are the same in every combination."""
+        flags = (int(bool(fused)) | (2 if prefetch else 0) | (4 if batch else 0) | (8 if copy else 0)
+                 | (16 if wide else 0))
         self._check(self.lib.dem_set_step_variant(self.handle, flags))
 
     def set_gravity(self, gx: float, gy: float) -> None:

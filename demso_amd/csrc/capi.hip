@@ -107,6 +107,7 @@ void series_grad_host(double a, double b, double& da, double& db) {
 
 int upload_tables(dem_context* ctx) {
   const uint32_t nt = (uint32_t)ctx->h_mat.size(), ni = (uint32_t)ctx->h_inst.size();
+  ctx->c.cells = ctx->total_cells;
   if (int r = grow(ctx, &ctx->d_mat, &ctx->mat_cap, nt)) return r;
   if (int r = grow(ctx, &ctx->d_inst, &ctx->inst_cap, ni)) return r;
   if (nt > dem::kMaxTypes) return fail(ctx, DEM_ERR_INVALID, "at most 4096 material rows per context");
@@ -193,6 +194,9 @@ int check_cells(dem_context* ctx, uint32_t num_cells) {
 }
 
 bool multi(const dem_context* ctx) { return ctx->c.num_instances > 1; }
+
+// The context's collide mode for launch_collide.
+int collide_mode(const dem_context* ctx) { return ctx->cmode | (ctx->wide ? kWideMode : 0); }
 
 }  // namespace dem
 
@@ -478,7 +482,7 @@ int dem_collide_ex(dem_context* ctx, const float* pos, const float* vel, float* 
   if (int r = ensure_device(ctx)) return r;
   DEM_HIP(ctx, dem::launch_collide(ctx->c, ctx->stream, ctx->bs_collide, (const float2*)pos, (const float2*)vel,
                                    (float2*)acc, omega, alpha, type, radius, cellStart, cellEnd, pressure,
-                                   numParticles, ctx->cmode),
+                                   numParticles, collide_mode(ctx)),
           "collide kernel");
   return DEM_OK;
 }
@@ -583,7 +587,7 @@ hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev, const Ste
   }
   const float* radius = m->rad ? m->radius[b] : nullptr;
   e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[b], m->vel[b], m->acc, m->omega[b], m->alpha, m->type[b],
-                          radius, m->cs[b], m->cell_end, m->pressure, n, ctx->cmode, control ? &co : nullptr);
+                          radius, m->cs[b], m->cell_end, m->pressure, n, collide_mode(ctx), control ? &co : nullptr);
   if (e != hipSuccess) return e;
   if (control) {
     e = dem::launch_control_finish(s, co, m->react_bx, m->react_by, m->react_bm, n, ctx->d_reaction, ctx->d_control,
@@ -707,7 +711,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
     const int D = gather ? o : S;
     const uint32_t* gat = gather ? m->sorted_idx : nullptr;
     e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[S], m->vel[D], m->acc, m->omega[D], m->alpha, m->type[t],
-                            S_radius, m->cs[t], m->cell_end, m->pressure, n, ctx->cmode,
+                            S_radius, m->cs[t], m->cell_end, m->pressure, n, collide_mode(ctx),
                             control ? &co : nullptr, gat);
     if (e != hipSuccess) return e;
     if (control) {
@@ -1007,13 +1011,14 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
 
 int dem_set_step_variant(dem_context* ctx, int flags) {
   if (!ctx) return DEM_ERR_INVALID;
-  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH | DEM_STEP_COPY))
+  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH | DEM_STEP_COPY | DEM_STEP_WIDE))
     return fail(ctx, DEM_ERR_INVALID, "unknown step variant flags");
   if ((flags & DEM_STEP_PREFETCH) && (flags & DEM_STEP_BATCH))
     return fail(ctx, DEM_ERR_INVALID, "DEM_STEP_PREFETCH and DEM_STEP_BATCH exclude each other");
   ctx->fuse = (flags & DEM_STEP_FUSED) != 0;
   ctx->cmode = (flags & DEM_STEP_PREFETCH) ? 1 : ((flags & DEM_STEP_BATCH) ? 2 : 0);
   ctx->copy = (flags & DEM_STEP_COPY) != 0;
+  ctx->wide = (flags & DEM_STEP_WIDE) != 0;
   ctx->generation++;  // captured graphs follow the choice
   return DEM_OK;
 }
@@ -1251,7 +1256,7 @@ int dem_collide_controlled(dem_context* ctx, const float* pos, const float* vel,
   co.react_m = momentVec;
   DEM_HIP(ctx, dem::launch_collide(ctx->c, ctx->stream, ctx->bs_collide, (const float2*)pos, (const float2*)vel,
                                    (float2*)acc, omega, alpha, type, nullptr, cellStart, cellEnd, pressure,
-                                   numParticles, false, &co),
+                                   numParticles, collide_mode(ctx) & kWideMode, &co),
           "collide kernel");
   DEM_HIP(ctx, dem::launch_control_finish(ctx->stream, co, blockX, blockY, blockM, numParticles,
                                           ctx->d_stage_reaction, ctx->d_stage_control, false),
@@ -1353,7 +1358,8 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   const uint32_t* gat = inplace ? nullptr : si;
   auto collide = [&](int b) {
     return dem::launch_collide(c, s, b, S_pos, D_vel, m->acc, D_omega, m->alpha, m->type[t], S_radius,
-                               m->cell_start, m->cell_end, m->pressure, n, pf, nullptr, gat);
+                               m->cell_start, m->cell_end, m->pressure, n, pf | (ctx->wide ? kWideMode : 0),
+                               nullptr, gat);
   };
   auto integrate = [&](int b) {
     return dem::launch_integrate_to(c, s, b, S_pos, D_vel, D_theta, D_omega, tpos, tvel, ttheta, tomega, m->acc,

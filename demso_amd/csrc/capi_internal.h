@@ -29,11 +29,12 @@ struct dem_context {
   uint32_t* d_copy_tmp = nullptr;
   int bs_hash = 256, bs_reorder = 256, bs_collide = 128, bs_integrate = 256;
   bool fuse = false;      // owned simulation: collide + integrate as one kernel (chosen by dem_sim_autotune)
-  int cmode = 0;  // collide's candidate loads: 0 one at a time, 1 one ahead, 2 first three of every row
+  int cmode = 0;  // collide's candidate loads: 0 one at a time, 1 one ahead, 2 first of every row
                  // at once (chosen by dem_sim_autotune)
   // owned simulation: re-sorted steps reorder the whole state into the other
   // buffer (the layout of radix-sorted steps) instead of re-sorting it in place
   bool copy = false;
+  bool wide = false;  // DEM_STEP_WIDE: collide with 64-bit addresses
   uint64_t generation = 1;  // bumped by every parameter change
   // Controlled particle of the owned simulations: host copy of the last
   // state set, and the device state (moved by every step) followed by the

@@ -39,7 +39,7 @@ struct StepConst {
   uint32_t attract;     // any material has a non-zero attraction coefficient
   uint32_t num_types;
   uint32_t num_instances;
-  uint32_t pad;
+  uint32_t cells;       // cells of all instances: every cell index is below it
   const Material* mat;
   const Instance* inst;
   // Per ordered pair of materials (row tA * num_types + tB): the series
@@ -160,6 +160,10 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
                           const float2* o_vel, const float* o_theta, const float* o_omega,
                           const uint32_t* o_id, const uint32_t* o_type, const float* o_radius,
                           uint32_t n, uint32_t num_cells, bool full = true);
+// launch_collide's mode: the candidate mode (0 one at a time, 1 one ahead, 2
+// the first of every row at once), plus kWideMode to force 64-bit addressing
+// (the plain mode), which states over 2^29 particles or 2^30 cells get anyway.
+constexpr int kWideMode = 4;
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,

@@ -266,11 +266,24 @@ __device__ __forceinline__ void attract_force(const PairCtx& q, float rx, float 
 // Velocity and spin of the sorted particles: vel[j], omega[j], or with a
 // gather index vel[gat[j]], omega[gat[j]] (the previous step's arrays,
 // before the sort).
+// Element i of an array.  kOff32: addressed by a 32-bit byte offset from the
+// array's base, which the loads take as a scalar base plus one 32-bit lane
+// register (64-bit address arithmetic per load otherwise); only for arrays
+// of at most 4 GiB, which the launcher checks.
+template <bool kOff32, class T>
+__device__ __forceinline__ const T& ld(const T* __restrict__ base, uint32_t i) {
+  if constexpr (kOff32)
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+  else
+    return base[i];
+}
+
 struct Dyn {
   const float2* __restrict__ vel;
   const float* __restrict__ omega;
   const uint32_t* __restrict__ gat;
-  __device__ __forceinline__ uint32_t at(uint32_t j) const { return gat ? gat[j] : j; }
+  template <bool kOff32>
+  __device__ __forceinline__ uint32_t at(uint32_t j) const { return gat ? ld<kOff32>(gat, j) : j; }
 };
 
 // A neighbour candidate as the pair law reads it.  With kDyn its velocity,
@@ -308,7 +321,7 @@ struct RowSum {
   }
 };
 
-template <bool kRadius, bool kOne, bool kDyn>
+template <bool kRadius, bool kOne, bool kDyn, bool kO>
 __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
                                              const Cand& b, const Dyn& d, const uint32_t* __restrict__ type,
                                              RowSum& rsum, float& fx, float& fy, float& moment, float& pres) {
@@ -324,7 +337,7 @@ __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& 
   const float dist = sqrtf(d2);
   const float rB = b.r;
   uint32_t tB = b.t;
-  if (kRadius && !kOne && !kDyn) tB = clamp_idx(type[j], c.num_types);
+  if (kRadius && !kOne && !kDyn) tB = clamp_idx(ld<kO>(type, j), c.num_types);
   if (dist < R) {
     float ffx, ffy;
     float2 vB;
@@ -333,9 +346,9 @@ __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& 
       vB = b.v;
       wB = b.w;
     } else {
-      const uint32_t jv = d.at(j);
-      vB = d.vel[jv];
-      wB = d.omega[jv];
+      const uint32_t jv = d.at<kO>(j);
+      vB = ld<kO>(d.vel, jv);
+      wB = ld<kO>(d.omega, jv);
     }
     contact_force(c, q, rx, ry, dist, R, rB, tB, vB, wB, ffx, ffy, moment, pres);
     cfx += ffx;
@@ -384,33 +397,33 @@ __device__ __forceinline__ void control_pair(const StepConst& c, const PairCtx& 
 // Candidate j's position and radius (and, without a radius array, its
 // material row), plus with kDyn its velocity, spin and material row.
 // (kOne: every particle has material row 0, so no type is read.)
-template <bool kRadius, bool kOne, bool kDyn>
+template <bool kRadius, bool kOne, bool kDyn, bool kO>
 __device__ __forceinline__ void candidate_load(const StepConst& c, uint32_t j, const float2* __restrict__ pos,
                                                const uint32_t* __restrict__ type, const float* __restrict__ radius,
                                                const Dyn& d, Cand& b) {
-  b.p = pos[j];
+  b.p = ld<kO>(pos, j);
   if (kRadius) {
-    b.r = radius[j];
-    b.t = kDyn && !kOne ? clamp_idx(type[j], c.num_types) : 0u;
+    b.r = ld<kO>(radius, j);
+    b.t = kDyn && !kOne ? clamp_idx(ld<kO>(type, j), c.num_types) : 0u;
   } else {
-    b.t = kOne ? 0u : clamp_idx(type[j], c.num_types);
+    b.t = kOne ? 0u : clamp_idx(ld<kO>(type, j), c.num_types);
     b.r = c.mat[b.t].radius;
   }
   if (kDyn) {
-    const uint32_t jv = d.at(j);
-    b.v = d.vel[jv];
-    b.w = d.omega[jv];
+    const uint32_t jv = d.at<kO>(j);
+    b.v = ld<kO>(d.vel, jv);
+    b.w = ld<kO>(d.omega, jv);
   }
 }
 
-template <bool kRadius, bool kOne, bool kDyn>
+template <bool kRadius, bool kOne, bool kDyn, bool kO>
 __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
                                           const float2* __restrict__ pos, const Dyn& d,
                                           const uint32_t* __restrict__ type, const float* __restrict__ radius,
                                           RowSum& rsum, float& fx, float& fy, float& moment, float& pres) {
   Cand b;
-  candidate_load<kRadius, kOne, kDyn>(c, j, pos, type, radius, d, b);
-  candidate_at<kRadius, kOne, kDyn>(c, q, j, self, b, d, type, rsum, fx, fy, moment, pres);
+  candidate_load<kRadius, kOne, kDyn, kO>(c, j, pos, type, radius, d, b);
+  candidate_at<kRadius, kOne, kDyn, kO>(c, q, j, self, b, d, type, rsum, fx, fy, moment, pres);
 }
 
 // Forces on sorted particle i from its 3x3 cell neighbourhood (the body of
@@ -440,7 +453,7 @@ constexpr int kBatchN = 1;
 // instance with several rows, or an ensemble of instances.
 constexpr int kOneMat = 0, kManyMat = 1, kInstances = 2;
 
-template <bool kRadius, int kMat, int kMode, bool kControl>
+template <bool kRadius, int kMat, int kMode, bool kControl, bool kO>
 __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i, const float2* __restrict__ pos,
                                                  const Dyn& d, const uint32_t* __restrict__ type,
                                                  const float* __restrict__ radius,
@@ -460,7 +473,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   own.r = q.rA;
   q.p = pos[i];
   {
-    const uint32_t iv = d.at(i);
+    const uint32_t iv = d.at<kO>(i);
     q.v = d.vel[iv];
     q.w = d.omega[iv];
   }
@@ -507,8 +520,8 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       // (a column or row outside it reads the centre column or row 0) and
       // then masked: no branch per load.
       const uint32_t a0 = row + (vx0 ? x0 : x1), a1 = row + x1, a2 = row + (vx2 ? x2 : x1);
-      const uint32_t s0r = cell_start[a0], s1r = cell_start[a1], s2r = cell_start[a2];
-      const uint32_t e0r = cell_end[a0], e1r = cell_end[a1], e2r = cell_end[a2];
+      const uint32_t s0r = ld<kO>(cell_start, a0), s1r = ld<kO>(cell_start, a1), s2r = ld<kO>(cell_start, a2);
+      const uint32_t e0r = ld<kO>(cell_end, a0), e1r = ld<kO>(cell_end, a1), e2r = ld<kO>(cell_end, a2);
       const uint32_t s0 = ok && vx0 ? s0r : kEmpty;
       const uint32_t s1 = ok ? s1r : kEmpty;
       const uint32_t s2 = ok && vx2 ? s2r : kEmpty;
@@ -529,7 +542,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
 #pragma unroll
         for (int k = 0; k < kBatchN; ++k) {
           const uint32_t j = rs[y] + (uint32_t)k;
-          candidate_load<kRadius, kOne, false>(c, j < re[y] ? j : 0u, pos, type, radius, d, cb[y][k]);
+          candidate_load<kRadius, kOne, false, kO>(c, j < re[y] ? j : 0u, pos, type, radius, d, cb[y][k]);
         }
     }
 #pragma unroll
@@ -543,11 +556,11 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
         // (pays off with many candidates per row, costs registers)
         uint32_t j = rs[y];
         Cand nx{};
-        if (j < end) candidate_load<kRadius, kOne, false>(c, j, pos, type, radius, d, nx);
+        if (j < end) candidate_load<kRadius, kOne, false, kO>(c, j, pos, type, radius, d, nx);
         for (; j < end; ++j) {
           const Cand b = nx;
-          if (j + 1u < end) candidate_load<kRadius, kOne, false>(c, j + 1u, pos, type, radius, d, nx);
-          candidate_at<kRadius, kOne, false>(c, q, j, j == i, b, d, type, rsum, fx, fy, moment, pres);
+          if (j + 1u < end) candidate_load<kRadius, kOne, false, kO>(c, j + 1u, pos, type, radius, d, nx);
+          candidate_at<kRadius, kOne, false, kO>(c, q, j, j == i, b, d, type, rsum, fx, fy, moment, pres);
         }
       } else {
         uint32_t j = rs[y];
@@ -555,10 +568,10 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
 #pragma unroll
           for (int k = 0; k < kBatchN; ++k, ++j)
             if (j < end)
-              candidate_at<kRadius, kOne, false>(c, q, j, j == i, cb[y][k], d, type, rsum, fx, fy, moment, pres);
+              candidate_at<kRadius, kOne, false, kO>(c, q, j, j == i, cb[y][k], d, type, rsum, fx, fy, moment, pres);
         }
         for (; j < end; ++j)
-          candidate<kRadius, kOne, kMode == kPlain>(c, q, j, j == i, pos, d, type, radius, rsum, fx, fy, moment,
+          candidate<kRadius, kOne, kMode == kPlain, kO>(c, q, j, j == i, pos, d, type, radius, rsum, fx, fy, moment,
                                                      pres);
       }
       fx += rsum.cfx;
@@ -579,7 +592,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
         rsum.b0 = kNoBoundary;
         rsum.b1 = kNoBoundary;
         for (uint32_t j = st; j < en; ++j)
-          candidate<kRadius, kOne, false>(c, q, j, j == i, pos, d, type, radius, rsum, fx, fy, moment, pres);
+          candidate<kRadius, kOne, false, kO>(c, q, j, j == i, pos, d, type, radius, rsum, fx, fy, moment, pres);
         fx += rsum.cfx;
         fy += rsum.cfy;
       }
@@ -605,42 +618,46 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,       \
       float* __restrict__ pressure, ControlOut ctl, const uint32_t* __restrict__ gat, uint32_t n
 
-template <bool kRadius, int kMat, int kMode, bool kControl>
+template <bool kRadius, int kMat, int kMode, bool kControl, bool kO>
 __device__ __forceinline__ void collide_kernel_body(DEM_COLLIDE_PARAMS) {
   const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Own own;
   float fx, fy, moment, pres;
   const Dyn d{vel, omega, gat};
-  collide_particle<kRadius, kMat, kMode, kControl>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
-                                                         ctl, fx, fy, moment, pres);
+  collide_particle<kRadius, kMat, kMode, kControl, kO>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
+                                                       ctl, fx, fy, moment, pres);
   acc[i] = make_float2(fx / own.mass, fy / own.mass);
   alpha[i] = moment / own.inertia;
   pressure[i] = pres;
 }
 
-template <bool kRadius, int kMat, int kMode, bool kControl>
+template <bool kRadius, int kMat, int kMode, bool kControl, bool kO>
 __global__ __launch_bounds__(256) void k_collide(DEM_COLLIDE_PARAMS) {
-  collide_kernel_body<kRadius, kMat, kMode, kControl>(c, pos, vel, acc, omega, alpha, type, radius, cell_start,
-                                                      cell_end, pressure, ctl, gat, n);
+  collide_kernel_body<kRadius, kMat, kMode, kControl, kO>(c, pos, vel, acc, omega, alpha, type, radius, cell_start,
+                                                          cell_end, pressure, ctl, gat, n);
 }
 
-// The one-material batched mode needs 65 VGPRs on its own; held to 64 (8
+// The one-material batched mode needed 65 VGPRs with 64-bit addresses; held to 64 (8
 // waves per SIMD) it does not spill.  The other batched layouts would spill
 // 12-19 registers under the same bound, so they keep k_collide.
 template <bool kRadius>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_collide_batch(DEM_COLLIDE_PARAMS) {
-  collide_kernel_body<kRadius, kOneMat, kBatch, false>(c, pos, vel, acc, omega, alpha, type, radius, cell_start,
-                                                       cell_end, pressure, ctl, gat, n);
+  collide_kernel_body<kRadius, kOneMat, kBatch, false, true>(c, pos, vel, acc, omega, alpha, type, radius,
+                                                             cell_start, cell_end, pressure, ctl, gat, n);
 }
 #undef DEM_COLLIDE_PARAMS
 
-template <bool kRadius, int kMat, int kMode, bool kControl>
+// kO (32-bit offsets) for states whose arrays stay under 4 GiB; larger ones
+// (or DEM_STEP_WIDE) run the plain mode with 64-bit addresses.
+template <bool kRadius, int kMat, int kMode, bool kControl, bool kO>
 constexpr auto collide_kernel() {
-  if constexpr (kMat == kOneMat && kMode == kBatch && !kControl)
+  if constexpr (!kO)
+    return &k_collide<kRadius, kMat, kPlain, kControl, false>;
+  else if constexpr (kMat == kOneMat && kMode == kBatch && !kControl)
     return &k_collide_batch<kRadius>;
   else
-    return &k_collide<kRadius, kMat, kMode, kControl>;
+    return &k_collide<kRadius, kMat, kMode, kControl, true>;
 }
 
 // ------------------------------------------------------------- integrate --
@@ -811,7 +828,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   Own own;
   float fx, fy, moment, pres;
   const Dyn d{vel_in, omega_in, nullptr};
-  collide_particle<kRadius, kMat, kMode, kControl>(c, i, pos_in, d, type, radius, cell_start, cell_end, n, own,
+  collide_particle<kRadius, kMat, kMode, kControl, false>(c, i, pos_in, d, type, radius, cell_start, cell_end, n, own,
                                                          ctl, fx, fy, moment, pres);
   const float2 a = make_float2(fx / own.mass, fy / own.mass);
   const float al = moment / own.inertia;
@@ -904,9 +921,19 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
                           const ControlOut* ctl, const uint32_t* gat) {
   if (n == 0) return hipSuccess;
   const ControlOut co = ctl ? *ctl : ControlOut{};
-#define DEM_COLLIDE(R, M, P, X)                                                                        \
-  hipLaunchKernelGGL((collide_kernel<R, M, P, X>()), grid_for(n, block), dim3(block), 0, s, c, pos, vel, acc, \
-                     omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, n)
+  // 32-bit byte offsets reach every element while n * 8 B and cells * 4 B
+  // stay within 4 GiB
+  const bool o32 = !(mode & kWideMode) && n <= (1u << 29) && c.cells <= (1u << 30);
+  mode &= ~kWideMode;
+#define DEM_COLLIDE(R, M, P, X)                                                                             \
+  do {                                                                                                      \
+    if (o32)                                                                                                \
+      hipLaunchKernelGGL((collide_kernel<R, M, P, X, true>()), grid_for(n, block), dim3(block), 0, s, c, pos, \
+                         vel, acc, omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, n);   \
+    else                                                                                                    \
+      hipLaunchKernelGGL((collide_kernel<R, M, P, X, false>()), grid_for(n, block), dim3(block), 0, s, c,     \
+                         pos, vel, acc, omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, n); \
+  } while (0)
   DEM_DISPATCH(DEM_COLLIDE);
 #undef DEM_COLLIDE
   return hipGetLastError();

@@ -273,6 +273,11 @@ int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
  * ranges around them); DEM_STEP_COPY makes them reorder the whole state
  * into the other buffer instead, as radix-sorted steps do.  Same results. */
 #define DEM_STEP_COPY 0x8
+/* Collide normally addresses its arrays by 32-bit offsets while they stay
+ * under 4 GiB (up to 2^29 particles and 2^30 cells) and with 64-bit
+ * addresses and one candidate at a time beyond; DEM_STEP_WIDE makes it use
+ * the latter at every size.  Same results. */
+#define DEM_STEP_WIDE 0x10
 int dem_set_step_variant(dem_context* ctx, int flags);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);

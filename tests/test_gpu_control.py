@@ -86,18 +86,20 @@ def pile_reference(oracle):
 
 
 @pytest.mark.parametrize("variant", [(False, False, 1), (True, False, 1), (False, False, 2), (True, True, 2),
-                                     (False, True, 0)])
+                                     (False, True, 0), (False, True, 2, True)])
 def test_owned_sim_with_controlled_particle(pile_reference, variant):
     """pilha.dsml's pile with its MOVE particle driven through it: state,
     controlled particle and reaction bit for bit with the oracle after 1400
     steps, for split and fused collide + integrate, radix sort and re-sort,
-    eager steps and graph replays."""
-    fused, graph, sort_mode = variant
+    eager steps and graph replays, 32-bit and 64-bit addressing in
+    collide."""
+    fused, graph, sort_mode = variant[:3]
+    wide = len(variant) > 3 and variant[3]
     o, cs, red = pile_reference
     s, c = controlled_pile()
     ctx = api.Context(0)
     ctx.load_scene(s)
-    ctx.set_step_variant(fused, False)
+    ctx.set_step_variant(fused, False, wide=wide)
     ctx.set_controlled(pos=c["pos"], type=c["type"], move=c["move"], wrap_below_y=c["wrap_below_y"],
                        reset_y=c["reset_y"], reset_x=c["reset_x"])
     sim = api.Simulation(ctx, s)

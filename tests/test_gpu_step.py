@@ -89,13 +89,15 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
 
 @pytest.mark.parametrize("variant", [(True, False), (False, True), (True, True), (False, False, True),
                                      (False, False, False, True), (False, False, True, True),
-                                     (True, False, True)])
+                                     (True, False, True), (False, False, False, False, True),
+                                     (False, True, False, True, True)])
 @pytest.mark.parametrize("name", ["pack_1024", "ensemble_5", "pile_wrap_attract"])
 def test_step_variants_equal(gpu_ctx, name, variant):
     """collide + integrate as one kernel or two, and each way of loading
-    collide's candidates (one at a time, one ahead, three a row at once),
-    re-sorted steps in place or into the other buffer (copy): the same
-    state and cell ranges, bit for bit."""
+    collide's candidates (one at a time, one ahead, the first of every row
+    at once), re-sorted steps in place or into the other buffer (copy), and
+    collide with 32-bit offsets or 64-bit addresses (wide): the same state
+    and cell ranges, bit for bit."""
     make, steps = CASES[name]
     s = make()
     ctx2 = api.Context(0)
