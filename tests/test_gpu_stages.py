@@ -143,8 +143,14 @@ def test_reorder(gpu_ctx, oracle, name):
         assert bits_equal(host(out.radius), st_ref["radius"])
 
 
+COLLIDE_VARIANTS = {"plain": {}, "prefetch": {"prefetch": True}, "batch": {"batch": True}, "wide": {"wide": True}}
+
+
+@pytest.mark.parametrize("variant", list(COLLIDE_VARIANTS))
 @pytest.mark.parametrize("name", list(SCENES))
-def test_collide(gpu_ctx, oracle, name):
+def test_collide(gpu_ctx, oracle, name, variant):
+    """The drop-in collide stage in each candidate-loading mode and with
+    64-bit addressing: acc, alpha and pressure equal the oracle's bit for bit."""
     s = SCENES[name]
     gpu_ctx.load_scene(s)
     system, h, i, hs, is_, cs, ce, st = oracle_sorted(oracle, s)
@@ -152,15 +158,17 @@ def test_collide(gpu_ctx, oracle, name):
     assert np.count_nonzero(pres_ref) > 0 or name == "settling128"
     n, C = s.n, s.total_cells
     acc, alpha, pres = dev(np.zeros((n, 2), np.float32)), dev(np.zeros(n, np.float32)), dev(np.full(n, 7.0, np.float32))
-    api.collide(gpu_ctx, dev(st["pos"]), dev(st["vel"]), acc, dev(st["omega"]), alpha, dev(st["type"]), dev(cs),
-                dev(ce), n, C, pres, radius=dev(st["radius"]) if st["radius"] is not None else None)
-    gpu_ctx.synchronize()
+    gpu_ctx.set_step_variant(**COLLIDE_VARIANTS[variant])
+    try:
+        api.collide(gpu_ctx, dev(st["pos"]), dev(st["vel"]), acc, dev(st["omega"]), alpha, dev(st["type"]), dev(cs),
+                    dev(ce), n, C, pres, radius=dev(st["radius"]) if st["radius"] is not None else None)
+        gpu_ctx.synchronize()
+    finally:
+        gpu_ctx.set_step_variant()
     a, al, p = host(acc), host(alpha), host(pres)
-    assert rel_err(a, acc_ref) <= TOL
-    assert rel_err(al, alpha_ref) <= TOL if np.abs(alpha_ref).max() > 0 else np.abs(al).max() == 0
-    assert rel_err(p, pres_ref) <= TOL if np.abs(pres_ref).max() > 0 else np.abs(p).max() == 0
-    print(f"{name}: bit-exact acc={bits_equal(a, acc_ref)} alpha={bits_equal(al, alpha_ref)} "
-          f"pressure={bits_equal(p, pres_ref)}")
+    assert bits_equal(a, acc_ref), rel_err(a, acc_ref)
+    assert bits_equal(al, alpha_ref), rel_err(al, alpha_ref)
+    assert bits_equal(p, pres_ref), rel_err(p, pres_ref)
 
 
 def integrate_inputs(s, seed=0):
