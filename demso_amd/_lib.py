@@ -143,6 +143,7 @@ PROTOTYPES = {
     "dem_sim_stage_times": (c_int, [c_void_p, c_uint32, POINTER(c_float)]),
     "dem_sim_autotune": (c_int, [c_void_p, POINTER(c_uint32)]),
     "dem_sim_sort_passes": (c_int, [c_void_p]),
+    "dem_sim_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_int)]),
     "dem_sim_set_sort_mode": (c_int, [c_void_p, c_int]),
     "dem_set_step_variant": (c_int, [c_void_p, c_int]),
     "dem_sim_mover_count": (c_int, [c_void_p, POINTER(c_uint32)]),
@@ -176,6 +177,7 @@ PROTOTYPES = {
     "dem_adj_create": (c_int, [c_void_p, c_uint32, POINTER(c_void_p)]),
     "dem_adj_destroy": (c_int, [c_void_p]),
     "dem_adj_forward": (c_int, [c_void_p, c_uint32]),
+    "dem_adj_generation": (c_int, [c_void_p, POINTER(ctypes.c_uint64)]),
     "dem_adj_backward": (c_int, [c_void_p, P, P, P, P, P]),
 }
 

@@ -607,6 +607,10 @@ struct dem_adjoint {
   // the slots hold the last segment as dem_adj_forward recorded it (until a
   // backward pass recomputes other segments into them)
   bool last_recorded = false;
+  // dem_adj_forward calls started so far, and the context's parameter
+  // generation the last complete forward ran under (0: none complete)
+  uint64_t generation = 0;
+  uint64_t ctx_gen = 0;
   // adjoint state (FP64)
   double2 *lx = nullptr, *lv = nullptr, *lxp = nullptr, *lvp = nullptr, *lF = nullptr;
   double *lw = nullptr, *lwp = nullptr, *lM = nullptr, *grad = nullptr;
@@ -731,8 +735,17 @@ int dem_adj_destroy(dem_adjoint* a) {
   return DEM_OK;
 }
 
+int dem_adj_generation(dem_adjoint* a, uint64_t* generation) {
+  if (!a || !generation) return DEM_ERR_INVALID;
+  *generation = a->generation;
+  return DEM_OK;
+}
+
 int dem_adj_forward(dem_adjoint* a, uint32_t steps) {
   if (!a) return DEM_ERR_INVALID;
+  ++a->generation;
+  a->steps = 0;
+  a->ctx_gen = 0;
   dem_sim* m = a->sim;
   dem_context* ctx = m->ctx;
   if (int r = sim_check(m)) return r;
@@ -766,6 +779,7 @@ int dem_adj_forward(dem_adjoint* a, uint32_t steps) {
   DEM_HIP(ctx, copy_state(m, a->final_state, true, s), "final state");
   a->steps = steps;
   a->last_recorded = steps > 0;
+  a->ctx_gen = ctx->generation;
   return DEM_OK;
 }
 
@@ -776,6 +790,8 @@ int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_ve
   dem_context* ctx = m->ctx;
   if (int r = sim_check(m)) return r;
   if (a->steps == 0) return fail(ctx, DEM_ERR_STATE, "dem_adj_forward has not run");
+  if (a->ctx_gen != ctx->generation)
+    return fail(ctx, DEM_ERR_STATE, "the context's parameters changed after dem_adj_forward; run it again");
   hipStream_t s = ctx->stream;
   const uint32_t n = m->n;
   const StepConst& c = ctx->c;

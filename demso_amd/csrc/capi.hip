@@ -1290,6 +1290,13 @@ int dem_sim_sort_passes(dem_sim* m) {
   return dem::sort_passes(m->ctx->key_bits);
 }
 
+int dem_sim_info(dem_sim* m, uint32_t* n, int* device) {
+  if (!m) return DEM_ERR_INVALID;
+  if (n) *n = m->n;
+  if (device) *device = m->ctx->device;
+  return DEM_OK;
+}
+
 int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   if (!m) return DEM_ERR_INVALID;
   dem_context* ctx = m->ctx;

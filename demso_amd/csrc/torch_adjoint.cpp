@@ -12,6 +12,7 @@
 // the null stream, as torch's default stream.
 #include <torch/extension.h>
 
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 
@@ -33,20 +34,36 @@ struct DemTrajectory : public Function<DemTrajectory> {
     auto* ctx = reinterpret_cast<dem_context*>(ctx_h);
     auto* sim = reinterpret_cast<dem_sim*>(sim_h);
     auto* adj = reinterpret_cast<dem_adjoint*>(adj_h);
+    TORCH_CHECK(ctx && sim && adj, "null context, sim or adjoint handle");
+    TORCH_CHECK(steps >= 0 && steps <= UINT32_MAX, "steps out of range");
+    uint32_t n = 0;
+    int device = -1;
+    check(dem_sim_info(sim, &n, &device), ctx, "dem_sim_info");
+    // every check happens before any launch: the kernels read pos0[id] for
+    // each of the sim's n particle ids
     TORCH_CHECK(pos0.is_cuda() && pos0.scalar_type() == torch::kFloat32 && pos0.dim() == 2 && pos0.size(1) == 2,
                 "pos0 must be an (n, 2) float32 GPU tensor");
+    TORCH_CHECK(pos0.size(0) == (int64_t)n, "pos0 has ", pos0.size(0), " rows; the simulation has ", n,
+                " particles");
+    TORCH_CHECK(pos0.get_device() == device, "pos0 is on cuda:", pos0.get_device(),
+                "; the simulation runs on cuda:", device);
     auto p = params.detach().to(torch::kCPU, torch::kFloat64).contiguous();
     TORCH_CHECK(p.dim() == 2 && p.size(1) == 4, "params must be (num_types, 4)");
     auto x0 = pos0.detach().contiguous();
     check(dem_set_contact_params(ctx, p.data_ptr<double>(), (uint32_t)p.size(0)), ctx, "dem_set_contact_params");
-    check(dem_sim_set_by_id(sim, x0.data_ptr<float>(), nullptr), sim ? ctx : nullptr, "dem_sim_set_by_id");
+    check(dem_sim_set_by_id(sim, x0.data_ptr<float>(), nullptr), ctx, "dem_sim_set_by_id");
     check(dem_adj_forward(adj, (uint32_t)steps), ctx, "dem_adj_forward");
+    uint64_t gen = 0;
+    check(dem_adj_generation(adj, &gen), ctx, "dem_adj_generation");
     auto out = torch::empty_like(x0);
     check(dem_sim_get_by_id(sim, out.data_ptr<float>(), nullptr), ctx, "dem_sim_get_by_id");
     check(dem_synchronize(ctx), ctx, "dem_synchronize");
     actx->saved_data["ctx"] = ctx_h;
     actx->saved_data["adj"] = adj_h;
     actx->saved_data["ntypes"] = (int64_t)p.size(0);
+    actx->saved_data["n"] = (int64_t)n;
+    actx->saved_data["device"] = (int64_t)device;
+    actx->saved_data["generation"] = (int64_t)gen;
     return out;
   }
 
@@ -54,7 +71,18 @@ struct DemTrajectory : public Function<DemTrajectory> {
     auto* ctx = reinterpret_cast<dem_context*>(actx->saved_data["ctx"].toInt());
     auto* adj = reinterpret_cast<dem_adjoint*>(actx->saved_data["adj"].toInt());
     const int64_t nt = actx->saved_data["ntypes"].toInt();
+    // the adjoint holds one trajectory (checkpoints, final state): the
+    // latest forward's.  A graph built by an earlier call cannot be
+    // differentiated through it any more.
+    uint64_t gen = 0;
+    check(dem_adj_generation(adj, &gen), ctx, "dem_adj_generation");
+    TORCH_CHECK((int64_t)gen == actx->saved_data["generation"].toInt(),
+                "this trajectory was replaced by a later call on the same Trajectory; call backward before "
+                "running it again, or use one Trajectory per differentiated run");
     auto g = grads[0].to(torch::kFloat32).contiguous();
+    TORCH_CHECK(g.is_cuda() && g.get_device() == actx->saved_data["device"].toInt() && g.dim() == 2 &&
+                    g.size(0) == actx->saved_data["n"].toInt() && g.size(1) == 2,
+                "the gradient of the final positions must be an (n, 2) tensor on the simulation's device");
     auto gp0 = torch::empty_like(g);
     auto gpar = torch::empty({nt, 4}, g.options().dtype(torch::kFloat64));
     check(dem_adj_backward(adj, g.data_ptr<float>(), nullptr, gp0.data_ptr<float>(), nullptr, gpar.data_ptr<double>()),

@@ -281,6 +281,10 @@ int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
 int dem_set_step_variant(dem_context* ctx, int flags);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);
+/* No counterpart in the reference (its particle count is a global,
+ * src/main.cuh:109).  The sim's particle count and the HIP device of its
+ * context, so callers can check array sizes before handing them over. */
+int dem_sim_info(dem_sim* sim, uint32_t* n, int* device);
 /* Current positions / velocities gathered by particle ID into device arrays
  * (n float2 each; either may be NULL). */
 int dem_sim_get_by_id(dem_sim* sim, float* pos_by_id, float* vel_by_id);
@@ -401,6 +405,10 @@ int dem_adj_create(dem_sim* sim, uint32_t checkpoint_every, dem_adjoint** out);
 int dem_adj_destroy(dem_adjoint* adj);
 /* Run `steps` steps from the sim's current state, keeping checkpoints. */
 int dem_adj_forward(dem_adjoint* adj, uint32_t steps);
+/* How many dem_adj_forward calls have started on this adjoint (successful
+ * or not).  dem_adj_backward differentiates the latest of them, so a caller
+ * that holds an earlier count knows its trajectory was replaced. */
+int dem_adj_generation(dem_adjoint* adj, uint64_t* generation);
 /* Given dL/dpos and dL/dvel of the final state (device, n float2 each,
  * indexed by particle ID; either may be NULL), write dL/dpos0, dL/dvel0 of
  * the state dem_adj_forward started from (device, by ID; either may be

@@ -134,6 +134,34 @@ def test_torch_autograd_function(gpu_ctx, weights):
     traj.close()
 
 
+def test_torch_function_rejects_bad_inputs_and_replaced_trajectories(gpu_ctx, weights):
+    """The autograd Function refuses initial positions whose row count is not
+    the simulation's before anything is launched, and refuses to
+    differentiate a trajectory that a later call on the same Trajectory has
+    replaced (its checkpoints are gone); the latest call still works."""
+    import torch
+
+    from demso_amd import torch_ext
+
+    s, W, _ = weights
+    traj = torch_ext.Trajectory(gpu_ctx, s, steps=8, checkpoint_every=4)
+    params = torch.tensor([[mm.kn, mm.kt, mm.cn, mm.mu] for mm in s.materials], dtype=torch.float64,
+                          requires_grad=True)
+    with pytest.raises(RuntimeError, match="rows"):
+        traj(torch.from_numpy(s.pos[:-1].copy()).cuda(), params)
+    with pytest.raises(RuntimeError, match="GPU tensor"):
+        traj(torch.from_numpy(s.pos.copy()), params)
+    a = torch.from_numpy(s.pos.copy()).cuda().requires_grad_(True)
+    b = torch.from_numpy(s.pos.copy()).cuda().requires_grad_(True)
+    out_a = traj(a, params)
+    out_b = traj(b, params)
+    with pytest.raises(RuntimeError, match="replaced"):
+        out_a.sum().backward()
+    out_b.sum().backward()
+    assert b.grad is not None and torch.isfinite(b.grad).all()
+    traj.close()
+
+
 def fd_subcase(gpu_ctx):
     """Config-3 finite-difference sub-case: 512 discs (32 x 16 lattice with
     config-0 distributions) settled for 2000 steps, then 200 steps with
