@@ -15,6 +15,7 @@ renderer.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from ctypes import byref, c_float, c_uint32, c_void_p
 from typing import Optional
 
@@ -108,9 +109,14 @@ class Context:
         self.handle = h.value
         self.device = device
         self.box = None  # (Lx, Ly) of the last scene loaded
+        # simulations made on this context: destroyed before it, so that no
+        # sim outlives the C context it points to
+        self._sims = weakref.WeakSet()
 
     def close(self) -> None:
         if self.handle:
+            for sim in list(self._sims):
+                sim.close()
             self.lib.dem_destroy(self.handle)
             self.handle = None
 
@@ -440,10 +446,14 @@ class Simulation:
         h = c_void_p()
         ctx._check(self.lib.dem_sim_create(ctx.handle, self.n, self.cells, int(self.with_radius), byref(h)))
         self.handle = h.value
+        self._adjoints = weakref.WeakSet()  # destroyed before this sim
+        ctx._sims.add(self)
         self.upload(scene)
 
     def close(self) -> None:
         if getattr(self, "handle", None):
+            for adj in list(getattr(self, "_adjoints", ())):
+                adj.close()
             self.lib.dem_sim_destroy(self.handle)
             self.handle = None
 
@@ -554,6 +564,7 @@ class Adjoint:
         h = c_void_p()
         sim.ctx._check(self.lib.dem_adj_create(sim.handle, checkpoint_every, byref(h)))
         self.handle = h.value
+        sim._adjoints.add(self)
 
     def close(self) -> None:
         if getattr(self, "handle", None):
