@@ -124,6 +124,7 @@ PROTOTYPES = {
     "dem_reserve": (c_int, [c_void_p, c_uint32]),
     "dem_calc_hash": (c_int, [c_void_p, P, P, P, c_uint32]),
     "dem_sort_particles": (c_int, [c_void_p, P, P, c_uint32]),
+    "dem_sort_particles_noalloc": (c_int, [c_void_p, P, P, c_uint32]),
     "dem_reorder_data_and_find_cell_start": (
         c_int,
         [c_void_p] + [P] * 17 + [c_uint32, c_uint32],

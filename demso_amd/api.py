@@ -277,6 +277,14 @@ def sortParticles(ctx: Context, gridParticleHash, gridParticleIndex, numParticle
                                           numParticles))
 
 
+def sortParticles_noalloc(ctx: Context, gridParticleHash, gridParticleIndex, numParticles: int) -> None:
+    """sortParticles without device allocation: the library's fixed per-GPU
+    scratch (at most DEM_SORT_NOALLOC_MAX keys; calls overlapping on one GPU
+    race)."""
+    ctx._check(ctx.lib.dem_sort_particles_noalloc(ctx.handle, _ptr(gridParticleHash), _ptr(gridParticleIndex),
+                                                  numParticles))
+
+
 def reorderDataAndFindCellStart(ctx: Context, cellStart, cellEnd, sortedPos, sortedVel, sortedTheta,
                                 sortedOmega, sortedID, sortedLoc, sortedType, gridParticleHash,
                                 gridParticleIndex, oldPos, oldVel, oldTheta, oldOmega, oldID, oldType,

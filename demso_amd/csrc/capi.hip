@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -238,6 +239,10 @@ const char* dem_last_error(const dem_context* ctx) {
   return ctx ? ctx->err.c_str() : g_create_error.c_str();
 }
 
+namespace {
+void resolve_scratch(int device);
+}
+
 int dem_create(dem_context** out, int device, void* hip_stream) {
   if (!out) return fail(nullptr, DEM_ERR_INVALID, "out is NULL");
   *out = nullptr;
@@ -253,6 +258,9 @@ int dem_create(dem_context** out, int device, void* hip_stream) {
     delete ctx;
     return fail(nullptr, DEM_ERR_NO_DEVICE, "hipSetDevice failed");
   }
+  // the no-allocation sort's scratch address on this device (its error, if
+  // any, is reported by dem_sort_particles_noalloc)
+  resolve_scratch(device);
   *out = ctx;
   return DEM_OK;
 }
@@ -406,6 +414,59 @@ int dem_calc_hash_ex(dem_context* ctx, const float* pos, const uint32_t* type,
   return DEM_OK;
 }
 
+namespace {
+
+// Sorts with the given scratch and leaves the result in the caller's arrays.
+int sort_into_place(dem_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t n, const dem::SortScratch& scratch) {
+  // Keys come from hashing, so they are < total_cells; without parameters
+  // the full 32 bits are sorted.
+  const uint32_t bits = ctx->params_set ? ctx->key_bits : 32u;
+  bool in_alt = false;
+  DEM_HIP(ctx, dem::launch_sort(ctx->stream, keys, vals, n, bits, scratch, &in_alt), "sort kernels");
+  if (in_alt) {
+    hipLaunchKernelGGL(k_copy2, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, scratch.keys_alt,
+                       scratch.vals_alt, keys, vals, n);
+    DEM_HIP(ctx, hipGetLastError(), "sort copy-back");
+  }
+  return DEM_OK;
+}
+
+// ---- scratch of dem_sort_particles_noalloc ----------------------------------
+// A fixed array per GPU: the sort's second key and value arrays
+// (kNoAllocKeys words each), its per-tile digit counts (256 per tile of
+// 4096 keys) and its 256 digit totals: 33 MiB + 1 KiB, held by every GPU
+// the library runs on until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address); it uses the per-device address hipGetSymbolAddress returns.
+constexpr uint32_t kNoAllocKeys = DEM_SORT_NOALLOC_MAX;
+constexpr size_t kNoAllocCounts = 256 * (((size_t)kNoAllocKeys + 4095) / 4096);
+constexpr size_t kNoAllocWords = 2 * (size_t)kNoAllocKeys + kNoAllocCounts + 256;
+static_assert(kNoAllocKeys % 64 == 0, "sub-arrays stay 256-byte aligned");
+
+__device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Once per device, with that device current: the device address of this
+// GPU's copy of `scratch`.  Called from dem_create.
+void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && dem::sort_block_count_elems(kNoAllocKeys) != kNoAllocCounts) e = hipErrorInvalidValue;
+  g_scratch_ptr[device] = e == hipSuccess ? static_cast<uint32_t*>(ptr) : nullptr;
+  g_scratch_err[device] = e;
+  g_scratch_done[device] = true;
+}
+
+}  // namespace
+
 int dem_sort_particles(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
                        uint32_t numParticles) {
   if (!ctx) return DEM_ERR_INVALID;
@@ -413,18 +474,38 @@ int dem_sort_particles(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* g
   if (!gridParticleHash || !gridParticleIndex) return fail(ctx, DEM_ERR_INVALID, "NULL array");
   if (int r = ensure_device(ctx)) return r;
   if (int r = alloc_sort(ctx, ctx->sort, numParticles)) return r;
-  // Keys come from hashing, so they are < total_cells; without parameters
-  // the full 32 bits are sorted.
-  const uint32_t bits = ctx->params_set ? ctx->key_bits : 32u;
-  bool in_alt = false;
-  DEM_HIP(ctx, dem::launch_sort(ctx->stream, gridParticleHash, gridParticleIndex, numParticles, bits, ctx->sort, &in_alt),
-          "sort kernels");
-  if (in_alt) {
-    hipLaunchKernelGGL(k_copy2, dim3((numParticles + 255) / 256), dim3(256), 0, ctx->stream, ctx->sort.keys_alt,
-                       ctx->sort.vals_alt, gridParticleHash, gridParticleIndex, numParticles);
-    DEM_HIP(ctx, hipGetLastError(), "sort copy-back");
+  return sort_into_place(ctx, gridParticleHash, gridParticleIndex, numParticles, ctx->sort);
+}
+
+int dem_sort_particles_noalloc(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
+                               uint32_t numParticles) {
+  if (!ctx) return DEM_ERR_INVALID;
+  if (numParticles > kNoAllocKeys)
+    return fail(ctx, DEM_ERR_INVALID,
+                "dem_sort_particles_noalloc sorts at most DEM_SORT_NOALLOC_MAX (4194304) keys; "
+                "use dem_sort_particles");
+  if (numParticles <= 1) return DEM_OK;
+  if (!gridParticleHash || !gridParticleIndex) return fail(ctx, DEM_ERR_INVALID, "NULL array");
+  if (int r = ensure_device(ctx)) return r;
+  const int dev = ctx->device;
+  uint32_t* base = nullptr;
+  hipError_t err = hipErrorInvalidDevice;
+  if (dev >= 0 && dev < kMaxDevices) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = g_scratch_ptr[dev];
+    err = g_scratch_done[dev] ? g_scratch_err[dev] : hipErrorNotInitialized;
   }
-  return DEM_OK;
+  if (!base) return hip_fail(ctx, err, "scratch address of dem_sort_particles_noalloc");
+  dem::SortScratch sc;
+  sc.keys_alt = base;
+  sc.vals_alt = base + kNoAllocKeys;
+  sc.block_counts = base + 2 * (size_t)kNoAllocKeys;
+  sc.row_totals = sc.block_counts + kNoAllocCounts;
+  sc.capacity = kNoAllocKeys;
+  // every scratch word a pass reads was written earlier in that pass (digit
+  // counts by the upsweep, totals by the row scan, keys and values by the
+  // previous pass's downsweep), so nothing needs clearing between calls
+  return sort_into_place(ctx, gridParticleHash, gridParticleIndex, numParticles, sc);
 }
 
 int dem_reorder_data_and_find_cell_start(

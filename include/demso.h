@@ -161,6 +161,19 @@ int dem_calc_hash(dem_context* ctx, const float* pos, uint32_t* gridParticleInde
 int dem_sort_particles(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
                        uint32_t numParticles);
 
+/* No counterpart in the reference.  dem_sort_particles for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): the same sort and result, with
+ * its scratch in a fixed per-GPU array of the library (33 MiB, held until
+ * the process ends) instead of buffers the context allocates.  Sorts at
+ * most DEM_SORT_NOALLOC_MAX keys (larger counts fail with DEM_ERR_INVALID
+ * before anything is launched).  All calls on one GPU share that array:
+ * calls that overlap on one GPU (other streams, other contexts, other host
+ * threads) race and must be serialised by the caller. */
+#define DEM_SORT_NOALLOC_MAX 4194304u
+int dem_sort_particles_noalloc(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
+                               uint32_t numParticles);
+
 /* reorderDataAndFindCellStart (src/functions.cu:347-393; kernel
  * src/particles_kernel.cuh:156-233): cellStart := EMPTY for all numCells,
  * then cellStart/cellEnd of every occupied cell, the sorted copies of the
