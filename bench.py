@@ -248,6 +248,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--collide-mode", choices=["plain", "prefetch", "batch"], default=None,
                     help="with --no-autotune: collide's candidate mode (default: one candidate at a time)")
+    ap.add_argument("--no-one-by-one", action="store_true",
+                    help="ensemble workload: skip timing the instances one by one (for profiler runs)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="finish with a plain device copy of this many MiB (reference for HBM counters)")
     args = ap.parse_args()
@@ -321,7 +323,7 @@ def main() -> int:
         extra["calibration"] = calibration_run(ctx, scene, args.adj_steps or 1000, args.iters, 100, 2000)
 
     ensemble = None
-    if args.workload == "ensemble":
+    if args.workload == "ensemble" and not args.no_one_by_one:
         sim.close()
         single = one_by_one(ctx, scene, max(10, args.steps // 4))
         ensemble = {"one_by_one": single, "speedup_vs_one_by_one": value / single["value"],

@@ -33,7 +33,9 @@ def max_contact_diameter(scene: Scene, k: Optional[int] = None) -> float:
 
 def with_cell_multiplier(scene: Scene, mult: float) -> Scene:
     """Copy of `scene` whose grids use cells of mult x (2 rmax) (>= 1), each
-    instance keeping its box; keys are re-based instance after instance."""
+    instance keeping its box and covering at least the area its grid
+    covered before (the hashing extent, which may reach above an open top);
+    keys are re-based instance after instance."""
     if mult < 1.0:
         raise ValueError("the cell multiplier must be >= 1 (cells must hold the contact range)")
     out = copy.copy(scene)
@@ -42,8 +44,11 @@ def with_cell_multiplier(scene: Scene, mult: float) -> Scene:
     for k, g in enumerate(scene.instances):
         d = max_contact_diameter(scene, k if len(scene.instances) > 1 else None)
         cell = mult * d
-        gx = max(1, int(math.ceil(g.box[0] / cell - 1e-9)))
-        gy = max(1, int(math.ceil(g.box[1] / cell - 1e-9)))
+        # cover the old hashing extent, not just the walls: discs above an
+        # open top would otherwise all clamp into the top cell row
+        span = (max(g.box[0], g.extent[0]), max(g.box[1], g.extent[1]))
+        gx = max(1, int(math.ceil(span[0] / cell - 1e-9)))
+        gy = max(1, int(math.ceil(span[1] / cell - 1e-9)))
         insts.append(Instance(box=g.box, extent=(gx * cell, gy * cell), grid=(gx, gy), cell_base=base))
         base += gx * gy
     out.instances = insts

@@ -37,3 +37,22 @@ def test_instance_scene():
     assert sub.materials[0].kn == s.materials[1].kn and sub.materials[0].instance == 0
     assert sub.instances[0].cell_base == 0 and sub.instances[0].grid == s.instances[1].grid
     assert np.array_equal(sub.id, np.arange(20))
+
+
+@pytest.mark.parametrize("mult", [1.0, 1.5])
+def test_regrid_keeps_discs_above_an_open_top_inside_the_grid(mult):
+    """The ensemble's lattices stand above their boxes' open tops; a re-grid
+    must still cover them, or every disc above the box would be clamped
+    into the top cell row (a crowded row that multiplies collide's work)."""
+    s = sc.ensemble(instances=4, counts=[2000, 9000, 3000, 60000], seed=2)
+    t = tuning.with_cell_multiplier(s, mult)
+    for k, g in enumerate(t.instances):
+        a, b = int(s.segments[k]), int(s.segments[k + 1])
+        p = s.pos[a:b].astype(np.float64)
+        assert p[:, 1].max() > s.instances[k].box[1]          # the lattice does reach above the box
+        assert p[:, 0].max() < g.extent[0] and p[:, 1].max() < g.extent[1]
+        cell = g.extent[0] / g.grid[0]
+        cx = np.floor(p[:, 0] / cell).astype(np.int64)
+        cy = np.floor(p[:, 1] / cell).astype(np.int64)
+        per_cell = np.bincount(cy * g.grid[0] + cx)
+        assert per_cell.max() <= 2 * mult * mult + 2
