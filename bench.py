@@ -154,6 +154,39 @@ def adjoint_cost(ctx, scene, steps: int, K: int, settle_steps: int) -> dict:
     }
 
 
+def ensemble_speedup(ctx, steps: int = 50) -> dict:
+    """The batched-ensemble workload (256 column-collapse instances, 8.2M
+    discs, its own grids) in one launch per stage, against the same
+    instances run one after another with the plain single-system step."""
+    from demso_amd import api, tuning
+    from demso_amd import scene as sc
+
+    scene = sc.ensemble()
+    sim = api.Simulation(ctx, scene)
+    sim.step(1)
+    sim.autotune()
+    sec = tuning.time_graph_steps(sim, steps)
+    sim.close()
+    single = one_by_one(ctx, scene, max(10, steps // 2))
+    batched = scene.n / sec
+    return {"value": batched / single["value"], "batched": batched, "one_by_one": single["value"],
+            "unit": "particle-steps/s", "instances": len(scene.instances), "particles": scene.n,
+            "steps": steps}
+
+
+def adjoint_ratio(ctx) -> dict:
+    """Backward (hand-written adjoint with recomputation from checkpoints)
+    over the plain forward of the 65,536-disc calibration column, 2,000
+    steps, checkpoints every 100."""
+    from demso_amd import scene as sc
+
+    r = adjoint_cost(ctx, sc.calibration_column(seed=3, cols=256, rows=256), 2000, 100, 5000)
+    return {"value": r["backward_over_forward"],
+            "forward_plus_backward_over_forward": r["forward_plus_backward_over_forward"],
+            "plain_forward_s": r["plain_forward_s"], "backward_s": r["backward_s"], "particles": r["particles"],
+            "steps": r["steps"], "checkpoint_every": r["checkpoint_every"]}
+
+
 def calibration_run(ctx, scene, steps: int, iters: int, K: int, settle_steps: int) -> dict:
     """Batched calibration: every instance starts from (kn, mu, e) =
     (1e5, 0.5, 0.5) and Adam recovers its own drawn parameters from the
@@ -234,6 +267,77 @@ def cpu_baseline(scene, steps: int, threads: int, seconds: float) -> dict:
     }
 
 
+def fixed_step_bytes(n: int, cells: int) -> int:
+    """The fixed algorithmic figure of one full step, 212 B per particle and
+    16 B per cell (hash 16, sort 16, reorder 76, collide 40, integrate 64 per
+    particle; cellStart/cellEnd written once and read once per cell), FP32
+    state and 32-bit keys, whatever the step does to avoid that traffic."""
+    return 212 * n + 16 * cells
+
+
+def cells_collide_reads(scene, sorted_keys) -> int:
+    """Cells whose cellStart/cellEnd collide reads: the 3x3 neighbourhoods of
+    the occupied cells, inside each instance's grid (clamp mode)."""
+    import numpy as np
+
+    keys = np.unique(np.asarray(sorted_keys, np.int64))
+    base = np.array([g.cell_base for g in scene.instances], np.int64)
+    order = np.argsort(base)
+    k = order[np.searchsorted(base[order], keys, side="right") - 1]
+    G0 = np.array([g.grid[0] for g in scene.instances], np.int64)[k]
+    G1 = np.array([g.grid[1] for g in scene.instances], np.int64)[k]
+    b = base[k]
+    local = keys - b
+    cx, cy = local % G0, local // G0
+    out = []
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            x, y = cx + dx, cy + dy
+            ok = (x >= 0) & (x < G0) & (y >= 0) & (y < G1)
+            out.append((b + y * G0 + x)[ok])
+    return int(np.unique(np.concatenate(out)).size)
+
+
+def full_run(sim, scene, steps: int, repeats: int) -> dict:
+    """The workload's whole run of `steps` steps from its initial state (the
+    first step radix-sorted, the rest re-sorted graph replays), timed
+    `repeats` times after an untimed run; median and all values."""
+    import statistics
+
+    vals = []
+    for r in range(repeats + 1):
+        sim.upload(scene)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step_graph(steps)
+        sim.synchronize()
+        if r:
+            vals.append(time.perf_counter() - t0)
+    med = statistics.median(vals)
+    return {"steps": steps, "repeats": repeats, "seconds_median": med, "seconds": vals,
+            "value_median": scene.n * steps / med, "unit": "particle-steps/s",
+            "from": "the initial state: first step radix-sorted, then re-sorted steps by graph replay"}
+
+
+def trajectory_error(workload: str):
+    """FP32-vs-FP64 trajectory error of the workload's run, from the oracle
+    runs in profiles/trajectory_error.json (made by oracle/trajectory_error.py
+    on the build host; the GPU's FP32 state equals the FP32 oracle's bit for
+    bit in the parity tests, so these are the GPU run's errors)."""
+    path = ROOT / "profiles" / "trajectory_error.json"
+    name = {"settling": "settling", "column_collapse": "column_collapse", "ensemble": "ensemble",
+            "adjoint": "adjoint_column", "calibration": "calibration"}.get(workload)
+    if not path.exists() or name is None:
+        return None
+    rec = json.loads(path.read_text()).get(name)
+    if rec is None:
+        return None
+    f = rec["final"]
+    return {"steps": rec["steps"], "max_pos_err_m": f["max_pos_err_m"], "rms_pos_err_m": f["rms_pos_err_m"],
+            "max_vel_err_mps": f["max_vel_err_mps"], "max_speed_mps": f["max_speed_mps"],
+            "source": "profiles/trajectory_error.json (oracle FP32 vs FP64 runs of the same workload)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,7 +348,12 @@ def main() -> int:
     ap.add_argument("--iters", type=int, default=5, help="Adam iterations of the calibration workload")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps that gauge the CPU baseline's rate (0 = skip it)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work of the whole CPU baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline (0 = the CPUs this process may run on, at most 16)")
+    ap.add_argument("--full-repeats", type=int, default=5,
+                    help="timed repeats of the workload's whole run from its initial state (0 = skip)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="settling workload: skip the ensemble and adjoint comparisons of the default line")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--collide-mode", choices=["plain", "prefetch", "batch"], default=None,
                     help="with --no-autotune: collide's candidate mode (default: one candidate at a time)")
@@ -304,6 +413,22 @@ def main() -> int:
     achieved = dom_bytes / (kernel_stages[dom] * 1e-3) / 1e9
     step_bytes = sum(algorithmic_bytes(k, n, cells, **acct) for k in STAGES)
 
+    full = None
+    if args.full_repeats > 0:
+        full = full_run(sim, scene, scene.steps if args.workload in ("settling", "column_collapse") else 1000,
+                        args.full_repeats)
+        sim.step(1)
+    fixed = fixed_step_bytes(n, cells)
+    touched = None
+    if "collide" in kernel_stages and layout != "fused":
+        used = cells_collide_reads(scene, sim.download()["hash"])
+        pp, _ = stage_bytes(layout, acct["radius"], acct["typed"])
+        tb = pp["collide"] * n + 8 * used
+        ach = tb / (stages["collide"] * 1e-3) / 1e9
+        touched = {"cells_read": used, "bytes": tb, "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS,
+                   "note": "collide's algorithmic bytes counting cellStart/cellEnd only at the cells it reads "
+                           "(the 3x3 neighbourhoods of occupied cells) instead of every cell of the grid"}
+
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{args.workload}.json"
     if tfile.exists():
@@ -312,7 +437,13 @@ def main() -> int:
 
     cpu = None
     if args.cpu_steps > 0:
-        cpu = cpu_baseline(scene, args.cpu_steps, args.cpu_threads, args.cpu_seconds)
+        share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = args.cpu_threads or min(16, share)
+        cpu = cpu_baseline(scene, args.cpu_steps, threads, args.cpu_seconds)
+        cpu["host_nproc"] = os.cpu_count()
+        cpu["cpus_available"] = share
+        cpu["threads_note"] = ("OpenMP over the CPUs this process may use, at most 16: a one-GPU job "
+                               "is given a 16-CPU share of the host")
 
     extra = {}
     if args.workload == "adjoint":
@@ -321,6 +452,11 @@ def main() -> int:
     if args.workload == "calibration":
         sim.close()
         extra["calibration"] = calibration_run(ctx, scene, args.adj_steps or 1000, args.iters, 100, 2000)
+
+    if args.workload == "settling" and not args.no_extras:
+        sim.close()
+        extra["ensemble_speedup"] = ensemble_speedup(ctx)
+        extra["adjoint_backward_over_forward"] = adjoint_ratio(ctx)
 
     ensemble = None
     if args.workload == "ensemble" and not args.no_one_by_one:
@@ -360,6 +496,18 @@ def main() -> int:
             "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
             "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
+        "step_roofline_fixed": {
+            "bytes_per_step": fixed,
+            "formula": "212 N + 16 C",
+            "achieved_GBs": fixed / (ms_per_step * 1e-3) / 1e9,
+            "frac": fixed / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "note": "charges a full sort and reorder of every particle each step; the step re-sorts in place "
+                    "and moves only the particles whose slot changes, so on a state where few discs change "
+                    "cell this figure can exceed the HBM peak",
+        },
+        "collide_touched": touched,
+        "full_run": full,
+        "traj_error_fp64": trajectory_error(args.workload),
         "stage_ms": stages,
         "cpu_baseline": cpu,
     }
