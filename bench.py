@@ -91,6 +91,10 @@ def make_scene(name: str):
 
     if name == "settling":
         return sc.settling(), "settling: 4,194,304 log-normal discs (r0=5 mm), 4731x2366 cells, FP32"
+    if name == "settling_dense":
+        s = sc.settling_dense()
+        return s, (f"settling_dense: the 4,194,304 settling discs on a lattice at their mean diameter "
+                   f"(about 2 contacts a disc), {s.instances[0].grid[0]}x{s.instances[0].grid[1]} cells, FP32")
     if name == "column_collapse":
         return sc.column_collapse(), "column collapse: 4,096 discs, 91x46 cells, FP32"
     if name == "ensemble":
@@ -343,7 +347,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="settling", choices=["settling", "column_collapse", "ensemble", "adjoint", "calibration"])
+    ap.add_argument("--workload", default="settling", choices=["settling", "settling_dense", "column_collapse", "ensemble", "adjoint",
+                             "calibration"])
     ap.add_argument("--adj-steps", type=int, default=0, help="steps of the adjoint / calibration run (0 = workload default)")
     ap.add_argument("--iters", type=int, default=5, help="Adam iterations of the calibration workload")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps that gauge the CPU baseline's rate (0 = skip it)")
@@ -355,7 +360,7 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true",
                     help="settling workload: skip the ensemble and adjoint comparisons of the default line")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--collide-mode", choices=["plain", "prefetch", "batch"], default=None,
+    ap.add_argument("--collide-mode", choices=["plain", "prefetch", "batch", "list"], default=None,
                     help="with --no-autotune: collide's candidate mode (default: one candidate at a time)")
     ap.add_argument("--no-one-by-one", action="store_true",
                     help="ensemble workload: skip timing the instances one by one (for profiler runs)")
@@ -381,7 +386,8 @@ def main() -> int:
         cell_tuning = tuning.autotune(ctx, scene, multipliers=(1.0, 1.25, 1.5, 2.0), steps=10)
         scene = tuning.with_cell_multiplier(scene, cell_tuning["multiplier"])
     if args.collide_mode:
-        ctx.set_step_variant(prefetch=args.collide_mode == "prefetch", batch=args.collide_mode == "batch")
+        ctx.set_step_variant(prefetch=args.collide_mode == "prefetch", batch=args.collide_mode == "batch",
+                             list=args.collide_mode == "list")
     sim = api.Simulation(ctx, scene)
     n, cells = scene.n, scene.total_cells
 
@@ -415,7 +421,7 @@ def main() -> int:
 
     full = None
     if args.full_repeats > 0:
-        full = full_run(sim, scene, scene.steps if args.workload in ("settling", "column_collapse") else 1000,
+        full = full_run(sim, scene, scene.steps if args.workload in ("settling", "settling_dense", "column_collapse") else 1000,
                         args.full_repeats)
         sim.step(1)
     fixed = fixed_step_bytes(n, cells)

@@ -195,6 +195,15 @@ def load() -> ctypes.CDLL:
             f"{LIB_PATH} is not built; run `python -m demso_amd.build` "
             "(there is no fallback path)"
         )
+    # torch ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7) as the ROCm one libdemso.so is linked against; the
+    # first one loaded serves the whole process.  Loading torch first keeps
+    # torch on the runtime it was built with (torch then finds no GPU if
+    # libdemso.so has brought in the other one), and libdemso.so uses it too.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(str(LIB_PATH))
     for name, (res, args) in PROTOTYPES.items():
         fn = getattr(lib, name)

@@ -148,16 +148,18 @@ class Context:
             self.set_materials(particle_props(scene, single_instance=False))
 
     def set_step_variant(self, fused: bool = False, prefetch: bool = False, batch: bool = False,
-                         copy: bool = False, wide: bool = False) -> None:
+                         copy: bool = False, wide: bool = False, list: bool = False) -> None:
         """Owned simulations: collide + integrate as one kernel or two, and
         how collide loads its candidates: one at a time, one ahead
         (prefetch) or the first of every row at once (batch); copy
         makes re-sorted steps reorder the whole state into the other buffer
         instead of moving only the particles whose slot changes; wide
         makes collide use 64-bit addresses (as states over 2^29 particles
-        or 2^30 cells do).  The results are the same in every combination."""
+        or 2^30 cells do); list makes collide mark the candidates that may
+        touch before running the contact law over them (contact-dense
+        states).  The results are the same in every combination."""
         flags = (int(bool(fused)) | (2 if prefetch else 0) | (4 if batch else 0) | (8 if copy else 0)
-                 | (16 if wide else 0))
+                 | (16 if wide else 0) | (32 if list else 0))
         self._check(self.lib.dem_set_step_variant(self.handle, flags))
 
     def set_gravity(self, gx: float, gy: float) -> None:
@@ -514,7 +516,8 @@ class Simulation:
         b = (c_uint32 * 5)()
         self.ctx._check(self.lib.dem_sim_autotune(self.handle, b))
         return {"hash": b[0], "reorder": b[1], "collide": b[2], "integrate": b[3], "fused": bool(b[4] & 1),
-                "prefetch": bool(b[4] & 2), "batch": bool(b[4] & 4), "copy": bool(b[4] & 8)}
+                "prefetch": bool(b[4] & 2), "batch": bool(b[4] & 4), "copy": bool(b[4] & 8),
+                "list": bool(b[4] & 32)}
 
     def sort_passes(self) -> int:
         return int(self.lib.dem_sim_sort_passes(self.handle))

@@ -1092,12 +1092,13 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
 
 int dem_set_step_variant(dem_context* ctx, int flags) {
   if (!ctx) return DEM_ERR_INVALID;
-  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH | DEM_STEP_COPY | DEM_STEP_WIDE))
+  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH | DEM_STEP_COPY | DEM_STEP_WIDE | DEM_STEP_LIST))
     return fail(ctx, DEM_ERR_INVALID, "unknown step variant flags");
-  if ((flags & DEM_STEP_PREFETCH) && (flags & DEM_STEP_BATCH))
-    return fail(ctx, DEM_ERR_INVALID, "DEM_STEP_PREFETCH and DEM_STEP_BATCH exclude each other");
+  const int modes = !!(flags & DEM_STEP_PREFETCH) + !!(flags & DEM_STEP_BATCH) + !!(flags & DEM_STEP_LIST);
+  if (modes > 1)
+    return fail(ctx, DEM_ERR_INVALID, "DEM_STEP_PREFETCH, DEM_STEP_BATCH and DEM_STEP_LIST exclude each other");
   ctx->fuse = (flags & DEM_STEP_FUSED) != 0;
-  ctx->cmode = (flags & DEM_STEP_PREFETCH) ? 1 : ((flags & DEM_STEP_BATCH) ? 2 : 0);
+  ctx->cmode = (flags & DEM_STEP_PREFETCH) ? 1 : ((flags & DEM_STEP_BATCH) ? 2 : ((flags & DEM_STEP_LIST) ? 3 : 0));
   ctx->copy = (flags & DEM_STEP_COPY) != 0;
   ctx->wide = (flags & DEM_STEP_WIDE) != 0;
   ctx->generation++;  // captured graphs follow the choice
@@ -1462,9 +1463,13 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
   float t_full = 1e30f;  // the fused layout's reorder (every field) at the best block size
   // collide and fused block sizes for each candidate mode
-  int best_c[3] = {ctx->bs_collide, ctx->bs_collide, ctx->bs_collide};
-  int best_f[3] = {ctx->bs_collide, ctx->bs_collide, ctx->bs_collide};
-  float t_c[3] = {1e30f, 1e30f, 1e30f}, t_f[3] = {1e30f, 1e30f, 1e30f};
+  constexpr int kModes = 4;
+  int best_c[kModes], best_f[kModes];
+  float t_c[kModes], t_f[kModes];
+  for (int p = 0; p < kModes; ++p) {
+    best_c[p] = best_f[p] = ctx->bs_collide;
+    t_c[p] = t_f[p] = 1e30f;
+  }
   for (int bs : cand) {
     if (e != hipSuccess) break;
     float tm = 0;
@@ -1480,7 +1485,7 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
     e = time_it(integrate, bs, &tm);
     if (e != hipSuccess) break;
     if (tm < best_t[3]) { best_t[3] = tm; best[3] = bs; }
-    for (int p = 0; p < 3 && e == hipSuccess; ++p) {
+    for (int p = 0; p < kModes && e == hipSuccess; ++p) {
       pf = p;
       e = time_it(collide, bs, &tm);
       if (e != hipSuccess) break;
@@ -1518,9 +1523,9 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   int mode = 0;
   float tbest = 1e30f;
   const float t_move = inplace ? t_apply : best_t[1];  // the split layout's reorder
-  for (int p = 0; p < 3; ++p) {
+  for (int p = 0; p < kModes; ++p) {
     if (t_move + t_c[p] + best_t[3] < tbest) { tbest = t_move + t_c[p] + best_t[3]; fuse = false; mode = p; }
-    if (t_full + t_f[p] < tbest) { tbest = t_full + t_f[p]; fuse = true; mode = p; }
+    if (p != 2 && t_full + t_f[p] < tbest) { tbest = t_full + t_f[p]; fuse = true; mode = p; }
   }
   ctx->bs_hash = best[0];
   ctx->bs_reorder = best[1];
@@ -1534,7 +1539,8 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
     block[1] = (uint32_t)best[1];
     block[2] = (uint32_t)ctx->bs_collide;
     block[3] = (uint32_t)best[3];
-    block[4] = (fuse ? 1u : 0u) | (mode == 1 ? 2u : 0u) | (mode == 2 ? 4u : 0u) | (ctx->copy ? 8u : 0u);
+    block[4] = (fuse ? 1u : 0u) | (mode == 1 ? 2u : 0u) | (mode == 2 ? 4u : 0u) | (ctx->copy ? 8u : 0u) |
+               (mode == 3 ? (uint32_t)DEM_STEP_LIST : 0u);
   }
   return DEM_OK;
 }

@@ -161,7 +161,7 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
                           const uint32_t* o_id, const uint32_t* o_type, const float* o_radius,
                           uint32_t n, uint32_t num_cells, bool full = true);
 // launch_collide's mode: the candidate mode (0 one at a time, 1 one ahead, 2
-// the first of every row at once), plus kWideMode to force 64-bit addressing
+// the first of every row at once, 3 contact lists), plus kWideMode to force 64-bit addressing
 // (the plain mode), which states over 2^29 particles or 2^30 cells get anyway.
 constexpr int kWideMode = 4;
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,

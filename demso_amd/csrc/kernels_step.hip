@@ -196,14 +196,35 @@ struct PairCtx {
   float2 p, v;
   float w, rA, dt;
   double den;
+  double rden;  // RN(1/den), for the contact-list mode's pressure terms
   Material mA;
   uint32_t tA;
 };
+
+// The pressure term with the particle's rounded reciprocal y = RN(1/den):
+// q = RN(a y), then one correction q' = RN(q + r y) with the exact remainder
+// r = a - q den (Markstein's theorem: q' is the correctly rounded a/den
+// when y is the correctly rounded reciprocal and q is within an ulp of the
+// quotient; a is a float length and den = 4 pi r^2 of a float r, so every
+// step stays clear of overflow and underflow in double).  A non-finite
+// length divides as the reference does.  Three FP64 operations per contact
+// instead of a division sequence.
+__device__ __forceinline__ float add_pressure_recip(float p, float len, double den, double rden) {
+  const double a = (double)len;
+  double qd = a * rden;
+  const double rem = fma(-qd, den, a);
+  if (__builtin_expect(__builtin_isfinite(rem), 1))
+    qd = fma(rem, rden, qd);
+  else
+    qd = a / den;
+  return (float)((double)p + qd);
+}
 
 // collideSpheres' contact law (src/particles_kernel.cuh:255-303) for a
 // partner B of material tB at offset (rx, ry) and distance dist < R: the
 // force on A as collideSpheres returns it (ffx, ffy), its pressure term, and
 // the friction moment added to `moment`.
+template <bool kRcp = false>
 __device__ __forceinline__ void contact_force(const StepConst& c, const PairCtx& q, float rx, float ry, float dist,
                                               float R, float rB, uint32_t tB, float2 vB, float wB, float& ffx,
                                               float& ffy, float& moment, float& pres) {
@@ -223,7 +244,7 @@ __device__ __forceinline__ void contact_force(const StepConst& c, const PairCtx&
     ffy += s2 * ny;
   }
   const float flen = sqrtf(ffx * ffx + ffy * ffy);
-  pres = add_pressure(pres, flen, q.den);
+  pres = kRcp ? add_pressure_recip(pres, flen, q.den, q.rden) : add_pressure(pres, flen, q.den);
   if (mu != 0.0f) {
     const float sr = q.rA * q.w + rB * wB;
     const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
@@ -321,7 +342,7 @@ struct RowSum {
   }
 };
 
-template <bool kRadius, bool kOne, bool kDyn, bool kO>
+template <bool kRadius, bool kOne, bool kDyn, bool kO, bool kRcp = false>
 __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
                                              const Cand& b, const Dyn& d, const uint32_t* __restrict__ type,
                                              RowSum& rsum, float& fx, float& fy, float& moment, float& pres) {
@@ -350,7 +371,7 @@ __device__ __forceinline__ void candidate_at(const StepConst& c, const PairCtx& 
       vB = ld<kO>(d.vel, jv);
       wB = ld<kO>(d.omega, jv);
     }
-    contact_force(c, q, rx, ry, dist, R, rB, tB, vB, wB, ffx, ffy, moment, pres);
+    contact_force<kRcp>(c, q, rx, ry, dist, R, rB, tB, vB, wB, ffx, ffy, moment, pres);
     cfx += ffx;
     cfy += ffy;
   }
@@ -416,14 +437,14 @@ __device__ __forceinline__ void candidate_load(const StepConst& c, uint32_t j, c
   }
 }
 
-template <bool kRadius, bool kOne, bool kDyn, bool kO>
+template <bool kRadius, bool kOne, bool kDyn, bool kO, bool kRcp = false>
 __device__ __forceinline__ void candidate(const StepConst& c, const PairCtx& q, uint32_t j, bool self,
                                           const float2* __restrict__ pos, const Dyn& d,
                                           const uint32_t* __restrict__ type, const float* __restrict__ radius,
                                           RowSum& rsum, float& fx, float& fy, float& moment, float& pres) {
   Cand b;
   candidate_load<kRadius, kOne, kDyn, kO>(c, j, pos, type, radius, d, b);
-  candidate_at<kRadius, kOne, kDyn, kO>(c, q, j, self, b, d, type, rsum, fx, fy, moment, pres);
+  candidate_at<kRadius, kOne, kDyn, kO, kRcp>(c, q, j, self, b, d, type, rsum, fx, fy, moment, pres);
 }
 
 // Forces on sorted particle i from its 3x3 cell neighbourhood (the body of
@@ -443,10 +464,14 @@ struct Own {
 // one load at a time (each candidate's velocity and spin loaded with its
 // position); kPrefetch loads each one iteration ahead; kBatch loads the first
 // candidate of all three rows at once before walking them (three loads in
-// flight).  The order of the sums is the same in every mode.  Preloading two
-// or three candidates a row needs 73 or 81 VGPRs (6 waves per SIMD instead of
-// 8) and was slower.
-constexpr int kPlain = 0, kPrefetch = 1, kBatch = 2;
+// flight); kList first walks the rows testing only position and radius and
+// sets a bit per candidate that may touch, then runs the contact law over the
+// set bits of all three rows, so a wave executes the contact law as often as
+// its busiest lane has contacts instead of at every candidate where some lane
+// has one.  The order of the sums is the same in every mode.  Preloading
+// two or three candidates a row needs 73 or 81 VGPRs (6 waves per SIMD
+// instead of 8) and was slower.
+constexpr int kPlain = 0, kPrefetch = 1, kBatch = 2, kList = 3;
 constexpr int kBatchN = 1;
 
 // Material layouts of the kernels: one material row (no type is read), one
@@ -481,6 +506,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   own.v = q.v;
   own.w = q.w;
   q.den = pressure_den(q.rA);
+  q.rden = kMode == kList ? 1.0 / q.den : 0.0;
   q.dt = c.dt;
   const bool clamp = c.flags & DEM_GRID_CLAMP;
   const int G0 = (int)in.grid[0], G1 = (int)in.grid[1];
@@ -545,6 +571,55 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
           candidate_load<kRadius, kOne, false, kO>(c, j < re[y] ? j : 0u, pos, type, radius, d, cb[y][k]);
         }
     }
+    if (kMode == kList && !c.attract && re[0] - rs[0] <= 64u && re[1] - rs[1] <= 64u && re[2] - rs[2] <= 64u) {
+      // Rows of at most 64 candidates: first one bit per candidate that may
+      // touch (the pair law's own prefilter, from position and radius), then
+      // the law over the set bits of all three rows in row order, so a wave
+      // runs the law as often as its busiest lane has contacts.  Candidates
+      // that exert no force add +0, so skipping them leaves every sum
+      // unchanged; a row's partial sum is flushed when the walk moves on to
+      // a later row.  (With attraction every candidate within reach pulls,
+      // so the rows are walked one candidate at a time instead.)
+      uint64_t h[3];
+#pragma unroll
+      for (int y = 0; y < 3; ++y) {
+        h[y] = 0;
+        for (uint32_t j = rs[y]; j < re[y]; ++j) {
+          Cand b;
+          candidate_load<kRadius, kOne, false, kO>(c, j, pos, type, radius, d, b);
+          const float rx = b.p.x - q.p.x, ry = b.p.y - q.p.y;
+          const float d2 = rx * rx + ry * ry;
+          const float R = q.rA + b.r;
+          const bool maybe = !(d2 > (R * R) * 1.00000095367431640625f) || R < 1.0842022e-19f;
+          h[y] |= (uint64_t)(maybe && j != i) << (j - rs[y]);
+        }
+      }
+      RowSum rsum;
+      rsum.b0 = b0[0];
+      rsum.b1 = b1[0];
+      int row = 0;
+      while (h[0] | h[1] | h[2]) {
+        const bool in0 = h[0] != 0, in1 = !in0 && h[1] != 0;
+        const int y = in0 ? 0 : (in1 ? 1 : 2);
+        const uint64_t hy = in0 ? h[0] : (in1 ? h[1] : h[2]);
+        const uint32_t j = (in0 ? rs[0] : (in1 ? rs[1] : rs[2])) + (uint32_t)__builtin_ctzll(hy);
+        const uint64_t rest = hy & (hy - 1u);
+        h[0] = in0 ? rest : h[0];
+        h[1] = in1 ? rest : h[1];
+        h[2] = (!in0 && !in1) ? rest : h[2];
+        if (y != row) {
+          fx += rsum.cfx;
+          fy += rsum.cfy;
+          rsum = RowSum{};
+          rsum.b0 = in0 ? b0[0] : (in1 ? b0[1] : b0[2]);
+          rsum.b1 = in0 ? b1[0] : (in1 ? b1[1] : b1[2]);
+          row = y;
+        }
+        candidate<kRadius, kOne, true, kO, true>(c, q, j, false, pos, d, type, radius, rsum, fx, fy, moment, pres);
+      }
+      fx += rsum.cfx;
+      fy += rsum.cfy;
+    } else {
 #pragma unroll
     for (int y = 0; y < 3; ++y) {
       RowSum rsum;
@@ -576,6 +651,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       }
       fx += rsum.cfx;
       fy += rsum.cfy;
+    }
     }
   } else {
     // wrap mode at the grid's x edge: the three cells one by one, in the
@@ -911,6 +987,7 @@ inline int mat_kind(const StepConst& c) {
     if (ctl) DEM_DISPATCH_RM(MACRO, kPlain, true);                       \
     else if (mode == kPrefetch) DEM_DISPATCH_RM(MACRO, kPrefetch, false);\
     else if (mode == kBatch) DEM_DISPATCH_RM(MACRO, kBatch, false);      \
+    else if (mode == kList) DEM_DISPATCH_RM(MACRO, kList, false);        \
     else DEM_DISPATCH_RM(MACRO, kPlain, false);                          \
   } while (0)
 
@@ -956,6 +1033,7 @@ hipError_t launch_collide_integrate(const StepConst& c, hipStream_t s, int block
   // of the batch it spills heavily; it loads one candidate at a time instead
   if (ctl) DEM_DISPATCH_RM(DEM_CI, kPlain, true);
   else if (mode == kPrefetch) DEM_DISPATCH_RM(DEM_CI, kPrefetch, false);
+  else if (mode == kList) DEM_DISPATCH_RM(DEM_CI, kList, false);
   else DEM_DISPATCH_RM(DEM_CI, kPlain, false);
 #undef DEM_CI
   return hipGetLastError();

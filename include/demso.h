@@ -255,7 +255,7 @@ int dem_sim_step_graph(dem_sim* sim, uint32_t steps);
 int dem_sim_stage_times(dem_sim* sim, uint32_t reps, float* ms);
 /* Choose block sizes per kernel, whether the sim runs collide and
  * integrate as one kernel or two, and how collide loads its neighbour
- * candidates (DEM_STEP_PREFETCH, DEM_STEP_BATCH or neither), by timing
+ * candidates (DEM_STEP_PREFETCH, DEM_STEP_BATCH, DEM_STEP_LIST or neither), by timing
  * candidates on the current state (the state is not modified).  Writes the
  * choice to block[5] (hash, reorder, collide, integrate, flags: the
  * DEM_STEP_* bits chosen) if non-NULL. */
@@ -291,6 +291,11 @@ int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
  * addresses and one candidate at a time beyond; DEM_STEP_WIDE makes it use
  * the latter at every size.  Same results. */
 #define DEM_STEP_WIDE 0x10
+/* DEM_STEP_LIST makes collide first mark, row by row, the candidates that
+ * may touch (position and radius only) and then run the contact law over
+ * the marked ones: fewer idle lanes on contact-dense states (not together
+ * with PREFETCH or BATCH).  Same results. */
+#define DEM_STEP_LIST 0x20
 int dem_set_step_variant(dem_context* ctx, int flags);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);

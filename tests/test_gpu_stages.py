@@ -143,7 +143,8 @@ def test_reorder(gpu_ctx, oracle, name):
         assert bits_equal(host(out.radius), st_ref["radius"])
 
 
-COLLIDE_VARIANTS = {"plain": {}, "prefetch": {"prefetch": True}, "batch": {"batch": True}, "wide": {"wide": True}}
+COLLIDE_VARIANTS = {"plain": {}, "prefetch": {"prefetch": True}, "batch": {"batch": True}, "wide": {"wide": True},
+                    "list": {"list": True}}
 
 
 @pytest.mark.parametrize("variant", list(COLLIDE_VARIANTS))

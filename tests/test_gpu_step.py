@@ -90,12 +90,14 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
 @pytest.mark.parametrize("variant", [(True, False), (False, True), (True, True), (False, False, True),
                                      (False, False, False, True), (False, False, True, True),
                                      (True, False, True), (False, False, False, False, True),
-                                     (False, True, False, True, True)])
+                                     (False, True, False, True, True), (False, False, False, False, False, True),
+                                     (True, False, False, False, False, True),
+                                     (False, False, False, True, False, True)])
 @pytest.mark.parametrize("name", ["pack_1024", "ensemble_5", "pile_wrap_attract"])
 def test_step_variants_equal(gpu_ctx, name, variant):
     """collide + integrate as one kernel or two, and each way of loading
     collide's candidates (one at a time, one ahead, the first of every row
-    at once), re-sorted steps in place or into the other buffer (copy), and
+    at once, contact lists), re-sorted steps in place or into the other buffer (copy), and
     collide with 32-bit offsets or 64-bit addresses (wide): the same state
     and cell ranges, bit for bit."""
     make, steps = CASES[name]
@@ -184,6 +186,39 @@ def test_dense_wide_pack_matches_oracle(gpu_ctx, oracle):
     sim.close()
     fused.close()
     ctx2.close()
+
+
+@pytest.mark.parametrize("mult", [1.0, 4.0])
+def test_contact_lists_on_dense_packs(gpu_ctx, oracle, mult):
+    """The contact-list candidate mode on the dense pack (36 candidates a
+    disc, contacts throughout), split and fused, equals the oracle after one
+    step and the plain mode after 20; with cells four times wider (rows of
+    about 400 candidates) it walks rows over 64 candidates one at a time."""
+    from demso_amd import tuning
+
+    s = tuning.with_cell_multiplier(wide_pack(side=128), mult)
+    ref = oracle.Sim(s, "f32")
+    ref.step(1)
+    ctxs, sims = [], []
+    for variant in ({}, {"list": True}, {"list": True, "fused": True}):
+        ctx = api.Context(0)
+        ctx.load_scene(s)
+        ctx.set_step_variant(**variant)
+        sim = api.Simulation(ctx, s)
+        sim.step(1)
+        compare(sim.download(), ref.state())
+        sim.step_graph(19)
+        ctxs.append(ctx)
+        sims.append(sim)
+    ref.close()
+    base = sims[0].download()
+    for sim in sims[1:]:
+        d = sim.download()
+        for k in ("hash", "index", "cellStart", "id", "pos", "vel", "theta", "omega", "acc", "alpha", "pressure"):
+            assert bits_equal(base[k], d[k]), k
+    for sim, ctx in zip(sims, ctxs):
+        sim.close()
+        ctx.close()
 
 
 @pytest.mark.parametrize("n", [1, 2, 63, 65, 4097])
