@@ -195,28 +195,55 @@ def calibration_run(ctx, scene, steps: int, iters: int, K: int, settle_steps: in
     """Batched calibration: every instance starts from (kn, mu, e) =
     (1e5, 0.5, 0.5) and Adam recovers its own drawn parameters from the
     final positions of a target run with them (all instances in one launch
-    per stage)."""
+    per stage), from the lattice after `settle_steps` plain steps (0: the
+    lattice as generated).  Reports the error of every instance's recovered
+    parameters, and the forward + adjoint particle-steps/s against the same
+    instances' plain forward run one by one."""
+    import statistics
+
     from demso_amd import calibrate
 
-    s = calibrate.settle(ctx, scene, settle_steps)
+    s = calibrate.settle(ctx, scene, settle_steps) if settle_steps else scene.copy()
     T = len(s.materials)
     truth = {"kn": [m.kn for m in s.materials], "mu": [m.mu for m in s.materials],
-             "e": list(s.notes.get("restitution", [])) or None}
+             "e": [float(v) for v in s.notes["restitution"]]}
     tgt = s.copy()
     start = calibrate.set_params(s, [1e5] * T, [0.5] * T, [0.5] * T)
     t0 = time.perf_counter()
     out = calibrate.calibrate(ctx, start, tgt, steps=steps, iters=iters, lr=0.1, checkpoint_every=K)
     secs = time.perf_counter() - t0
     hist = out["history"]
-    kn_err = [abs(a / b - 1) for a, b in zip(out["kn"], truth["kn"])]
-    mu_err = [abs(a - b) for a, b in zip(out["mu"], truth["mu"])]
+
+    def errors(kn, mu, e):
+        return {"kn_rel": [abs(a / b - 1) for a, b in zip(kn, truth["kn"])],
+                "mu_abs": [abs(a - b) for a, b in zip(mu, truth["mu"])],
+                "e_abs": [abs(a - b) for a, b in zip(e, truth["e"])]}
+
+    def summary(err):
+        return {k: {"median": statistics.median(v), "p90": sorted(v)[int(0.9 * (len(v) - 1))], "max": max(v)}
+                for k, v in err.items()}
+
+    start_err = errors([1e5] * T, [0.5] * T, [0.5] * T)
+    final_err = errors(list(out["kn"]), list(out["mu"]), list(out["e"]))
+    t_fwd = sum(h["t_fwd"] for h in hist) / iters
+    t_bwd = sum(h["t_bwd"] for h in hist) / iters
+    plain = one_by_one(ctx, s, min(steps, 200))
+    fwd_adj = s.n * steps / (t_fwd + t_bwd)
     return {
         "instances": T, "particles": s.n, "steps": steps, "iterations": iters, "checkpoint_every": K,
+        "settled_first": settle_steps,
         "seconds": secs, "seconds_per_iteration": secs / iters,
-        "forward_s_mean": sum(h["t_fwd"] for h in hist) / iters,
-        "backward_s_mean": sum(h["t_bwd"] for h in hist) / iters,
+        "forward_s_mean": t_fwd, "backward_s_mean": t_bwd, "backward_over_forward": t_bwd / t_fwd,
         "loss_first": hist[0]["loss"], "loss_last": hist[-1]["loss"],
-        "kn_rel_err_median": sorted(kn_err)[len(kn_err) // 2], "mu_abs_err_median": sorted(mu_err)[len(mu_err) // 2],
+        "error_start": summary(start_err), "error_final": summary(final_err),
+        "per_instance": [{"instance": k, "n": int(s.segments[k + 1] - s.segments[k]),
+                          "kn": [truth["kn"][k], float(out["kn"][k])], "mu": [truth["mu"][k], float(out["mu"][k])],
+                          "e": [truth["e"][k], float(out["e"][k])]} for k in range(T)],
+        "forward_adjoint_particle_steps_per_s": fwd_adj,
+        "plain_forward_one_by_one_particle_steps_per_s": plain["value"],
+        "forward_adjoint_vs_plain_one_by_one": fwd_adj / plain["value"],
+        "kn_rel_err_median": statistics.median(final_err["kn_rel"]),
+        "mu_abs_err_median": statistics.median(final_err["mu_abs"]),
     }
 
 
@@ -351,6 +378,8 @@ def main() -> int:
                              "calibration"])
     ap.add_argument("--adj-steps", type=int, default=0, help="steps of the adjoint / calibration run (0 = workload default)")
     ap.add_argument("--iters", type=int, default=5, help="Adam iterations of the calibration workload")
+    ap.add_argument("--settle", type=int, default=2000,
+                    help="calibration workload: plain steps that settle the lattice first (0: start as generated)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps that gauge the CPU baseline's rate (0 = skip it)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work of the whole CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -457,7 +486,7 @@ def main() -> int:
         extra["adjoint"] = adjoint_cost(ctx, scene, args.adj_steps or 2000, 100, 5000)
     if args.workload == "calibration":
         sim.close()
-        extra["calibration"] = calibration_run(ctx, scene, args.adj_steps or 1000, args.iters, 100, 2000)
+        extra["calibration"] = calibration_run(ctx, scene, args.adj_steps or 1000, args.iters, 100, args.settle)
 
     if args.workload == "settling" and not args.no_extras:
         sim.close()

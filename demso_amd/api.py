@@ -148,7 +148,8 @@ class Context:
             self.set_materials(particle_props(scene, single_instance=False))
 
     def set_step_variant(self, fused: bool = False, prefetch: bool = False, batch: bool = False,
-                         copy: bool = False, wide: bool = False, list: bool = False) -> None:
+                         copy: bool = False, wide: bool = False, list: bool = False,
+                         per_lane: Optional[dict] = None) -> None:
         """Owned simulations: collide + integrate as one kernel or two, and
         how collide loads its candidates: one at a time, one ahead
         (prefetch) or the first of every row at once (batch); copy
@@ -157,9 +158,17 @@ class Context:
         makes collide use 64-bit addresses (as states over 2^29 particles
         or 2^30 cells do); list makes collide mark the candidates that may
         touch before running the contact law over them (contact-dense
-        states).  The results are the same in every combination."""
+        states); per_lane gives the particles a lane (1, 2 or 4) of "hash",
+        "reorder" and "integrate".  The results are the same in every
+        combination."""
         flags = (int(bool(fused)) | (2 if prefetch else 0) | (4 if batch else 0) | (8 if copy else 0)
                  | (16 if wide else 0) | (32 if list else 0))
+        lg = {1: 0, 2: 1, 4: 2}
+        for shift, k in ((8, "hash"), (10, "reorder"), (12, "integrate")):
+            per = (per_lane or {}).get(k, 1)
+            if per not in lg:
+                raise ValueError("particles a lane must be 1, 2 or 4")
+            flags |= lg[per] << shift
         self._check(self.lib.dem_set_step_variant(self.handle, flags))
 
     def set_gravity(self, gx: float, gy: float) -> None:
@@ -517,7 +526,9 @@ class Simulation:
         self.ctx._check(self.lib.dem_sim_autotune(self.handle, b))
         return {"hash": b[0], "reorder": b[1], "collide": b[2], "integrate": b[3], "fused": bool(b[4] & 1),
                 "prefetch": bool(b[4] & 2), "batch": bool(b[4] & 4), "copy": bool(b[4] & 8),
-                "list": bool(b[4] & 32)}
+                "list": bool(b[4] & 32),
+                "per_lane": {"hash": 1 << ((b[4] >> 8) & 3), "reorder": 1 << ((b[4] >> 10) & 3),
+                             "integrate": 1 << ((b[4] >> 12) & 3)}}
 
     def sort_passes(self) -> int:
         return int(self.lib.dem_sim_sort_passes(self.handle))

@@ -409,7 +409,7 @@ int dem_calc_hash_ex(dem_context* ctx, const float* pos, const uint32_t* type,
   if (multi(ctx) && !type) return fail(ctx, DEM_ERR_INVALID, "an ensemble needs the type array to hash");
   if (int r = ensure_device(ctx)) return r;
   DEM_HIP(ctx, dem::launch_hash(ctx->c, ctx->stream, ctx->bs_hash, (const float2*)pos, type, gridParticleHash,
-                                gridParticleIndex, numParticles),
+                                gridParticleIndex, numParticles, ctx->per_hash),
           "hash kernel");
   return DEM_OK;
 }
@@ -540,7 +540,7 @@ int dem_reorder_ex(dem_context* ctx, uint32_t* cellStart, uint32_t* cellEnd, flo
                                    (float2*)sortedVel, sortedTheta, sortedOmega, sortedID, sortedLoc, sortedType,
                                    sortedRadius, gridParticleHash, gridParticleIndex, (const float2*)oldPos,
                                    (const float2*)oldVel, oldTheta, oldOmega, oldID, oldType, oldRadius,
-                                   numParticles, numCells),
+                                   numParticles, numCells, true, ctx->per_reorder),
           "reorder kernel");
   return DEM_OK;
 }
@@ -583,7 +583,7 @@ int dem_integrate_ex(dem_context* ctx, float* pos, float* vel, const float* acc,
   if (int r = ensure_device(ctx)) return r;
   DEM_HIP(ctx, dem::launch_integrate(ctx->c, ctx->stream, ctx->bs_integrate, (float2*)pos, (float2*)vel,
                                      (const float2*)acc, theta, omega, alpha, type, radius, pressure,
-                                     numParticles),
+                                     numParticles, nullptr, ctx->per_integrate),
           "integrate kernel");
   return DEM_OK;
 }
@@ -688,7 +688,7 @@ hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev, const Ste
   nk.tile_count = m->resort.tile_count;
   e = dem::launch_integrate_to(c, s, ctx->bs_integrate, m->pos[b], m->vel[b], m->theta[b], m->omega[b], m->pos[b],
                                m->vel[b], m->theta[b], m->omega[b], m->acc, m->alpha, m->type[b], radius,
-                               m->pressure, n, &nk);
+                               m->pressure, n, &nk, nullptr, ctx->per_integrate);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[6], s);
   m->sorted_key = m->skey[b];
@@ -729,7 +729,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
                            m->skey[t], m->sidx, m->resort, n, m->cs[t], m->cells);
     if (e != hipSuccess) return e;
   } else {
-    e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], type_for_hash, m->key, m->idx, n);
+    e = dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], type_for_hash, m->key, m->idx, n, ctx->per_hash);
     if (e != hipSuccess) return e;
     // this step's collide + integrate accumulates the next step's mover counts
     const uint32_t tiles = (n + dem::kResortTile - 1u) / dem::kResortTile;
@@ -757,7 +757,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
                           gather ? nullptr : m->theta[S], gather ? nullptr : m->omega[S], m->id[t], m->loc[t],
                           m->type[t], m->rad ? m->radius[t] : nullptr, m->sorted_key, m->sorted_idx, m->pos[o],
                           m->vel[o], m->theta[o], m->omega[o], m->id[o], m->type[o],
-                          m->rad ? m->radius[o] : nullptr, n, m->cells, !gather);
+                          m->rad ? m->radius[o] : nullptr, n, m->cells, !gather, ctx->per_reorder);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[4], s);
   dem::NextKeys nk;
@@ -805,7 +805,7 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
     // velocity, angle and spin through the index from buffer o)
     e = dem::launch_integrate_to(c, s, ctx->bs_integrate, m->pos[S], m->vel[D], m->theta[D], m->omega[D], m->pos[t],
                                  m->vel[t], m->theta[t], m->omega[t], m->acc, m->alpha, m->type[t], S_radius,
-                                 m->pressure, n, &nk, gat);
+                                 m->pressure, n, &nk, gat, ctx->per_integrate);
     if (e != hipSuccess) return e;
   }
   if (hook && hook->fn) {
@@ -1092,8 +1092,12 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
 
 int dem_set_step_variant(dem_context* ctx, int flags) {
   if (!ctx) return DEM_ERR_INVALID;
-  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH | DEM_STEP_COPY | DEM_STEP_WIDE | DEM_STEP_LIST))
+  if (flags & ~(DEM_STEP_FUSED | DEM_STEP_PREFETCH | DEM_STEP_BATCH | DEM_STEP_COPY | DEM_STEP_WIDE | DEM_STEP_LIST |
+                DEM_STEP_PER_MASK))
     return fail(ctx, DEM_ERR_INVALID, "unknown step variant flags");
+  const int lg_hash = (flags >> 8) & 3, lg_reorder = (flags >> 10) & 3, lg_integrate = (flags >> 12) & 3;
+  if (lg_hash == 3 || lg_reorder == 3 || lg_integrate == 3)
+    return fail(ctx, DEM_ERR_INVALID, "particles a lane must be 1, 2 or 4");
   const int modes = !!(flags & DEM_STEP_PREFETCH) + !!(flags & DEM_STEP_BATCH) + !!(flags & DEM_STEP_LIST);
   if (modes > 1)
     return fail(ctx, DEM_ERR_INVALID, "DEM_STEP_PREFETCH, DEM_STEP_BATCH and DEM_STEP_LIST exclude each other");
@@ -1101,6 +1105,9 @@ int dem_set_step_variant(dem_context* ctx, int flags) {
   ctx->cmode = (flags & DEM_STEP_PREFETCH) ? 1 : ((flags & DEM_STEP_BATCH) ? 2 : ((flags & DEM_STEP_LIST) ? 3 : 0));
   ctx->copy = (flags & DEM_STEP_COPY) != 0;
   ctx->wide = (flags & DEM_STEP_WIDE) != 0;
+  ctx->per_hash = 1 << lg_hash;
+  ctx->per_reorder = 1 << lg_reorder;
+  ctx->per_integrate = 1 << lg_integrate;
   ctx->generation++;  // captured graphs follow the choice
   return DEM_OK;
 }
@@ -1398,12 +1405,13 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   // the fully sorted copy into kScratch (the fused layout); the gather
   // layout's reorder writes the sorted positions there too
   const int K = kScratch;
+  int tper = 1;  // particles a lane under test (hash, reorder, integrate)
   auto reorder_as = [&](int bs, bool full) {
     return dem::launch_reorder(s, bs, m->cell_start, m->cell_end, m->pos[K], full ? m->vel[K] : nullptr,
                                full ? m->theta[K] : nullptr, full ? m->omega[K] : nullptr, m->id[t], m->loc[t],
                                m->type[t], m->rad ? m->radius[t] : nullptr, sk, si, m->pos[o], m->vel[o],
                                m->theta[o], m->omega[o], m->id[o], m->type[o], m->rad ? m->radius[o] : nullptr, n,
-                               m->cells, full);
+                               m->cells, full, tper);
   };
   DEM_HIP(ctx, reorder_as(ctx->bs_reorder, true), "tune");
   hipEvent_t e0, e1;
@@ -1452,7 +1460,7 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   };
   auto integrate = [&](int b) {
     return dem::launch_integrate_to(c, s, b, S_pos, D_vel, D_theta, D_omega, tpos, tvel, ttheta, tomega, m->acc,
-                                    m->alpha, m->type[t], S_radius, m->pressure, n, &nk, gat);
+                                    m->alpha, m->type[t], S_radius, m->pressure, n, &nk, gat, tper);
   };
   auto fused = [&](int b) {
     return dem::launch_collide_integrate(c, s, b, S_pos, S_vel, S_omega, S_theta, m->type[t], S_radius,
@@ -1460,8 +1468,10 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
                                          tomega, ttheta, nk, n, pf);
   };
   int best[4] = {ctx->bs_hash, ctx->bs_reorder, ctx->bs_collide, ctx->bs_integrate};
+  int best_per[4] = {ctx->per_hash, ctx->per_reorder, 1, ctx->per_integrate};
   float best_t[4] = {1e30f, 1e30f, 1e30f, 1e30f};
   float t_full = 1e30f;  // the fused layout's reorder (every field) at the best block size
+  int bs_full = ctx->bs_reorder, per_full = ctx->per_reorder;
   // collide and fused block sizes for each candidate mode
   constexpr int kModes = 4;
   int best_c[kModes], best_f[kModes];
@@ -1473,18 +1483,24 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   for (int bs : cand) {
     if (e != hipSuccess) break;
     float tm = 0;
-    e = time_it([&](int b) { return dem::launch_hash(c, s, b, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n); }, bs, &tm);
+    // the streaming kernels at 1, 2 and 4 particles a lane
+    for (int per : {1, 2, 4}) {
+      tper = per;
+      e = time_it([&](int b) { return dem::launch_hash(c, s, b, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n, tper); }, bs, &tm);
+      if (e != hipSuccess) break;
+      if (tm < best_t[0]) { best_t[0] = tm; best[0] = bs; best_per[0] = per; }
+      e = time_it([&](int b) { return reorder_as(b, true); }, bs, &tm);
+      if (e != hipSuccess) break;
+      if (tm < t_full) { t_full = tm; bs_full = bs; per_full = per; }
+      e = time_it([&](int b) { return reorder_as(b, false); }, bs, &tm);
+      if (e != hipSuccess) break;
+      if (tm < best_t[1]) { best_t[1] = tm; best[1] = bs; best_per[1] = per; }
+      e = time_it(integrate, bs, &tm);
+      if (e != hipSuccess) break;
+      if (tm < best_t[3]) { best_t[3] = tm; best[3] = bs; best_per[3] = per; }
+    }
     if (e != hipSuccess) break;
-    if (tm < best_t[0]) { best_t[0] = tm; best[0] = bs; }
-    e = time_it([&](int b) { return reorder_as(b, true); }, bs, &tm);
-    if (e != hipSuccess) break;
-    t_full = std::min(t_full, tm);
-    e = time_it([&](int b) { return reorder_as(b, false); }, bs, &tm);
-    if (e != hipSuccess) break;
-    if (tm < best_t[1]) { best_t[1] = tm; best[1] = bs; }
-    e = time_it(integrate, bs, &tm);
-    if (e != hipSuccess) break;
-    if (tm < best_t[3]) { best_t[3] = tm; best[3] = bs; }
+    tper = best_per[3];  // the fused kernel and collide run one particle a lane
     for (int p = 0; p < kModes && e == hipSuccess; ++p) {
       pf = p;
       e = time_it(collide, bs, &tm);
@@ -1528,9 +1544,12 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
     if (p != 2 && t_full + t_f[p] < tbest) { tbest = t_full + t_f[p]; fuse = true; mode = p; }
   }
   ctx->bs_hash = best[0];
-  ctx->bs_reorder = best[1];
+  ctx->bs_reorder = fuse ? bs_full : best[1];
   ctx->bs_collide = fuse ? best_f[mode] : best_c[mode];
   ctx->bs_integrate = best[3];
+  ctx->per_hash = best_per[0];
+  ctx->per_reorder = fuse ? per_full : best_per[1];
+  ctx->per_integrate = best_per[3];
   ctx->fuse = fuse;
   ctx->cmode = mode;
   ctx->generation++;
@@ -1539,8 +1558,10 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
     block[1] = (uint32_t)best[1];
     block[2] = (uint32_t)ctx->bs_collide;
     block[3] = (uint32_t)best[3];
+    auto lg = [](int per) { return per == 4 ? 2u : (per == 2 ? 1u : 0u); };
     block[4] = (fuse ? 1u : 0u) | (mode == 1 ? 2u : 0u) | (mode == 2 ? 4u : 0u) | (ctx->copy ? 8u : 0u) |
-               (mode == 3 ? (uint32_t)DEM_STEP_LIST : 0u);
+               (mode == 3 ? (uint32_t)DEM_STEP_LIST : 0u) | (lg(ctx->per_hash) << 8) |
+               (lg(ctx->per_reorder) << 10) | (lg(ctx->per_integrate) << 12);
   }
   return DEM_OK;
 }

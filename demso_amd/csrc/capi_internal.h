@@ -28,6 +28,9 @@ struct dem_context {
   dem::SortScratch sort;
   uint32_t* d_copy_tmp = nullptr;
   int bs_hash = 256, bs_reorder = 256, bs_collide = 128, bs_integrate = 256;
+  // particles a lane of hash, reorder and integrate (1, 2 or 4; chosen by
+  // dem_sim_autotune with the block sizes)
+  int per_hash = 1, per_reorder = 1, per_integrate = 1;
   bool fuse = false;      // owned simulation: collide + integrate as one kernel (chosen by dem_sim_autotune)
   int cmode = 0;  // collide's candidate loads: 0 one at a time, 1 one ahead, 2 first of every row
                  // at once (chosen by dem_sim_autotune)

@@ -81,8 +81,9 @@ __device__ __forceinline__ uint32_t xcd_block() {
 }
 
 // Launch helpers (defined in the .hip files).
+// per: particles a lane (1, 2 or 4) of hash, reorder and integrate.
 hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float2* pos,
-                       const uint32_t* type, uint32_t* hash, uint32_t* index, uint32_t n);
+                       const uint32_t* type, uint32_t* hash, uint32_t* index, uint32_t n, int per = 1);
 hipError_t launch_fill(hipStream_t s, uint32_t* p, uint32_t value, uint32_t count);
 
 // ---- re-sort of a state already sorted by its previous keys --------------
@@ -159,7 +160,7 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
                           const uint32_t* hash, const uint32_t* index, const float2* o_pos,
                           const float2* o_vel, const float* o_theta, const float* o_omega,
                           const uint32_t* o_id, const uint32_t* o_type, const float* o_radius,
-                          uint32_t n, uint32_t num_cells, bool full = true);
+                          uint32_t n, uint32_t num_cells, bool full = true, int per = 1);
 // launch_collide's mode: the candidate mode (0 one at a time, 1 one ahead, 2
 // the first of every row at once, 3 contact lists), plus kWideMode to force 64-bit addressing
 // (the plain mode), which states over 2^29 particles or 2^30 cells get anyway.
@@ -182,13 +183,14 @@ struct NextKeys {
 hipError_t launch_integrate(const StepConst& c, hipStream_t s, int block, float2* pos, float2* vel,
                             const float2* acc, float* theta, float* omega, const float* alpha,
                             const uint32_t* type, const float* radius, float* pressure, uint32_t n,
-                            const NextKeys* next = nullptr);
+                            const NextKeys* next = nullptr, int per = 1);
 // integrate from one set of state arrays into another (may be the same).
 hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, const float2* pos_in,
                                const float2* vel_in, const float* theta_in, const float* omega_in, float2* pos_out,
                                float2* vel_out, float* theta_out, float* omega_out, const float2* acc,
                                const float* alpha, const uint32_t* type, const float* radius, float* pressure,
-                               uint32_t n, const NextKeys* next = nullptr, const uint32_t* gat = nullptr);
+                               uint32_t n, const NextKeys* next = nullptr, const uint32_t* gat = nullptr,
+                               int per = 1);
 // collide + integrate + the next step's hash in one kernel: forces from the
 // sorted state *_in, acc/alpha/pressure as collide writes them, the
 // integrated state to *_out (a different buffer).

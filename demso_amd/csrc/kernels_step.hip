@@ -91,16 +91,23 @@ __device__ __forceinline__ uint32_t cell_key(const StepConst& c, float2 p, const
   return in.cell_base + cy * in.grid[0] + cx;
 }
 
-// calcHashD (src/particles_kernel.cuh:133-151).
-template <bool kMulti>
+// calcHashD (src/particles_kernel.cuh:133-151).  kPer particles a lane: a
+// block covers kPer * blockDim consecutive particles, lane t taking t,
+// t + blockDim, ... (each pass over the block's lanes stays coalesced).
+template <bool kMulti, int kPer>
 __global__ __launch_bounds__(256) void k_hash(StepConst c, const float2* __restrict__ pos,
                                               const uint32_t* __restrict__ type,
                                               uint32_t* __restrict__ hash,
                                               uint32_t* __restrict__ index, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  hash[i] = cell_key<kMulti>(c, pos[i], type, i);
-  index[i] = i;
+  const uint32_t base = blockIdx.x * blockDim.x * kPer + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const uint32_t i = base + (uint32_t)k * blockDim.x;
+    if (i < n) {
+      hash[i] = cell_key<kMulti>(c, pos[i], type, i);
+      index[i] = i;
+    }
+  }
 }
 
 // -------------------------------------------------------------- cell fill --
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint32_t
 // positions and the per-particle constants only: the owned simulation's step
 // reads velocity, angle and spin through the sorted index instead (collide
 // for its own particle and its contacts, integrate for every particle).
-template <bool kRadius, bool kFull>
+template <bool kRadius, bool kFull, int kPer>
 __global__ __launch_bounds__(256) void k_reorder(
     uint32_t* __restrict__ cell_start, uint32_t* __restrict__ cell_end, float2* __restrict__ s_pos,
     float2* __restrict__ s_vel, float* __restrict__ s_theta, float* __restrict__ s_omega,
@@ -135,33 +142,38 @@ __global__ __launch_bounds__(256) void k_reorder(
     const float* __restrict__ o_omega, const uint32_t* __restrict__ o_id,
     const uint32_t* __restrict__ o_type, const float* __restrict__ o_radius, uint32_t n,
     uint32_t num_cells) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t h = hash[i];
-  if (i == 0) {
-    if (h < num_cells) cell_start[h] = 0;
-  } else {
-    const uint32_t hp = hash[i - 1];
-    if (h != hp) {
-      if (h < num_cells) cell_start[h] = i;
-      if (hp < num_cells) cell_end[hp] = i;
+  // kPer particles a lane, as k_hash
+  const uint32_t base = blockIdx.x * blockDim.x * kPer + threadIdx.x;
+#pragma unroll
+  for (int kk = 0; kk < kPer; ++kk) {
+    const uint32_t i = base + (uint32_t)kk * blockDim.x;
+    if (i >= n) continue;
+    const uint32_t h = hash[i];
+    if (i == 0) {
+      if (h < num_cells) cell_start[h] = 0;
+    } else {
+      const uint32_t hp = hash[i - 1];
+      if (h != hp) {
+        if (h < num_cells) cell_start[h] = i;
+        if (hp < num_cells) cell_end[hp] = i;
+      }
     }
-  }
-  if (i == n - 1 && h < num_cells) cell_end[h] = n;
+    if (i == n - 1 && h < num_cells) cell_end[h] = n;
 
-  const uint32_t k = index[i];
-  if (k >= n) return;
-  s_pos[i] = o_pos[k];
-  if (kFull) {
-    s_vel[i] = o_vel[k];
-    s_theta[i] = o_theta[k];
-    s_omega[i] = o_omega[k];
+    const uint32_t k = index[i];
+    if (k >= n) continue;
+    s_pos[i] = o_pos[k];
+    if (kFull) {
+      s_vel[i] = o_vel[k];
+      s_theta[i] = o_theta[k];
+      s_omega[i] = o_omega[k];
+    }
+    const uint32_t id = o_id[k];
+    s_id[i] = id;
+    if (id < n) s_loc[id] = i;
+    s_type[i] = o_type[k];
+    if (kRadius) s_radius[i] = o_radius[k];
   }
-  const uint32_t id = o_id[k];
-  s_id[i] = id;
-  if (id < n) s_loc[id] = i;
-  s_type[i] = o_type[k];
-  if (kRadius) s_radius[i] = o_radius[k];
 }
 
 // --------------------------------------------------------------- collide --
@@ -855,33 +867,55 @@ __device__ __forceinline__ void next_key(const StepConst& c, const NextKeys& nk,
 // the same (in place, as the reference) or different (the owned
 // simulation writes the integrated state back to its home buffer); each
 // lane reads and writes only its own slot.
-template <bool kRadius, int kMat, bool kNext>
+template <bool kRadius, int kMat, bool kNext, int kPer>
 __global__ __launch_bounds__(256) void k_integrate(
     StepConst c, const float2* pos_in, const float2* vel_in, const float* theta_in, const float* omega_in,
     float2* pos_out, float2* vel_out, float* theta_out, float* omega_out, const float2* __restrict__ acc,
     const float* __restrict__ alpha, const uint32_t* __restrict__ type, const float* __restrict__ radius,
     float* __restrict__ pressure, NextKeys nk, const uint32_t* __restrict__ gat, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   constexpr bool kMulti = kMat == kInstances;
-  const uint32_t t = kMat == kOneMat ? 0u : clamp_idx(type[i], c.num_types);
-  const Material m = c.mat[t];
-  const Instance in = kMulti ? c.inst[clamp_idx(m.inst, c.num_instances)] : c.inst0;
-  float r, mass, inertia;
-  body<kRadius>(m, radius, i, r, mass, inertia);
-  // velocity, spin and angle through the gather index when one is given
-  const uint32_t g = gat ? gat[i] : i;
-  const float2 p_old = pos_in[i];
-  float2 p = p_old, v = vel_in[g];
-  float w = omega_in[g], th = theta_in[g];
-  float pres = 0.0f;
-  if (integrate_particle(c, m, in, r, mass, inertia, acc[i], alpha[i], p, v, w, th, pres, pressure + i))
-    pressure[i] = pres;
-  pos_out[i] = p;
-  vel_out[i] = v;
-  omega_out[i] = w;
-  theta_out[i] = th;
-  if (kNext) next_key<kMulti>(c, nk, p, p_old, type, i, n);
+  // kPer particles a lane, as k_hash.  The inputs of all kPer particles are
+  // loaded before any output is stored (the in and out arrays may be the
+  // same, so the compiler could not move a later particle's loads above an
+  // earlier one's stores); each lane reads and writes only its own slots.
+  const uint32_t base = blockIdx.x * blockDim.x * kPer + threadIdx.x;
+  uint32_t t[kPer];
+  float2 p_old[kPer], v[kPer], a[kPer];
+  float w[kPer], th[kPer], al[kPer], rad[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const uint32_t i = base + (uint32_t)k * blockDim.x;
+    if (i >= n) continue;
+    t[k] = kMat == kOneMat ? 0u : clamp_idx(type[i], c.num_types);
+    // velocity, spin and angle through the gather index when one is given
+    const uint32_t g = gat ? gat[i] : i;
+    p_old[k] = pos_in[i];
+    v[k] = vel_in[g];
+    w[k] = omega_in[g];
+    th[k] = theta_in[g];
+    a[k] = acc[i];
+    al[k] = alpha[i];
+    rad[k] = kRadius ? radius[i] : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const uint32_t i = base + (uint32_t)k * blockDim.x;
+    if (i >= n) continue;
+    const Material m = c.mat[t[k]];
+    const Instance in = kMulti ? c.inst[clamp_idx(m.inst, c.num_instances)] : c.inst0;
+    float r, mass, inertia;
+    body<kRadius>(m, rad + k, 0u, r, mass, inertia);
+    float2 p = p_old[k], vv = v[k];
+    float ww = w[k], tt = th[k];
+    float pres = 0.0f;
+    if (integrate_particle(c, m, in, r, mass, inertia, a[k], al[k], p, vv, ww, tt, pres, pressure + i))
+      pressure[i] = pres;
+    pos_out[i] = p;
+    vel_out[i] = vv;
+    omega_out[i] = ww;
+    theta_out[i] = tt;
+    if (kNext) next_key<kMulti>(c, nk, p, p_old[k], type, i, n);
+  }
 }
 
 // collide + integrate + the next step's hash in one pass over the sorted
@@ -925,13 +959,30 @@ inline dim3 grid_for(uint32_t n, int block) { return dim3((n + (uint32_t)block -
 
 }  // namespace
 
+// Particles a lane of the streaming kernels (hash, reorder, integrate): 1, 2
+// or 4 (the autotuner's choice); anything else runs as 1.
+inline int per_of(int per) { return per == 2 || per == 4 ? per : 1; }
+#define DEM_PER(MACRO, PER) \
+  do {                      \
+    if ((PER) == 4) MACRO(4); \
+    else if ((PER) == 2) MACRO(2); \
+    else MACRO(1);          \
+  } while (0)
+
 hipError_t launch_hash(const StepConst& c, hipStream_t s, int block, const float2* pos,
-                       const uint32_t* type, uint32_t* hash, uint32_t* index, uint32_t n) {
+                       const uint32_t* type, uint32_t* hash, uint32_t* index, uint32_t n, int per) {
   if (n == 0) return hipSuccess;
-  if (type && c.num_instances > 1)
-    hipLaunchKernelGGL(k_hash<true>, grid_for(n, block), dim3(block), 0, s, c, pos, type, hash, index, n);
-  else
-    hipLaunchKernelGGL(k_hash<false>, grid_for(n, block), dim3(block), 0, s, c, pos, type, hash, index, n);
+  per = per_of(per);
+  const dim3 grid = grid_for((n + (uint32_t)per - 1u) / (uint32_t)per, block);
+#define DEM_HASH(P)                                                                                        \
+  do {                                                                                                    \
+    if (type && c.num_instances > 1)                                                                      \
+      hipLaunchKernelGGL((k_hash<true, P>), grid, dim3(block), 0, s, c, pos, type, hash, index, n);       \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_hash<false, P>), grid, dim3(block), 0, s, c, pos, type, hash, index, n);      \
+  } while (0)
+  DEM_PER(DEM_HASH, per);
+#undef DEM_HASH
   return hipGetLastError();
 }
 
@@ -949,17 +1000,24 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
                           const uint32_t* hash, const uint32_t* index, const float2* o_pos,
                           const float2* o_vel, const float* o_theta, const float* o_omega,
                           const uint32_t* o_id, const uint32_t* o_type, const float* o_radius,
-                          uint32_t n, uint32_t num_cells, bool full) {
+                          uint32_t n, uint32_t num_cells, bool full, int per) {
   if (n == 0) return hipSuccess;
-#define DEM_REORDER(R, F)                                                                                     \
-  hipLaunchKernelGGL((k_reorder<R, F>), grid_for(n, block), dim3(block), 0, s, cell_start, cell_end, s_pos,    \
-                     s_vel, s_theta, s_omega, s_id, s_loc, s_type, s_radius, hash, index, o_pos, o_vel, o_theta, \
+  per = per_of(per);
+  const dim3 grid = grid_for((n + (uint32_t)per - 1u) / (uint32_t)per, block);
+#define DEM_REORDER(R, F, P)                                                                                 \
+  hipLaunchKernelGGL((k_reorder<R, F, P>), grid, dim3(block), 0, s, cell_start, cell_end, s_pos, s_vel,       \
+                     s_theta, s_omega, s_id, s_loc, s_type, s_radius, hash, index, o_pos, o_vel, o_theta,     \
                      o_omega, o_id, o_type, o_radius, n, num_cells)
-  if (s_radius) {
-    if (full) DEM_REORDER(true, true); else DEM_REORDER(true, false);
-  } else {
-    if (full) DEM_REORDER(false, true); else DEM_REORDER(false, false);
-  }
+#define DEM_REORDER_P(P)                                                     \
+  do {                                                                       \
+    if (s_radius) {                                                          \
+      if (full) DEM_REORDER(true, true, P); else DEM_REORDER(true, false, P); \
+    } else {                                                                 \
+      if (full) DEM_REORDER(false, true, P); else DEM_REORDER(false, false, P); \
+    }                                                                        \
+  } while (0)
+  DEM_PER(DEM_REORDER_P, per);
+#undef DEM_REORDER_P
 #undef DEM_REORDER
   return hipGetLastError();
 }
@@ -1106,38 +1164,45 @@ hipError_t launch_control_finish(hipStream_t s, const ControlOut& ctl, float* bx
 hipError_t launch_integrate(const StepConst& c, hipStream_t s, int block, float2* pos, float2* vel,
                             const float2* acc, float* theta, float* omega, const float* alpha,
                             const uint32_t* type, const float* radius, float* pressure, uint32_t n,
-                            const NextKeys* next) {
+                            const NextKeys* next, int per) {
   return launch_integrate_to(c, s, block, pos, vel, theta, omega, pos, vel, theta, omega, acc, alpha, type, radius,
-                             pressure, n, next, nullptr);
+                             pressure, n, next, nullptr, per);
 }
 
 hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, const float2* pos_in,
                                const float2* vel_in, const float* theta_in, const float* omega_in, float2* pos_out,
                                float2* vel_out, float* theta_out, float* omega_out, const float2* acc,
                                const float* alpha, const uint32_t* type, const float* radius, float* pressure,
-                               uint32_t n, const NextKeys* next, const uint32_t* gat) {
+                               uint32_t n, const NextKeys* next, const uint32_t* gat, int per) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
+  per = per_of(per);
+  const dim3 grid = grid_for((n + (uint32_t)per - 1u) / (uint32_t)per, block);
   const NextKeys nk = next ? *next : NextKeys{};
-#define DEM_INTEGRATE(R, M, X)                                                                          \
-  hipLaunchKernelGGL((k_integrate<R, M, X>), grid_for(n, block), dim3(block), 0, s, c, pos_in, vel_in, \
-                     theta_in, omega_in, pos_out, vel_out, theta_out, omega_out, acc, alpha, type, radius,  \
-                     pressure, nk, gat, n)
-#define DEM_INTEGRATE_X(R, M)            \
-  do {                                   \
-    if (next) DEM_INTEGRATE(R, M, true); \
-    else DEM_INTEGRATE(R, M, false);     \
+#define DEM_INTEGRATE(R, M, X, P)                                                                         \
+  hipLaunchKernelGGL((k_integrate<R, M, X, P>), grid, dim3(block), 0, s, c, pos_in, vel_in, theta_in,    \
+                     omega_in, pos_out, vel_out, theta_out, omega_out, acc, alpha, type, radius, pressure, nk, \
+                     gat, n)
+#define DEM_INTEGRATE_X(R, M, P)            \
+  do {                                      \
+    if (next) DEM_INTEGRATE(R, M, true, P); \
+    else DEM_INTEGRATE(R, M, false, P);     \
   } while (0)
-  const int mk = mat_kind(c);
-  if (radius) {
-    if (mk == kInstances) DEM_INTEGRATE_X(true, kInstances);
-    else if (mk == kOneMat) DEM_INTEGRATE_X(true, kOneMat);
-    else DEM_INTEGRATE_X(true, kManyMat);
-  } else {
-    if (mk == kInstances) DEM_INTEGRATE_X(false, kInstances);
-    else if (mk == kOneMat) DEM_INTEGRATE_X(false, kOneMat);
-    else DEM_INTEGRATE_X(false, kManyMat);
-  }
+#define DEM_INTEGRATE_P(P)                                                \
+  do {                                                                    \
+    const int mk = mat_kind(c);                                           \
+    if (radius) {                                                         \
+      if (mk == kInstances) DEM_INTEGRATE_X(true, kInstances, P);         \
+      else if (mk == kOneMat) DEM_INTEGRATE_X(true, kOneMat, P);          \
+      else DEM_INTEGRATE_X(true, kManyMat, P);                            \
+    } else {                                                              \
+      if (mk == kInstances) DEM_INTEGRATE_X(false, kInstances, P);        \
+      else if (mk == kOneMat) DEM_INTEGRATE_X(false, kOneMat, P);         \
+      else DEM_INTEGRATE_X(false, kManyMat, P);                           \
+    }                                                                     \
+  } while (0)
+  DEM_PER(DEM_INTEGRATE_P, per);
+#undef DEM_INTEGRATE_P
 #undef DEM_INTEGRATE_X
 #undef DEM_INTEGRATE
   return hipGetLastError();

@@ -255,10 +255,13 @@ int dem_sim_step_graph(dem_sim* sim, uint32_t steps);
 int dem_sim_stage_times(dem_sim* sim, uint32_t reps, float* ms);
 /* Choose block sizes per kernel, whether the sim runs collide and
  * integrate as one kernel or two, and how collide loads its neighbour
- * candidates (DEM_STEP_PREFETCH, DEM_STEP_BATCH, DEM_STEP_LIST or neither), by timing
- * candidates on the current state (the state is not modified).  Writes the
- * choice to block[5] (hash, reorder, collide, integrate, flags: the
- * DEM_STEP_* bits chosen) if non-NULL. */
+ * candidates (DEM_STEP_PREFETCH, DEM_STEP_BATCH, DEM_STEP_LIST or neither), and
+ * how many particles (1, 2 or 4) a lane of hash, reorder and integrate
+ * takes, by timing candidates on the current state (the state is not
+ * modified).  Writes the choice to block[5] (hash, reorder, collide,
+ * integrate, flags: the DEM_STEP_* bits chosen, and in bits 8-9, 10-11 and
+ * 12-13 log2 of the particles a lane of hash, reorder and integrate) if
+ * non-NULL. */
 int dem_sim_autotune(dem_sim* sim, uint32_t* block);
 /* Sort method of the sim's steps.  After a step the state is in the sorted
  * order of its keys, so the next step can re-sort: only particles whose cell
@@ -296,6 +299,13 @@ int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
  * the marked ones: fewer idle lanes on contact-dense states (not together
  * with PREFETCH or BATCH).  Same results. */
 #define DEM_STEP_LIST 0x20
+/* Particles a lane of hash, reorder and integrate: log2 of 1, 2 or 4 in bits
+ * 8-9 (hash), 10-11 (reorder) and 12-13 (integrate); 0 is one a lane.
+ * Same results. */
+#define DEM_STEP_PER_HASH(lg) (((lg) & 3) << 8)
+#define DEM_STEP_PER_REORDER(lg) (((lg) & 3) << 10)
+#define DEM_STEP_PER_INTEGRATE(lg) (((lg) & 3) << 12)
+#define DEM_STEP_PER_MASK 0x3F00
 int dem_set_step_variant(dem_context* ctx, int flags);
 /* Number of radix passes the sort uses for this sim's key range. */
 int dem_sim_sort_passes(dem_sim* sim);

@@ -38,3 +38,13 @@ def gpu_ctx():
     ctx = api.Context(0)
     yield ctx
     ctx.close()
+
+
+@pytest.fixture(autouse=True)
+def _default_step_variant(request):
+    """Tests that change the shared context's kernel variants leave it as
+    they found it (every variant gives the same results; this keeps each
+    test's timing and coverage independent of the order)."""
+    yield
+    if "gpu_ctx" in request.fixturenames:
+        request.getfixturevalue("gpu_ctx").set_step_variant()

@@ -42,10 +42,15 @@ def oracle_sorted(oracle, s):
     return system, h, i, hs, is_, cs, ce, st
 
 
+PER = [1, 2, 4]  # particles a lane of hash, reorder and integrate
+
+
+@pytest.mark.parametrize("per", PER)
 @pytest.mark.parametrize("name", list(SCENES))
-def test_hash(gpu_ctx, oracle, name):
+def test_hash(gpu_ctx, oracle, name, per):
     s = SCENES[name]
     gpu_ctx.load_scene(s)
+    gpu_ctx.set_step_variant(per_lane={"hash": per, "reorder": per, "integrate": per})
     system = oracle.System(s)
     h_ref, i_ref = oracle.calc_hash(system, s.pos, s.type)
     pos, typ = dev(s.pos), dev(s.type)
@@ -118,10 +123,12 @@ def test_sort_full_32bit_keys(n):
         ctx.close()
 
 
+@pytest.mark.parametrize("per", PER)
 @pytest.mark.parametrize("name", list(SCENES))
-def test_reorder(gpu_ctx, oracle, name):
+def test_reorder(gpu_ctx, oracle, name, per):
     s = SCENES[name]
     gpu_ctx.load_scene(s)
+    gpu_ctx.set_step_variant(per_lane={"hash": per, "reorder": per, "integrate": per})
     system, h, i, hs, is_, cs_ref, ce_ref, st_ref = oracle_sorted(oracle, s)
     n, C = s.n, s.total_cells
     rad = s.radius is not None
@@ -193,10 +200,12 @@ def integrate_inputs(s, seed=0):
     return st, acc, alpha, pres
 
 
+@pytest.mark.parametrize("per", PER)
 @pytest.mark.parametrize("name", list(SCENES))
-def test_integrate(gpu_ctx, oracle, name):
+def test_integrate(gpu_ctx, oracle, name, per):
     s = SCENES[name]
     gpu_ctx.load_scene(s)
+    gpu_ctx.set_step_variant(per_lane={"hash": per, "reorder": per, "integrate": per})
     system = oracle.System(s)
     st, acc, alpha, pres = integrate_inputs(s)
     out_ref, pres_ref = oracle.integrate(system, st, acc, alpha, pres)

@@ -78,13 +78,48 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
     prefetch = choice.pop("prefetch")
     batch = choice.pop("batch")
     copy = choice.pop("copy")
+    lists = choice.pop("list")
+    per = choice.pop("per_lane")
     assert isinstance(fused, bool) and isinstance(prefetch, bool) and isinstance(batch, bool)
-    assert not (prefetch and batch) and copy is False  # the tuner keeps the context's re-sort layout
+    assert prefetch + batch + lists <= 1 and copy is False  # the tuner keeps the context's re-sort layout
+    assert set(per) == {"hash", "reorder", "integrate"} and set(per.values()) <= {1, 2, 4}
     assert set(choice.values()) <= {64, 128, 256}
     sim.step(4)
     ref.step(4)
     compare(sim.download(), ref.state())
     sim.close()
+
+
+PER_LANE = [{"hash": 2, "reorder": 4, "integrate": 2}, {"hash": 4, "reorder": 2, "integrate": 4}]
+
+
+@pytest.mark.parametrize("per", PER_LANE)
+@pytest.mark.parametrize("name", ["pack_1024", "ensemble_5", "pile_wrap_attract", "mixed_pack_1024"])
+def test_particles_per_lane_equal(gpu_ctx, name, per):
+    """Two or four particles a lane in hash, reorder and integrate (split and
+    fused layouts, radix-sorted and re-sorted steps): the same bits as one a
+    lane."""
+    make, steps = CASES[name]
+    s = make()
+    for fused in (False, True):
+        ctx2 = api.Context(0)
+        ctx2.load_scene(s)
+        gpu_ctx.load_scene(s)
+        gpu_ctx.set_step_variant(fused)
+        ctx2.set_step_variant(fused, per_lane=per)
+        a = api.Simulation(gpu_ctx, s)
+        b = api.Simulation(ctx2, s)
+        for chunk in (1, steps):
+            a.step_graph(chunk)
+            b.step_graph(chunk)
+            da, db = a.download(), b.download()
+            for k in ("hash", "index", "cellStart", "cellEnd", "id", "loc", "pos", "vel", "theta", "omega", "acc",
+                      "alpha", "pressure"):
+                assert bits_equal(da[k], db[k]), (fused, k)
+        a.close()
+        b.close()
+        ctx2.close()
+    gpu_ctx.set_step_variant()
 
 
 @pytest.mark.parametrize("variant", [(True, False), (False, True), (True, True), (False, False, True),
