@@ -70,6 +70,7 @@ def stage_bytes(layout: str, radius: bool, typed: bool) -> tuple[dict, dict]:
     if layout == "fused":
         pp = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 76 + 2 * r, "collide": 64 + r + t, "integrate": 0}
         per_cell["sort"] = 4
+
     elif layout == "copy":
         pp = {"hash": 0, "sort": 12, "cell_fill": 0, "reorder": 44 + 2 * r,
               "collide": 4 + 8 + r + t + 12 + 16, "integrate": 4 + 8 + 16 + 12 + r + t + 24}
@@ -298,6 +299,48 @@ def cpu_baseline(scene, steps: int, threads: int, seconds: float) -> dict:
     }
 
 
+def stage_loop(ctx, scene, steps: int, warmup: int) -> dict:
+    """The reference's own step loop (SimLooping, src/main.cu:431-507)
+    through the drop-in stage calls: calcHash, sortParticles (radix sort
+    every step), reorderDataAndFindCellStart (full reorder into the other
+    buffer set), collide and integrateSystem, then the buffer swap; eager
+    launches on the context stream, as the reference issues them."""
+    from demso_amd import api
+
+    n, cells = scene.n, scene.total_cells
+    v = api.allocateVectors(scene, ctx)
+    v["s1"].upload(scene)
+    a, b = v["s1"], v["s2"]
+    h, idx = v["gridParticleHash"], v["gridParticleIndex"]
+    cs, ce = v["cellStart"], v["cellEnd"]
+
+    def step(a, b):
+        api.calcHash(ctx, a.pos, idx, h, n)
+        api.sortParticles(ctx, h, idx, n)
+        api.reorderDataAndFindCellStart(ctx, cs, ce, b.pos, b.vel, b.theta, b.omega, b.id, b.loc, b.type, h, idx,
+                                        a.pos, a.vel, a.theta, a.omega, a.id, a.type, n, cells,
+                                        sortedRadius=b.radius, oldRadius=a.radius)
+        api.collide(ctx, b.pos, b.vel, v["acc"], b.omega, v["alpha"], b.type, cs, ce, n, cells, v["pressure"],
+                    radius=b.radius)
+        api.integrateSystem(ctx, b.pos, b.vel, v["acc"], b.theta, b.omega, v["alpha"], b.type, v["pressure"], n,
+                            radius=b.radius)
+        return b, a
+
+    for _ in range(warmup):
+        a, b = step(a, b)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a, b = step(a, b)
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    api.desAllocateVectors(v)
+    return {"value": n * steps / secs, "unit": "particle-steps/s", "ms_per_step": 1e3 * secs / steps,
+            "steps": steps, "sort_passes": (max(cells - 1, 1).bit_length() + 7) // 8,
+            "note": "the reference's stage loop (hash, radix sort, full reorder, collide, integrate, swap) "
+                    "through the C ABI, eager launches"}
+
+
 def fixed_step_bytes(n: int, cells: int) -> int:
     """The fixed algorithmic figure of one full step, 212 B per particle and
     16 B per cell (hash 16, sort 16, reorder 76, collide 40, integrate 64 per
@@ -488,6 +531,9 @@ def main() -> int:
         sim.close()
         extra["calibration"] = calibration_run(ctx, scene, args.adj_steps or 1000, args.iters, 100, args.settle)
 
+    if args.workload in ("settling", "settling_dense", "column_collapse") and not args.no_extras:
+        sim.close()
+        extra["drop_in_stage_loop"] = stage_loop(ctx, scene, max(10, min(args.steps, 50)), 3)
     if args.workload == "settling" and not args.no_extras:
         sim.close()
         extra["ensemble_speedup"] = ensemble_speedup(ctx)
