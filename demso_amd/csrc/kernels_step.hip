@@ -72,11 +72,7 @@ __device__ __forceinline__ double pressure_den(float r) {
 // ---------------------------------------------------------------- calcHash --
 // Cell key of one particle: calcGridHash (src/particles_kernel.cuh:123-130)
 // with the instance's key base; clamped or wrapped as the flags say.
-template <bool kMulti>
-__device__ __forceinline__ uint32_t cell_key(const StepConst& c, float2 p, const uint32_t* __restrict__ type,
-                                             uint32_t i) {
-  const Instance in =
-      kMulti ? c.inst[clamp_idx(c.mat[clamp_idx(type[i], c.num_types)].inst, c.num_instances)] : c.inst0;
+__device__ __forceinline__ uint32_t cell_key_in(const StepConst& c, const Instance& in, float2 p) {
   int gx, gy;
   grid_pos(in, p, gx, gy);
   uint32_t cx, cy;
@@ -89,6 +85,14 @@ __device__ __forceinline__ uint32_t cell_key(const StepConst& c, float2 p, const
     cy = (uint32_t)gy & (in.grid[1] - 1u);
   }
   return in.cell_base + cy * in.grid[0] + cx;
+}
+
+template <bool kMulti>
+__device__ __forceinline__ uint32_t cell_key(const StepConst& c, float2 p, const uint32_t* __restrict__ type,
+                                             uint32_t i) {
+  const Instance in =
+      kMulti ? c.inst[clamp_idx(c.mat[clamp_idx(type[i], c.num_types)].inst, c.num_instances)] : c.inst0;
+  return cell_key_in(c, in, p);
 }
 
 // calcHashD (src/particles_kernel.cuh:133-151).  kPer particles a lane: a
@@ -490,17 +494,21 @@ constexpr int kBatchN = 1;
 // instance with several rows, or an ensemble of instances.
 constexpr int kOneMat = 0, kManyMat = 1, kInstances = 2;
 
-template <bool kRadius, int kMat, int kMode, bool kControl, bool kO>
+// kUni: every lane of the wave has material row t_uni (a wave-uniform value,
+// so the material row and the instance geometry are scalar loads and the
+// grid arithmetic scalar); the ensemble kernels take this path for every
+// wave whose particles all belong to one instance.
+template <bool kRadius, int kMat, int kMode, bool kControl, bool kO, bool kUni = false>
 __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i, const float2* __restrict__ pos,
                                                  const Dyn& d, const uint32_t* __restrict__ type,
                                                  const float* __restrict__ radius,
                                                  const uint32_t* __restrict__ cell_start,
                                                  const uint32_t* __restrict__ cell_end, uint32_t n, Own& own,
                                                  const ControlOut& ctl, float& fx_out, float& fy_out,
-                                                 float& moment_out, float& pres_out) {
+                                                 float& moment_out, float& pres_out, uint32_t t_uni = 0) {
   constexpr bool kMulti = kMat == kInstances, kOne = kMat == kOneMat;
   PairCtx q;
-  own.t = kOne ? 0u : clamp_idx(type[i], c.num_types);
+  own.t = kOne ? 0u : (kUni ? t_uni : clamp_idx(type[i], c.num_types));
   q.tA = own.t;
   q.mA = c.mat[own.t];
   own.m = q.mA;
@@ -713,8 +721,20 @@ __device__ __forceinline__ void collide_kernel_body(DEM_COLLIDE_PARAMS) {
   Own own;
   float fx, fy, moment, pres;
   const Dyn d{vel, omega, gat};
-  collide_particle<kRadius, kMat, kMode, kControl, kO>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
-                                                       ctl, fx, fy, moment, pres);
+  bool done = false;
+  if constexpr (kMat == kInstances) {
+    // waves inside one instance (all but the few that straddle a boundary)
+    const uint32_t t = clamp_idx(type[i], c.num_types);
+    const uint32_t t0 = __builtin_amdgcn_readfirstlane(t);
+    if (__ballot(t != t0) == 0) {
+      collide_particle<kRadius, kMat, kMode, kControl, kO, true>(c, i, pos, d, type, radius, cell_start, cell_end, n,
+                                                                 own, ctl, fx, fy, moment, pres, t0);
+      done = true;
+    }
+  }
+  if (!done)
+    collide_particle<kRadius, kMat, kMode, kControl, kO>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
+                                                         ctl, fx, fy, moment, pres);
   acc[i] = make_float2(fx / own.mass, fy / own.mass);
   alpha[i] = moment / own.inertia;
   pressure[i] = pres;
@@ -847,11 +867,10 @@ __device__ __forceinline__ bool integrate_particle(const StepConst& c, const Mat
 // state stays in this order until the next sort).  The key the particle is
 // sorted by is the hash of its position before integrate, p_old, so it is
 // recomputed here rather than read.
-template <bool kMulti>
 __device__ __forceinline__ void next_key(const StepConst& c, const NextKeys& nk, float2 p, float2 p_old,
-                                         const uint32_t* __restrict__ type, uint32_t i, uint32_t n) {
-  const uint32_t k = cell_key<kMulti>(c, p, type, i);
-  const bool mover = k != cell_key<kMulti>(c, p_old, type, i);
+                                         const Instance& in, uint32_t i, uint32_t n) {
+  const uint32_t k = cell_key_in(c, in, p);
+  const bool mover = k != cell_key_in(c, in, p_old);
   if (mover) nk.key[i] = k;  // a stayer's key is its sorted key
   const uint64_t moved = __ballot(mover);
   if (__lane_id() == 0) {  // lane 0 holds the wave's first index, a multiple of 64
@@ -901,20 +920,32 @@ __global__ __launch_bounds__(256) void k_integrate(
   for (int k = 0; k < kPer; ++k) {
     const uint32_t i = base + (uint32_t)k * blockDim.x;
     if (i >= n) continue;
+    auto run = [&](const Material& m, const Instance& in) {
+      float r, mass, inertia;
+      body<kRadius>(m, rad + k, 0u, r, mass, inertia);
+      float2 p = p_old[k], vv = v[k];
+      float ww = w[k], tt = th[k];
+      float pres = 0.0f;
+      if (integrate_particle(c, m, in, r, mass, inertia, a[k], al[k], p, vv, ww, tt, pres, pressure + i))
+        pressure[i] = pres;
+      pos_out[i] = p;
+      vel_out[i] = vv;
+      omega_out[i] = ww;
+      theta_out[i] = tt;
+      if (kNext) next_key(c, nk, p, p_old[k], in, i, n);
+    };
+    if constexpr (kMulti) {
+      // a wave inside one instance reads its material row and geometry as
+      // scalars
+      const uint32_t t0 = __builtin_amdgcn_readfirstlane(t[k]);
+      if (__ballot(t[k] != t0) == 0) {
+        const Material m = c.mat[t0];
+        run(m, c.inst[clamp_idx(m.inst, c.num_instances)]);
+        continue;
+      }
+    }
     const Material m = c.mat[t[k]];
-    const Instance in = kMulti ? c.inst[clamp_idx(m.inst, c.num_instances)] : c.inst0;
-    float r, mass, inertia;
-    body<kRadius>(m, rad + k, 0u, r, mass, inertia);
-    float2 p = p_old[k], vv = v[k];
-    float ww = w[k], tt = th[k];
-    float pres = 0.0f;
-    if (integrate_particle(c, m, in, r, mass, inertia, a[k], al[k], p, vv, ww, tt, pres, pressure + i))
-      pressure[i] = pres;
-    pos_out[i] = p;
-    vel_out[i] = vv;
-    omega_out[i] = ww;
-    theta_out[i] = tt;
-    if (kNext) next_key<kMulti>(c, nk, p, p_old[k], type, i, n);
+    run(m, kMulti ? c.inst[clamp_idx(m.inst, c.num_instances)] : c.inst0);
   }
 }
 
@@ -952,7 +983,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   vel_out[i] = v;
   omega_out[i] = w;
   theta_out[i] = th;
-  next_key<kMat == kInstances>(c, nk, p, own.p, type, i, n);
+  next_key(c, nk, p, own.p, own.in, i, n);
 }
 
 inline dim3 grid_for(uint32_t n, int block) { return dim3((n + (uint32_t)block - 1u) / (uint32_t)block); }
