@@ -511,7 +511,10 @@ def main() -> int:
     tfile = ROOT / "profiles" / f"traffic_{args.workload}.json"
     if tfile.exists():
         t = json.loads(tfile.read_text())
-        traffic = t.get(dom)
+        # per candidate mode where the file has one (collide's traffic differs a little by mode)
+        mode = (args.collide_mode or "plain") if not tuned else next(
+            (k for k in ("prefetch", "batch", "list") if tuned.get(k)), "plain")
+        traffic = t.get(f"{dom}_{mode}", t.get(dom))
 
     cpu = None
     if args.cpu_steps > 0:
