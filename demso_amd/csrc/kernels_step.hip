@@ -526,7 +526,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   own.v = q.v;
   own.w = q.w;
   q.den = pressure_den(q.rA);
-  q.rden = kMode == kList ? 1.0 / q.den : 0.0;
+  q.rden = 0.0;  // the contact-list walk sets it before its first contact
   q.dt = c.dt;
   const bool clamp = c.flags & DEM_GRID_CLAMP;
   const int G0 = (int)in.grid[0], G1 = (int)in.grid[1];
@@ -618,6 +618,8 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       rsum.b0 = b0[0];
       rsum.b1 = b1[0];
       int row = 0;
+      // the pressure terms' reciprocal, only where a candidate may touch
+      if (h[0] | h[1] | h[2]) q.rden = 1.0 / q.den;
       while (h[0] | h[1] | h[2]) {
         const bool in0 = h[0] != 0, in1 = !in0 && h[1] != 0;
         const int y = in0 ? 0 : (in1 ? 1 : 2);
@@ -735,8 +737,19 @@ __device__ __forceinline__ void collide_kernel_body(DEM_COLLIDE_PARAMS) {
   if (!done)
     collide_particle<kRadius, kMat, kMode, kControl, kO>(c, i, pos, d, type, radius, cell_start, cell_end, n, own,
                                                          ctl, fx, fy, moment, pres);
-  acc[i] = make_float2(fx / own.mass, fy / own.mass);
-  alpha[i] = moment / own.inertia;
+  // A wave whose discs all feel no force stores zeros without the three
+  // divisions: the sums start at +0 and are never -0, so fx, fy and moment
+  // are +0 exactly when nothing acted, and +0 over a positive finite mass or
+  // inertia is +0.
+  const bool quiet = fx == 0.0f && fy == 0.0f && moment == 0.0f && own.mass > 0.0f && own.inertia > 0.0f &&
+                     own.mass <= 3.402823466e38f && own.inertia <= 3.402823466e38f;
+  if (__ballot(!quiet) == 0) {
+    acc[i] = make_float2(0.0f, 0.0f);
+    alpha[i] = 0.0f;
+  } else {
+    acc[i] = make_float2(fx / own.mass, fy / own.mass);
+    alpha[i] = moment / own.inertia;
+  }
   pressure[i] = pres;
 }
 
