@@ -459,7 +459,7 @@ def main() -> int:
         scene = tuning.with_cell_multiplier(scene, cell_tuning["multiplier"])
     if args.collide_mode:
         ctx.set_step_variant(prefetch=args.collide_mode == "prefetch", batch=args.collide_mode == "batch",
-                             list=args.collide_mode == "list")
+                             contact_lists=args.collide_mode == "list")
     sim = api.Simulation(ctx, scene)
     n, cells = scene.n, scene.total_cells
 
@@ -513,7 +513,7 @@ def main() -> int:
         t = json.loads(tfile.read_text())
         # per candidate mode where the file has one (collide's traffic differs a little by mode)
         mode = (args.collide_mode or "plain") if not tuned else next(
-            (k for k in ("prefetch", "batch", "list") if tuned.get(k)), "plain")
+            (m for k, m in (("prefetch", "prefetch"), ("batch", "batch"), ("contact_lists", "list")) if tuned.get(k)), "plain")
         traffic = t.get(f"{dom}_{mode}", t.get(dom))
 
     cpu = None

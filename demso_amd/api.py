@@ -148,7 +148,7 @@ class Context:
             self.set_materials(particle_props(scene, single_instance=False))
 
     def set_step_variant(self, fused: bool = False, prefetch: bool = False, batch: bool = False,
-                         copy: bool = False, wide: bool = False, list: bool = False,
+                         copy: bool = False, wide: bool = False, contact_lists: bool = False,
                          per_lane: Optional[dict] = None) -> None:
         """Owned simulations: collide + integrate as one kernel or two, and
         how collide loads its candidates: one at a time, one ahead
@@ -156,13 +156,13 @@ class Context:
         makes re-sorted steps reorder the whole state into the other buffer
         instead of moving only the particles whose slot changes; wide
         makes collide use 64-bit addresses (as states over 2^29 particles
-        or 2^30 cells do); list makes collide mark the candidates that may
+        or 2^30 cells do); contact_lists makes collide mark the candidates that may
         touch before running the contact law over them (contact-dense
         states); per_lane gives the particles a lane (1, 2 or 4) of "hash",
         "reorder" and "integrate".  The results are the same in every
         combination."""
         flags = (int(bool(fused)) | (2 if prefetch else 0) | (4 if batch else 0) | (8 if copy else 0)
-                 | (16 if wide else 0) | (32 if list else 0))
+                 | (16 if wide else 0) | (32 if contact_lists else 0))
         lg = {1: 0, 2: 1, 4: 2}
         for shift, k in ((8, "hash"), (10, "reorder"), (12, "integrate")):
             per = (per_lane or {}).get(k, 1)
@@ -526,7 +526,7 @@ class Simulation:
         self.ctx._check(self.lib.dem_sim_autotune(self.handle, b))
         return {"hash": b[0], "reorder": b[1], "collide": b[2], "integrate": b[3], "fused": bool(b[4] & 1),
                 "prefetch": bool(b[4] & 2), "batch": bool(b[4] & 4), "copy": bool(b[4] & 8),
-                "list": bool(b[4] & 32),
+                "contact_lists": bool(b[4] & 32),
                 "per_lane": {"hash": 1 << ((b[4] >> 8) & 3), "reorder": 1 << ((b[4] >> 10) & 3),
                              "integrate": 1 << ((b[4] >> 12) & 3)}}
 

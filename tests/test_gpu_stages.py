@@ -151,7 +151,7 @@ def test_reorder(gpu_ctx, oracle, name, per):
 
 
 COLLIDE_VARIANTS = {"plain": {}, "prefetch": {"prefetch": True}, "batch": {"batch": True}, "wide": {"wide": True},
-                    "list": {"list": True}}
+                    "list": {"contact_lists": True}}
 
 
 @pytest.mark.parametrize("variant", list(COLLIDE_VARIANTS))
@@ -278,3 +278,16 @@ def test_error_codes(gpu_ctx):
     with pytest.raises(api.DemError):
         gpu_ctx.set_params(bad, api.particle_props(s, True))
     gpu_ctx.load_scene(s)
+
+
+def test_step_variant_errors(gpu_ctx):
+    """Exclusive candidate modes, particles a lane other than 1, 2, 4 and
+    unknown flags are refused with a message; the context keeps working."""
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    for flags in (0x2 | 0x20, 0x4 | 0x20, 0x2 | 0x4, 3 << 8, 3 << 12, 0x4000):
+        assert lib.dem_set_step_variant(h, flags) == -1, hex(flags)
+        assert lib.dem_last_error(h)
+    with pytest.raises(ValueError):
+        gpu_ctx.set_step_variant(per_lane={"hash": 3})
+    gpu_ctx.set_step_variant(contact_lists=True, per_lane={"hash": 4, "reorder": 2, "integrate": 4})
+    gpu_ctx.set_step_variant()

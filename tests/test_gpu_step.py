@@ -78,7 +78,7 @@ def test_autotune_keeps_state(gpu_ctx, oracle):
     prefetch = choice.pop("prefetch")
     batch = choice.pop("batch")
     copy = choice.pop("copy")
-    lists = choice.pop("list")
+    lists = choice.pop("contact_lists")
     per = choice.pop("per_lane")
     assert isinstance(fused, bool) and isinstance(prefetch, bool) and isinstance(batch, bool)
     assert prefetch + batch + lists <= 1 and copy is False  # the tuner keeps the context's re-sort layout
@@ -235,7 +235,7 @@ def test_contact_lists_on_dense_packs(gpu_ctx, oracle, mult):
     ref = oracle.Sim(s, "f32")
     ref.step(1)
     ctxs, sims = [], []
-    for variant in ({}, {"list": True}, {"list": True, "fused": True}):
+    for variant in ({}, {"contact_lists": True}, {"contact_lists": True, "fused": True}):
         ctx = api.Context(0)
         ctx.load_scene(s)
         ctx.set_step_variant(**variant)
