@@ -148,6 +148,7 @@ PROTOTYPES = {
     "dem_sim_set_sort_mode": (c_int, [c_void_p, c_int]),
     "dem_set_step_variant": (c_int, [c_void_p, c_int]),
     "dem_sim_mover_count": (c_int, [c_void_p, POINTER(c_uint32)]),
+    "dem_sim_resort_counts": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
     "dem_sim_get_by_id": (c_int, [c_void_p, P, P]),
     "dem_sim_set_by_id": (c_int, [c_void_p, P, P]),
     "dem_gather_rows": (c_int, [c_void_p, P, c_uint32] + [P] * 10 + [c_uint32, P]),

@@ -544,6 +544,14 @@ class Simulation:
         self.ctx._check(self.lib.dem_sim_mover_count(self.handle, byref(v)))
         return int(v.value)
 
+    def resort_counts(self) -> dict:
+        """The last re-sort's movers and, in place, the particles it listed
+        (moved to another slot or given a new key)."""
+        self.ctx.synchronize()
+        a, b = c_uint32(), c_uint32()
+        self.ctx._check(self.lib.dem_sim_resort_counts(self.handle, byref(a), byref(b)))
+        return {"movers": int(a.value), "listed": int(b.value)}
+
     def get_by_id(self):
         """Current positions and velocities as (n, 2) GPU tensors indexed by particle id."""
         import torch

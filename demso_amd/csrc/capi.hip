@@ -822,13 +822,35 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
 
 bool sim_inplace(const dem_sim* m, const StepHook*) { return !m->ctx->copy && !m->ctx->fuse; }
 
-bool sim_use_resort(const dem_sim* m) {
+// In place, the re-sort stays cheaper than the radix sort while it lists at
+// most n / kListedFrac particles.
+constexpr uint32_t kListedFrac = 4;
+
+bool sim_use_resort(dem_sim* m) {
   if (!m->resort_ok || m->sort_mode == 1) return false;
   if (m->sort_mode == 2) return true;
-  // The re-sort's movers step is one workgroup: fast up to a few thousand
-  // movers, slower than the radix sort far beyond.
-  const uint32_t last = *(volatile const uint32_t*)m->h_count;
-  return last <= std::max(4096u, m->n / 64u);
+  // The radix sort and full reorder cost about the same every step.  The
+  // re-sort's movers step is one workgroup, fast up to a few thousand
+  // movers; in place, every particle it lists (its slot changed: each mover
+  // shifts the particles between its old and new slot) has its state staged
+  // and written back.  Past either limit the steps radix sort, and the
+  // re-sort is tried again after 1, 2, 4 ... 64 radix-sorted steps (its
+  // counts are only measured by a re-sort).  The counts are the latest the
+  // device has written, which may trail the steps already enqueued.
+  const volatile uint32_t* h = m->h_count;
+  const uint32_t movers = h[0], listed = h[1];
+  const bool cheap = movers <= std::max(4096u, m->n / 64u) && (!sim_inplace(m) || listed <= m->n / kListedFrac);
+  if (cheap) {
+    m->radix_run = m->radix_wait = 0;
+    return true;
+  }
+  if (m->radix_run < m->radix_wait) {
+    ++m->radix_run;
+    return false;
+  }
+  m->radix_run = 0;
+  m->radix_wait = m->radix_wait ? std::min(2u * m->radix_wait, 64u) : 1u;
+  return true;
 }
 
 // The sim's cell arrays were sized at creation; a later parameter change
@@ -956,14 +978,15 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   m->shift.omega = m->omega[kScratch];
   m->shift.capacity = n;
   if (!r) {
-    if (hipHostMalloc((void**)&m->h_count, sizeof(uint32_t), hipHostMallocMapped) != hipSuccess) {
+    if (hipHostMalloc((void**)&m->h_count, 2 * sizeof(uint32_t), hipHostMallocMapped) != hipSuccess) {
       m->h_count = nullptr;
-      r = fail(ctx, DEM_ERR_ALLOC, "hipHostMalloc(mover count)");
+      r = fail(ctx, DEM_ERR_ALLOC, "hipHostMalloc(re-sort counts)");
     } else {
-      *m->h_count = 0;
+      m->h_count[0] = m->h_count[1] = 0;
       void* dp = nullptr;
       if (hipHostGetDevicePointer(&dp, m->h_count, 0) != hipSuccess) r = fail(ctx, DEM_ERR_ALLOC, "hipHostGetDevicePointer");
       m->resort.host_count = (uint32_t*)dp;
+      m->shift.host_listed = dp ? (uint32_t*)dp + 1 : nullptr;
     }
   }
   m->resort.capacity = n;
@@ -1018,7 +1041,8 @@ int dem_sim_upload(dem_sim* m, const float* pos, const float* vel, const float* 
   DEM_HIP(ctx, hipStreamSynchronize(s), "upload sync");
   m->resort_ok = false;  // an uploaded state is in no known sorted order
   m->perm_list = false;
-  *m->h_count = 0;
+  m->h_count[0] = m->h_count[1] = 0;
+  m->radix_run = m->radix_wait = 0;
   return DEM_OK;
 }
 
@@ -1122,6 +1146,14 @@ int dem_sim_set_sort_mode(dem_sim* m, int mode) {
 int dem_sim_mover_count(dem_sim* m, uint32_t* count) {
   if (!m || !count) return m ? fail(m->ctx, DEM_ERR_INVALID, "count is NULL") : DEM_ERR_INVALID;
   *count = *(volatile const uint32_t*)m->h_count;
+  return DEM_OK;
+}
+
+int dem_sim_resort_counts(dem_sim* m, uint32_t* movers, uint32_t* listed) {
+  if (!m || !movers || !listed) return m ? fail(m->ctx, DEM_ERR_INVALID, "NULL count") : DEM_ERR_INVALID;
+  const volatile uint32_t* h = m->h_count;
+  *movers = h[0];
+  *listed = h[1];
   return DEM_OK;
 }
 

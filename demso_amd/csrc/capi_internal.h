@@ -96,7 +96,12 @@ struct dem_sim {
   // except at the listed slots, and sidx is rebuilt from the list on demand.
   dem::ShiftScratch shift;
   bool perm_list = false;
-  uint32_t* h_count = nullptr;  // host-mapped mirror of the last step's mover count
+  // host-mapped: [0] the mover count of the last re-sort, [1] the particles
+  // the last in-place re-sort listed (slot or key changed)
+  uint32_t* h_count = nullptr;
+  // sort_mode 0 after a re-sort judged too costly: radix-sorted steps since
+  // then, and how many to take before the re-sort is tried again
+  uint32_t radix_run = 0, radix_wait = 0;
   bool resort_ok = false;       // state in the order of skey[cur]; cell arrays describe skey[cur]
   int sort_mode = 0;            // 0 auto, 1 radix sort every step, 2 re-sort whenever resort_ok
   int cur = 0;
@@ -137,7 +142,7 @@ struct StepHook {
 hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, const StepHook* hook = nullptr);
 // Sort method for the next step of m (auto mode: re-sort while the last
 // observed mover count is small).
-bool sim_use_resort(const dem_sim* m);
+bool sim_use_resort(dem_sim* m);
 // Whether a re-sorted step of m runs in place (see sim_step_on).
 bool sim_inplace(const dem_sim* m, const StepHook* hook = nullptr);
 

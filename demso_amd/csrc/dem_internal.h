@@ -145,6 +145,7 @@ struct ShiftScratch {
   // per slot: (epoch << 32 | new key) of the last step that listed it, so
   // that the new key of any slot is known while the state is written back
   uint64_t* tag = nullptr;
+  uint32_t* host_listed = nullptr;          // device address of a host-mapped mirror of count[0] (or null)
   uint32_t capacity = 0;                    // n
 };
 hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const ShiftState& st, const ShiftScratch& sh,

@@ -635,8 +635,10 @@ __device__ __forceinline__ void cell_record(const uint32_t* skey, const uint64_t
 // the same value).
 template <bool kRadius>
 __global__ __launch_bounds__(256) void k_shift_apply(ShiftState st, ShiftScratch sh, uint32_t num_cells, uint32_t n) {
-  const uint32_t S = min(sh.count[0], sh.capacity), E = min(sh.count[1], sh.capacity), epoch = sh.count[2];
+  const uint32_t listed = sh.count[0];
+  const uint32_t S = min(listed, sh.capacity), E = min(sh.count[1], sh.capacity), epoch = sh.count[2];
   const uint32_t stride = gridDim.x * blockDim.x;
+  if (sh.host_listed && blockIdx.x == 0 && threadIdx.x == 0) *sh.host_listed = listed;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += stride) {
     const uint32_t d = sh.dst[k];
     if (d >= n) continue;
@@ -678,6 +680,10 @@ __global__ __launch_bounds__(256) void k_shift_index(const uint32_t* __restrict_
 // Workgroups of the list kernels (grid-stride over a count known only on
 // the device): enough to cover a large list in a few passes.
 uint32_t list_blocks(uint32_t n) { return std::max(1u, std::min(128u, (n + 2047u) / 2048u)); }
+// Workgroups of the apply kernel: a step may list most of the particles
+// (bursts of movers shift long runs of them), so one per 4,096 particles,
+// at most four per CU.
+uint32_t apply_blocks(uint32_t n) { return std::max(1u, std::min(1024u, (n + 4095u) / 4096u)); }
 
 }  // namespace
 
@@ -710,7 +716,7 @@ hipError_t launch_resort_shift_apply(hipStream_t s, const ShiftState& st, const 
   if (n > sh.capacity || !sh.count || !sh.dst || !sh.tag || !st.skey || !st.loc || !st.cell_start ||
       !st.cell_end || (st.radius && !sh.radius))
     return hipErrorInvalidValue;
-  const dim3 grid(list_blocks(n));
+  const dim3 grid(apply_blocks(n));
   if (st.radius) hipLaunchKernelGGL(k_shift_apply<true>, grid, dim3(256), 0, s, st, sh, num_cells, n);
   else hipLaunchKernelGGL(k_shift_apply<false>, grid, dim3(256), 0, s, st, sh, num_cells, n);
   return hipGetLastError();

@@ -266,15 +266,24 @@ int dem_sim_autotune(dem_sim* sim, uint32_t* block);
 /* Sort method of the sim's steps.  After a step the state is in the sorted
  * order of its keys, so the next step can re-sort: only particles whose cell
  * changed are sorted and merged with the rest (same result as the radix
- * sort, bit for bit).  mode 0 (default): re-sort while the last step's count
- * of such particles stays small, else radix sort; 1: radix sort every step;
- * 2: re-sort whenever the state allows it.  The first step after
+ * sort, bit for bit).  mode 0 (default): re-sort while the last re-sort's
+ * counts (dem_sim_resort_counts) stay small (movers at most max(4096, n/64);
+ * in place, at most n/4 particles listed), else radix sort and try the
+ * re-sort again after 1, 2, 4 ... 64 radix-sorted steps; 1: radix sort every
+ * step; 2: re-sort whenever the state allows it.  The first step after
  * dem_sim_upload, dem_sim_set_by_id (positions), dem_sim_autotune or
  * dem_adj_backward always radix sorts. */
 int dem_sim_set_sort_mode(dem_sim* sim, int mode);
 /* Particles whose cell changed in the last re-sorted step (exact once the
  * stream is synchronized). */
 int dem_sim_mover_count(dem_sim* sim, uint32_t* count);
+/* The last re-sort's counts (exact once the stream is synchronized): movers
+ * as dem_sim_mover_count, and listed, the particles the last in-place
+ * re-sort moved to another slot or gave a new key (the movers plus the
+ * particles their moves shift), whose state it staged and wrote back.  No
+ * counterpart in the reference (its steps always radix sort); for callers
+ * that choose the sort mode themselves. */
+int dem_sim_resort_counts(dem_sim* sim, uint32_t* movers, uint32_t* listed);
 /* Kernel variants of the owned simulations of ctx (dem_sim_autotune sets
  * the fastest): DEM_STEP_FUSED runs collide and integrate as one kernel,
  * DEM_STEP_PREFETCH makes collide fetch each neighbour candidate one

@@ -73,7 +73,8 @@ def test_resort_equals_radix(gpu_ctx, name, graph, copy):
 
 def test_mover_count_is_cell_changes(gpu_ctx):
     """The mover count of a re-sorted step is the number of particles whose
-    cell key differs from the previous step's."""
+    cell key differs from the previous step's; the in-place re-sort lists
+    those and every particle whose slot changed."""
     s = fast_column(n_side=64, speed=3.0)
     sim = api.Simulation(gpu_ctx, s)
     sim.set_sort_mode(2)
@@ -86,10 +87,39 @@ def test_mover_count_is_cell_changes(gpu_ctx):
         k1 = np.empty(s.n, np.uint32)
         k0[d0["id"]] = d0["hash"]
         k1[d1["id"]] = d1["hash"]
+        s0 = np.empty(s.n, np.int64)
+        s1 = np.empty(s.n, np.int64)
+        s0[d0["id"]] = np.arange(s.n)
+        s1[d1["id"]] = np.arange(s.n)
         changed = int((k0 != k1).sum())
+        listed = int(((k0 != k1) | (s0 != s1)).sum())
         assert sim.mover_count() == changed
-        assert changed > 0
+        assert sim.resort_counts() == {"movers": changed, "listed": listed}
+        assert changed > 0 and listed > changed
     sim.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_auto_sort_under_bursts_equals_radix(gpu_ctx, graph):
+    """The automatic choice on a state where most particles change slot
+    every step (it alternates radix-sorted runs with re-sort trials) gives
+    the radix-sorted steps' state."""
+    s = fast_column()
+    a = api.Simulation(gpu_ctx, s)
+    b = api.Simulation(gpu_ctx, s)
+    a.set_sort_mode(1)
+    try:
+        listed = []
+        for chunk in (1, 1, 2, 3, 8, 16):
+            for sim in (a, b):
+                (sim.step_graph if graph else sim.step)(chunk)
+            listed.append(b.resort_counts()["listed"])
+            same(a.download(), b.download())
+        print(f"graph={graph}: listed after each chunk {listed}", flush=True)
+        assert max(listed) > s.n // 4  # the radix-sorted runs were taken
+    finally:
+        a.close()
+        b.close()
 
 
 def test_resort_matches_oracle_long(gpu_ctx, oracle):
