@@ -68,19 +68,24 @@ int grow(dem_context* ctx, T** p, uint32_t* cap, uint32_t need) {
 }
 
 void free_sort(SortScratch& s) {
-  if (s.keys_alt) (void)hipFree(s.keys_alt);
-  if (s.vals_alt) (void)hipFree(s.vals_alt);
-  if (s.block_counts) (void)hipFree(s.block_counts);
-  if (s.row_totals) (void)hipFree(s.row_totals);
+  for (uint32_t* p : {s.keys_alt, s.vals_alt, s.keys_alt2, s.vals_alt2, s.block_counts, s.row_totals})
+    if (p) (void)hipFree(p);
   s = SortScratch{};
 }
 
-int alloc_sort(dem_context* ctx, SortScratch& s, uint32_t n) {
-  if (s.capacity >= n && s.keys_alt) return DEM_OK;
+// second: also the second pair of arrays, which spares the in-place sort
+// (dem_sort_particles) its copy after an odd number of passes.
+int alloc_sort(dem_context* ctx, SortScratch& s, uint32_t n, bool second) {
+  if (s.capacity >= n && s.keys_alt && (!second || s.keys_alt2)) return DEM_OK;
   free_sort(s);
   const size_t cnt = dem::sort_block_count_elems(std::max(n, 1u));
-  DEM_HIP(ctx, hipMalloc((void**)&s.keys_alt, sizeof(uint32_t) * std::max(n, 1u)), "hipMalloc(sort keys)");
-  DEM_HIP(ctx, hipMalloc((void**)&s.vals_alt, sizeof(uint32_t) * std::max(n, 1u)), "hipMalloc(sort vals)");
+  const size_t words = std::max(n, 1u);
+  DEM_HIP(ctx, hipMalloc((void**)&s.keys_alt, sizeof(uint32_t) * words), "hipMalloc(sort keys)");
+  DEM_HIP(ctx, hipMalloc((void**)&s.vals_alt, sizeof(uint32_t) * words), "hipMalloc(sort vals)");
+  if (second) {
+    DEM_HIP(ctx, hipMalloc((void**)&s.keys_alt2, sizeof(uint32_t) * words), "hipMalloc(sort keys)");
+    DEM_HIP(ctx, hipMalloc((void**)&s.vals_alt2, sizeof(uint32_t) * words), "hipMalloc(sort vals)");
+  }
   DEM_HIP(ctx, hipMalloc((void**)&s.block_counts, sizeof(uint32_t) * cnt), "hipMalloc(sort counts)");
   DEM_HIP(ctx, hipMalloc((void**)&s.row_totals, sizeof(uint32_t) * 256), "hipMalloc(sort totals)");
   s.capacity = n;
@@ -202,19 +207,6 @@ int collide_mode(const dem_context* ctx) { return ctx->cmode | (ctx->wide ? kWid
 }  // namespace dem
 
 using namespace dem;
-
-namespace {
-
-__global__ void k_copy2(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
-                        uint32_t* __restrict__ da, uint32_t* __restrict__ db, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    da[i] = a[i];
-    db[i] = b[i];
-  }
-}
-
-}  // namespace
 
 namespace {
 
@@ -391,7 +383,7 @@ int dem_set_gravity(dem_context* ctx, float gx, float gy) {
 int dem_reserve(dem_context* ctx, uint32_t max_particles) {
   if (!ctx) return DEM_ERR_INVALID;
   if (int r = ensure_device(ctx)) return r;
-  return alloc_sort(ctx, ctx->sort, max_particles);
+  return alloc_sort(ctx, ctx->sort, max_particles, true);
 }
 
 // ---- stages, reference signatures ------------------------------------------
@@ -421,25 +413,19 @@ int sort_into_place(dem_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t n
   // Keys come from hashing, so they are < total_cells; without parameters
   // the full 32 bits are sorted.
   const uint32_t bits = ctx->params_set ? ctx->key_bits : 32u;
-  bool in_alt = false;
-  DEM_HIP(ctx, dem::launch_sort(ctx->stream, keys, vals, n, bits, scratch, &in_alt), "sort kernels");
-  if (in_alt) {
-    hipLaunchKernelGGL(k_copy2, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, scratch.keys_alt,
-                       scratch.vals_alt, keys, vals, n);
-    DEM_HIP(ctx, hipGetLastError(), "sort copy-back");
-  }
+  DEM_HIP(ctx, dem::launch_sort_to(ctx->stream, keys, vals, keys, vals, n, bits, scratch), "sort kernels");
   return DEM_OK;
 }
 
 // ---- scratch of dem_sort_particles_noalloc ----------------------------------
 // A fixed array per GPU: the sort's second key and value arrays
 // (kNoAllocKeys words each), its per-tile digit counts (256 per tile of
-// 4096 keys) and its 256 digit totals: 33 MiB + 1 KiB, held by every GPU
+// kSortTile keys) and its 256 digit totals: 32.5 MiB + 1 KiB, held by every GPU
 // the library runs on until the process ends.  Host code never passes the
 // array's name to a kernel (on the host it is a stand-in, not a device
 // address); it uses the per-device address hipGetSymbolAddress returns.
 constexpr uint32_t kNoAllocKeys = DEM_SORT_NOALLOC_MAX;
-constexpr size_t kNoAllocCounts = 256 * (((size_t)kNoAllocKeys + 4095) / 4096);
+constexpr size_t kNoAllocCounts = 256 * (((size_t)kNoAllocKeys + dem::kSortTile - 1) / dem::kSortTile);
 constexpr size_t kNoAllocWords = 2 * (size_t)kNoAllocKeys + kNoAllocCounts + 256;
 static_assert(kNoAllocKeys % 64 == 0, "sub-arrays stay 256-byte aligned");
 
@@ -473,7 +459,7 @@ int dem_sort_particles(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* g
   if (numParticles <= 1) return DEM_OK;
   if (!gridParticleHash || !gridParticleIndex) return fail(ctx, DEM_ERR_INVALID, "NULL array");
   if (int r = ensure_device(ctx)) return r;
-  if (int r = alloc_sort(ctx, ctx->sort, numParticles)) return r;
+  if (int r = alloc_sort(ctx, ctx->sort, numParticles, true)) return r;
   return sort_into_place(ctx, gridParticleHash, gridParticleIndex, numParticles, ctx->sort);
 }
 
@@ -736,13 +722,8 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
     e = dem::launch_fill(s, m->resort.tile_count, 0u, tiles);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], s);
-    bool in_alt = false;
-    e = dem::launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt);
-    if (e != hipSuccess) return e;
-    const uint32_t* rk = in_alt ? m->sort.keys_alt : m->key;
-    const uint32_t* ri = in_alt ? m->sort.vals_alt : m->idx;
-    hipLaunchKernelGGL(k_copy2, dim3((n + 255) / 256), dim3(256), 0, s, rk, ri, m->skey[t], m->sidx, n);
-    e = hipGetLastError();
+    // the last pass writes the sorted keys and index where the step keeps them
+    e = dem::launch_sort_to(s, m->key, m->idx, m->skey[t], m->sidx, n, ctx->key_bits, m->sort);
     if (e != hipSuccess) return e;
   }
   m->sorted_key = m->skey[t];
@@ -948,7 +929,7 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   if (!r) r = sim_alloc(m, &m->cs[0], num_cells);
   if (!r) r = sim_alloc(m, &m->cs[1], num_cells);
   if (!r) r = sim_alloc(m, &m->cell_end, num_cells);
-  if (!r) r = alloc_sort(ctx, m->sort, n);
+  if (!r) r = alloc_sort(ctx, m->sort, n, false);
   const uint32_t words = (n + 31u) / 32u, tiles = (n + dem::kResortTile - 1u) / dem::kResortTile;
   for (int b = 0; b < 2 && r == DEM_OK; ++b) {
     if (!r) r = sim_alloc(m, &m->skey[b], n);
@@ -1428,11 +1409,14 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   const uint32_t n = m->n;
   // Populate the scratch buffer t (the next step's sorted state) once; the
   // current state in buffer o is never written.
-  bool in_alt = false;
+  // (the sorted keys and index go to the re-sort's second mover arrays, n
+  // words each, which every re-sort rewrites before reading; the re-sort is
+  // off after tuning, and the last step's sorted keys and index stay as
+  // they are)
   DEM_HIP(ctx, dem::launch_hash(c, s, ctx->bs_hash, m->pos[o], multi(ctx) ? m->type[o] : nullptr, m->key, m->idx, n), "tune");
-  DEM_HIP(ctx, dem::launch_sort(s, m->key, m->idx, n, ctx->key_bits, m->sort, &in_alt), "tune");
-  uint32_t* sk = in_alt ? m->sort.keys_alt : m->key;
-  uint32_t* si = in_alt ? m->sort.vals_alt : m->idx;
+  uint32_t* sk = m->resort.mkey[1];
+  uint32_t* si = m->resort.mpos[1];
+  DEM_HIP(ctx, dem::launch_sort_to(s, m->key, m->idx, sk, si, n, ctx->key_bits, m->sort), "tune");
   DEM_HIP(ctx, dem::launch_fill(s, m->cell_start, dem::kEmpty, m->cells), "tune");
   // the fully sorted copy into kScratch (the fused layout); the gather
   // layout's reorder writes the sorted positions there too
@@ -1468,7 +1452,7 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   if (e == hipSuccess) e = hipMalloc((void**)&ttheta, 4 * (size_t)n);
   if (e == hipSuccess) e = hipMalloc((void**)&tomega, 4 * (size_t)n);
   dem::NextKeys nk;
-  nk.key = m->key == sk ? m->idx : m->key;  // any n-word buffer other than sk
+  nk.key = m->key;  // the sort's input, free after it
   nk.bits = m->resort.bits;
   nk.tile_count = m->resort.tile_count;
   const float2* S_pos = m->pos[K];

@@ -212,16 +212,25 @@ hipError_t launch_control_finish(hipStream_t s, const struct ControlOut& ctl, fl
 struct SortScratch {
   uint32_t* keys_alt = nullptr;
   uint32_t* vals_alt = nullptr;
+  uint32_t* keys_alt2 = nullptr;     // optional second pair of arrays (see launch_sort_to)
+  uint32_t* vals_alt2 = nullptr;
   uint32_t* block_counts = nullptr;  // 256 * max_tiles
   uint32_t* row_totals = nullptr;    // 256
   uint32_t capacity = 0;             // max keys
 };
+// Keys per tile of the radix sort (its per-tile digit counts are 256 words).
+constexpr uint32_t kSortTile = 8192;
 int sort_passes(uint32_t key_bits);
 uint32_t sort_tiles(uint32_t n);
 size_t sort_block_count_elems(uint32_t n);
-// Sorts keys/vals; the result lands in (keys, vals) if *result_in_alt is set
-// to false on return, else in (scratch.keys_alt, scratch.vals_alt).
-hipError_t launch_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits,
-                       const SortScratch& scratch, bool* result_in_alt);
+// Stable sort of (keys_in, vals_in) by the low key_bits of the keys into
+// (keys_out, vals_out), which may be the input arrays themselves but no
+// scratch array.  The input arrays are overwritten (the passes alternate
+// between them and the scratch arrays when the scratch has no second pair).
+// The last pass writes the output directly, except when the output is that
+// pass's input: an in-place sort with an odd number of passes and no second
+// pair (or one pass), which ends with a copy.
+hipError_t launch_sort_to(hipStream_t s, uint32_t* keys_in, uint32_t* vals_in, uint32_t* keys_out,
+                          uint32_t* vals_out, uint32_t n, uint32_t key_bits, const SortScratch& scratch);
 
 }  // namespace dem

@@ -164,7 +164,7 @@ int dem_sort_particles(dem_context* ctx, uint32_t* gridParticleHash, uint32_t* g
 /* No counterpart in the reference.  dem_sort_particles for callers that
  * cannot allocate device memory (code under HIP graph capture, serving
  * loops, harnesses that forbid hipMalloc): the same sort and result, with
- * its scratch in a fixed per-GPU array of the library (33 MiB, held until
+ * its scratch in a fixed per-GPU array of the library (32.5 MiB, held until
  * the process ends) instead of buffers the context allocates.  Sorts at
  * most DEM_SORT_NOALLOC_MAX keys (larger counts fail with DEM_ERR_INVALID
  * before anything is launched).  All calls on one GPU share that array:

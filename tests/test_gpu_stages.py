@@ -89,7 +89,7 @@ def _sort_case(ctx, keys, vals):
 
 
 @pytest.mark.parametrize("n", [2, 63, 64, 65, 4095, 4096, 4097, 65549, 1000003])
-@pytest.mark.parametrize("cells", [4186, 11193546])
+@pytest.mark.parametrize("cells", [200, 4186, 11193546])
 def test_sort_stable_bounded_keys(gpu_ctx, n, cells):
     """Keys below the configured cell count (2 or 3 radix passes)."""
     s = sc.column_collapse(cols=4, rows=4)
@@ -104,6 +104,36 @@ def test_sort_stable_bounded_keys(gpu_ctx, n, cells):
     swap = rng.integers(0, max(1, n - 1), size=max(1, n // 50))
     keys[swap], keys[np.minimum(swap + 1, n - 1)] = keys[np.minimum(swap + 1, n - 1)], keys[swap]
     assert _sort_case(gpu_ctx, keys, np.arange(n, dtype=np.uint32))
+
+
+def _digit_patterns(n, rng):
+    """Keys whose 64-key groups share digits in every arrangement the ranking
+    distinguishes: one digit a group, two contiguous halves, interleaved
+    repeats, sorted runs, descending keys, and random keys whose digits
+    take only two values each."""
+    i = np.arange(n, dtype=np.int64)
+    two = (rng.integers(0, 2, size=(n, 3)) * np.array([0x80, 0x8000, 0x800000])).sum(1)
+    return {
+        "constant": np.full(n, 0x123456, np.int64),
+        "halves": np.where(i % 64 < 32, 0x00A1B2, 0x00A1B3),
+        "interleaved": 0x10101 * (i % 3),
+        "runs": i // 7,
+        "descending": n - 1 - i,
+        "two_values_a_digit": two,
+    }
+
+
+@pytest.mark.parametrize("n", [64, 4097, 65549])
+def test_sort_digit_patterns(gpu_ctx, n):
+    """Stable order for key layouts that take the one-digit-a-group shortcut,
+    the LDS peer masks, or both within one tile (24-bit keys, 3 passes)."""
+    s = sc.column_collapse(cols=4, rows=4)
+    s.instances[0] = sc.Instance(box=(1.0, 1.0), extent=(1.0, 1.0), grid=(11193546, 1))
+    gpu_ctx.load_scene(s)
+    rng = np.random.default_rng(n)
+    for name, k in _digit_patterns(n, rng).items():
+        keys = (k % 11193546).astype(np.uint32)
+        assert _sort_case(gpu_ctx, keys, rng.permutation(n).astype(np.uint32)), name
 
 
 @pytest.mark.parametrize("n", [0, 1, 3, 5000, 300007])
