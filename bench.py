@@ -438,6 +438,9 @@ def main() -> int:
                     help="ensemble workload: skip timing the instances one by one (for profiler runs)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="finish with a plain device copy of this many MiB (reference for HBM counters)")
+    ap.add_argument("--drop-in-only", action="store_true",
+                    help="time only the reference's stage loop through the drop-in calls (--steps steps after "
+                         "--warmup) and print its line (for profiler runs of the radix sort and full reorder)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -450,6 +453,13 @@ def main() -> int:
 
     scene, workload = make_scene(args.workload)
     ctx = api.Context(0)
+    if args.drop_in_only:
+        line = stage_loop(ctx, scene, args.steps, args.warmup)
+        line.update(workload=workload, particles=scene.n, cells=scene.total_cells)
+        print(json.dumps(line), flush=True)
+        ctx.close()
+        calibration_copy(args.calibration_copy_mb)
+        return 0
     cell_tuning = None
     if args.workload == "ensemble" and not args.no_autotune:
         # the ensemble workload tunes the cell-size multiplier as well
@@ -601,18 +611,26 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     sim.close()
     ctx.close()
-    if args.calibration_copy_mb > 0:
-        # a plain copy of known size whose source is not cache-resident: the
-        # source is written, then 1 GiB of other data evicts the 256 MB MALL
-        nbytes = args.calibration_copy_mb << 20
-        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-        evict = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del evict
+    calibration_copy(args.calibration_copy_mb)
     return 0
+
+
+def calibration_copy(mb: int) -> None:
+    """A plain copy of known size whose source is not cache-resident (the
+    source is written, then 1 GiB of other data evicts the 256 MB MALL): the
+    reference for the HBM byte counters."""
+    if mb <= 0:
+        return
+    import torch
+
+    nbytes = mb << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    evict = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del evict
 
 
 if __name__ == "__main__":

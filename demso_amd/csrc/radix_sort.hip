@@ -84,7 +84,9 @@ __device__ __forceinline__ uint64_t peer_mask(uint32_t d, bool ok, uint64_t vali
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   if (ok) peers = __hip_atomic_load(&tab[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  if (ok) __hip_atomic_store(&tab[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  // one lane a digit clears its word (fewer bank conflicts than every lane)
+  if (ok && __lane_id() == (uint32_t)(__ffsll((unsigned long long)peers) - 1))
+    __hip_atomic_store(&tab[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   return peers;
 }
 
@@ -196,9 +198,10 @@ __global__ __launch_bounds__(kDThreads, 4) void k_downsweep(
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
     const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ row_totals,
     uint32_t num_tiles) {
-  // the tile's keys and values: in load order, then (after the ranking, for
-  // which its first kDWaves * kRadix * 8 bytes hold the waves' peer masks)
-  // in sorted order
+  // the tile's keys and values: in load order (keys in the first half,
+  // values in the second), then, after the ranking (for which its first
+  // kDWaves * kRadix * 8 bytes hold the waves' peer masks), in sorted order
+  // as (value << 32 | key)
   __shared__ uint64_t stage[kTile];
   __shared__ uint32_t cnt[kDWaves][kRadix];
   __shared__ uint32_t g_start[kRadix];
@@ -302,10 +305,8 @@ __global__ __launch_bounds__(kDThreads, 4) void k_downsweep(
       if (idx < n) {
         const uint32_t d = (key[it] >> shift) & (kRadix - 1);
         const uint32_t p = cnt[w][d] + rank[it];
-        if (p < kTile) {  // always true for a consistent ranking; guards the store
-          s_keys[p] = key[it];
-          s_vals[p] = val[it];
-        }
+        if (p < kTile)  // always true for a consistent ranking; guards the store
+          stage[p] = ((uint64_t)val[it] << 32) | key[it];
       }
     }
     __syncthreads();
@@ -314,11 +315,12 @@ __global__ __launch_bounds__(kDThreads, 4) void k_downsweep(
     if (next < num_tiles) load_tile<kVec>(keys_in, vals_in, next * kTile + w * (kWave * kDItems), n, kq, vq);
     const uint32_t tile_n = min(kTile, n - tile * kTile);
     for (uint32_t s = tid; s < tile_n; s += kDThreads) {
-      const uint32_t k = s_keys[s];
+      const uint64_t kv = stage[s];
+      const uint32_t k = (uint32_t)kv;
       const uint32_t g = g_start[(k >> shift) & (kRadix - 1)] + s;
       if (g < n) {  // always true for a consistent ranking; guards the store
         keys_out[g] = k;
-        vals_out[g] = s_vals[s];
+        vals_out[g] = (uint32_t)(kv >> 32);
       }
     }
   }
