@@ -55,16 +55,17 @@ __device__ __forceinline__ uint64_t composite(uint32_t key, uint32_t pos) { retu
 
 // Movers among positions [0, p): the tile's offset where p's tile has no
 // movers (its word prefixes are not written), else the word prefix plus the
-// movers before p in its word.
+// movers before p in its word.  All four words are loaded at once (one
+// round trip); the prefix is used only where it was written.
 __device__ __forceinline__ uint32_t movers_before(uint32_t p, uint32_t n, uint32_t m, const uint32_t* __restrict__ bits,
                                                   const uint32_t* __restrict__ word_prefix,
                                                   const uint32_t* __restrict__ tile_off) {
   if (p >= n) return m;
-  const uint32_t t = p / kResortTile;
-  const uint32_t o = tile_off[t];
-  if (tile_off[t + 1u] == o) return o;
-  const uint32_t w = p >> 5, b = p & 31u;
-  return word_prefix[w] + (uint32_t)__popc(bits[w] & ((1u << b) - 1u));
+  const uint32_t t = p / kResortTile, w = p >> 5, b = p & 31u;
+  const uint32_t o = tile_off[t], o1 = tile_off[t + 1u];
+  const uint32_t wp = word_prefix[w], bw = bits[w];
+  if (o1 == o) return o;
+  return wp + (uint32_t)__popc(bw & ((1u << b) - 1u));
 }
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
@@ -107,19 +108,33 @@ __device__ __forceinline__ uint32_t block_scan_1024(uint32_t x, uint32_t* wsum) 
   return off + inc - x;
 }
 
+// Compare-exchange of bitonic stage (k, j) for element i held in a register;
+// its partner i ^ j (j < 64) is held by lane ^ j of the same wave.
+__device__ __forceinline__ uint64_t bitonic_lane(uint64_t x, uint32_t i, uint32_t k, uint32_t j) {
+  const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  const uint64_t y = ((uint64_t)(uint32_t)__shfl_xor((int)hi, (int)j, kWave) << 32) |
+                     (uint32_t)__shfl_xor((int)lo, (int)j, kWave);
+  const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+  return keep_min ? (x < y ? x : y) : (x < y ? y : x);
+}
+
 // One workgroup: the movers, compacted in position order, and their stable
 // sort by key.
 //  * tile_off[t] = movers in tiles before t (tile_off[tiles] = the count),
 //    from the per-tile counts, which are then cleared for the next step's
-//    integrate (it accumulates into them);
+//    integrate (it accumulates into them); the offsets and the list of the
+//    tiles that have movers stay in LDS up to kLdsTiles tiles;
 //  * for every tile that has movers (one wave per tile): the movers before
 //    each of its 32-particle words (word_prefix; not written for the other
-//    tiles, where tile_off gives the answer) and its movers appended to
-//    (mkey[0], mpos[0]) in position order;
-//  * the sort: a bitonic sort of the unique 64-bit (key << 32 | position)
-//    values in LDS up to 4096 movers, a stable chunked LSD radix sort through
-//    global memory beyond.
+//    tiles, where tile_off gives the answer) and its movers, in position
+//    order, as (key << 32 | position) values in LDS up to kBitonicMax movers,
+//    else appended to (mkey[0], mpos[0]);
+//  * the sort: a bitonic sort of those unique values up to kBitonicMax movers
+//    (the stages whose pairs lie within a wave's 64 values in registers, by
+//    lane shuffles; only the wider ones through LDS, with a barrier each), a
+//    stable chunked LSD radix sort through global memory beyond.
 // info[0] = count, info[1] = the buffer (0 or 1) that holds the sorted movers.
+constexpr uint32_t kLdsTiles = 2048;
 __global__ __launch_bounds__(kSortThreads) void k_resort_movers(
     uint32_t* __restrict__ tile_count, uint32_t tiles, const uint32_t* __restrict__ bits,
     const uint32_t* __restrict__ key, uint32_t n, uint32_t* __restrict__ tile_off, uint32_t* __restrict__ tile_list,
@@ -131,8 +146,11 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(
   __shared__ uint32_t wcnt[kSortWaves][kDigits];
   __shared__ uint32_t base[kDigits];
   __shared__ uint32_t ctot[kDigits];
+  __shared__ uint32_t s_off[kLdsTiles];
+  __shared__ uint32_t s_list[kLdsTiles];
   __shared__ uint32_t s_carry, s_nz;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid / kWave;
+  const bool lds_tiles = tiles <= kLdsTiles;
   // the in-place placement that follows appends to these lists, under a
   // new epoch
   if (shift_count) {
@@ -152,7 +170,12 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(
     if (t < tiles) {
       tile_off[t] = carry + ex;
       tile_count[t] = 0;
-      if (cnt) tile_list[atomicAdd(&s_nz, 1u)] = t;
+      if (lds_tiles) s_off[t] = carry + ex;
+      if (cnt) {
+        const uint32_t slot = atomicAdd(&s_nz, 1u);
+        if (lds_tiles) s_list[slot] = t;
+        else tile_list[slot] = t;
+      }
     }
     __syncthreads();
     if (tid == kSortThreads - 1u) s_carry = carry + ex + cnt;
@@ -164,10 +187,11 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(
     info[0] = m;
     if (host_count) *host_count = m;
   }
+  const bool in_lds = m <= kBitonicMax;
   const uint32_t nwords = (n + 31u) / 32u;
   for (uint32_t k = w; k < nz; k += kSortWaves) {
-    const uint32_t t = tile_list[k];
-    const uint32_t off = tile_off[t];
+    const uint32_t t = lds_tiles ? s_list[k] : tile_list[k];
+    const uint32_t off = lds_tiles ? s_off[t] : tile_off[t];
     const uint32_t wa = t * kWordsPerTile + lane, wb = wa + (uint32_t)kWave;
     const uint32_t ba = wa < nwords ? bits[wa] : 0u, bb = wb < nwords ? bits[wb] : 0u;
     const uint32_t ca = (uint32_t)__popc(ba), cb = (uint32_t)__popc(bb);
@@ -178,30 +202,63 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(
     if (wb < nwords) word_prefix[wb] = rb;
     for (uint32_t x = ba; x; x &= x - 1u) {
       const uint32_t p = wa * 32u + (uint32_t)(__ffs(x) - 1);
-      mkey0[ra] = key[p];
-      mpos0[ra] = p;
+      const uint32_t kp = key[p];
+      if (in_lds) {
+        v[ra] = composite(kp, p);
+      } else {
+        mkey0[ra] = kp;
+        mpos0[ra] = p;
+      }
       ++ra;
     }
     for (uint32_t x = bb; x; x &= x - 1u) {
       const uint32_t p = wb * 32u + (uint32_t)(__ffs(x) - 1);
-      mkey0[rb] = key[p];
-      mpos0[rb] = p;
+      const uint32_t kp = key[p];
+      if (in_lds) {
+        v[rb] = composite(kp, p);
+      } else {
+        mkey0[rb] = kp;
+        mpos0[rb] = p;
+      }
       ++rb;
     }
   }
-  __syncthreads();  // the compacted movers are read by the whole workgroup below
-  if (m <= 1u) {
+  if (m == 0u) {
     if (tid == 0) info[1] = 0;
     return;
   }
 
-  if (m <= kBitonicMax) {
+  if (in_lds) {
     uint32_t P = 1;
     while (P < m) P <<= 1;
-    for (uint32_t i = tid; i < P; i += kSortThreads) v[i] = i < m ? composite(mkey0[i], mpos0[i]) : ~0ull;
-    __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+    for (uint32_t i = m + tid; i < P; i += kSortThreads) v[i] = ~0ull;  // padding sorts last
+    __syncthreads();  // the compacted values and the padding are read by every wave below
+    constexpr int kR = (int)(kBitonicMax / kSortThreads);  // values a thread holds
+    // a wave works on its r-th group of 64 values only when the group starts
+    // below P (uniform over the wave: the shuffles need every lane)
+    const uint32_t wave_first = tid - lane;
+    uint64_t x[kR];
+    // stages k = 2 .. 64: every pair lies within one wave's 64 values
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      const uint32_t i = tid + (uint32_t)r * kSortThreads;
+      x[r] = i < P ? v[i] : ~0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      if (wave_first + (uint32_t)r * kSortThreads >= P) continue;
+      for (uint32_t k = 2; k <= 64u && k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) x[r] = bitonic_lane(x[r], tid + (uint32_t)r * kSortThreads, k, j);
+    }
+    // wider stages: pairs across waves through LDS, the rest in registers
+    for (uint32_t k = 128; k <= P; k <<= 1) {
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        const uint32_t i = tid + (uint32_t)r * kSortThreads;
+        if (i < P) v[i] = x[r];
+      }
+      __syncthreads();
+      for (uint32_t j = k >> 1; j >= 64u; j >>= 1) {
         for (uint32_t i = tid; i < P; i += kSortThreads) {
           const uint32_t l = i ^ j;
           if (l > i) {
@@ -215,14 +272,30 @@ __global__ __launch_bounds__(kSortThreads) void k_resort_movers(
         }
         __syncthreads();
       }
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        const uint32_t i = tid + (uint32_t)r * kSortThreads;
+        x[r] = i < P ? v[i] : ~0ull;
+      }
+      __syncthreads();  // every value is read before the next k writes
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        if (wave_first + (uint32_t)r * kSortThreads >= P) continue;
+        for (uint32_t j = 32; j > 0; j >>= 1) x[r] = bitonic_lane(x[r], tid + (uint32_t)r * kSortThreads, k, j);
+      }
     }
-    for (uint32_t i = tid; i < m; i += kSortThreads) {
-      mkey0[i] = (uint32_t)(v[i] >> 32);
-      mpos0[i] = (uint32_t)v[i];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      const uint32_t i = tid + (uint32_t)r * kSortThreads;
+      if (i < m) {
+        mkey0[i] = (uint32_t)(x[r] >> 32);
+        mpos0[i] = (uint32_t)x[r];
+      }
     }
     if (tid == 0) info[1] = 0;
     return;
   }
+  __syncthreads();  // the compacted movers are read by the whole workgroup below
 
   // stable LSD radix sort, one digit per pass, chunks of 1024 in order
   uint32_t* kb[2] = {mkey0, mkey1};
@@ -292,21 +365,23 @@ __device__ __forceinline__ uint32_t movers_below(const uint32_t* __restrict__ sk
 
 // Insertion point of a mover's pair (k, p) among the previous sorted pairs
 // (prev_key[q], q): the number of them that are smaller.  From the cell
-// ranges of prev_key when valid, else a binary search over prev_key.
+// ranges of prev_key when valid (the cell's own range, else the start of the
+// first non-empty cell among the next 8, all loaded at once), else a binary
+// search over prev_key.
+constexpr uint32_t kNextCells = 8;
 __device__ __forceinline__ uint32_t insertion_point(uint32_t k, uint32_t p, const uint32_t* __restrict__ prev_key,
                                                     const uint32_t* __restrict__ prev_cell_start,
                                                     const uint32_t* __restrict__ prev_cell_end, uint32_t num_cells,
                                                     int cells_valid, uint32_t n) {
   if (cells_valid && k < num_cells) {
-    const uint32_t s0 = prev_cell_start[k];
-    if (s0 != kEmpty) {
-      const uint32_t e0 = prev_cell_end[k];
-      return p < s0 ? s0 : (p >= e0 ? e0 : p);
-    }
-    for (uint32_t kk = k + 1u; kk < num_cells && kk <= k + 8u; ++kk) {
-      const uint32_t s1 = prev_cell_start[kk];
-      if (s1 != kEmpty) return s1;
-    }
+    const uint32_t s0 = prev_cell_start[k], e0 = prev_cell_end[k];
+    uint32_t next[kNextCells];
+#pragma unroll
+    for (uint32_t d = 0; d < kNextCells; ++d) next[d] = k + 1u + d < num_cells ? prev_cell_start[k + 1u + d] : kEmpty;
+    if (s0 != kEmpty) return p < s0 ? s0 : (p >= e0 ? e0 : p);
+#pragma unroll
+    for (uint32_t d = 0; d < kNextCells; ++d)
+      if (next[d] != kEmpty) return next[d];
   }
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
@@ -429,8 +504,9 @@ __device__ __forceinline__ uint32_t wave_append(bool emit, uint32_t* counter) {
   return base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
 }
 
-// Sorted movers whose key is below k (keys are non-decreasing in rank).
-__device__ __forceinline__ uint32_t keys_below(const uint32_t* __restrict__ sk, uint32_t m, uint32_t k) {
+// Sorted movers whose key is below k (keys are non-decreasing in rank); sk
+// in global memory or LDS.
+__device__ __forceinline__ uint32_t keys_below(const uint32_t* sk, uint32_t m, uint32_t k) {
   uint32_t lo = 0, hi = m;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -478,14 +554,22 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
   const uint32_t tid = threadIdx.x;
 
   if (blockIdx.x < kMoverBlocks) {
+    // the sorted movers' keys in LDS when they fit (the arriving counts of
+    // the old cells are binary searches over them)
+    uint32_t* mk = reinterpret_cast<uint32_t*>(win);
+    const bool keys_lds = m <= 2u * kWindowMax;
+    if (keys_lds)
+      for (uint32_t j = tid; j < m; j += kPlaceThreads) mk[j] = sk[j];
+    __syncthreads();
     for (uint32_t r0 = blockIdx.x * kPlaceThreads; r0 < m; r0 += kMoverBlocks * kPlaceThreads) {
       const uint32_t r = r0 + tid;
       const bool ok = r < m;
-      uint32_t k = 0, p = 0, slot = n;
+      uint32_t k = 0, p = 0, slot = n, ko = kEmpty;
       if (ok) {
-        k = sk[r];
+        k = keys_lds ? mk[r] : sk[r];
         p = sp[r];
         if (p < n) {
+          ko = prev_key[p];  // the old cell, loaded with the insertion point's words
           const uint32_t P = insertion_point(k, p, prev_key, st.cell_start, st.cell_end, num_cells, 1, n);
           slot = r + P - movers_before(P, n, m, bits, word_prefix, tile_off);
         }
@@ -496,18 +580,16 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
       // the mover's old cell is left empty when all its particles moved
       // out and none moved in (checked by the cell's first mover only)
       bool emptied = false;
-      uint32_t ko = 0;
-      if (emit) {
-        ko = prev_key[p];
-        if (ko < num_cells) {
-          const uint32_t s0 = st.cell_start[ko], e0 = st.cell_end[ko];
-          if (s0 != kEmpty && s0 <= p && p < e0) {
-            const uint32_t mb0 = movers_before(s0, n, m, bits, word_prefix, tile_off);
-            if (movers_before(p, n, m, bits, word_prefix, tile_off) == mb0) {
-              const uint32_t leaving = movers_before(e0, n, m, bits, word_prefix, tile_off) - mb0;
-              const uint32_t entering = keys_below(sk, m, ko + 1u) - keys_below(sk, m, ko);
-              emptied = e0 - s0 == leaving && entering == 0;
-            }
+      if (emit && ko < num_cells) {
+        const uint32_t s0 = st.cell_start[ko], e0 = st.cell_end[ko];
+        if (s0 != kEmpty && s0 <= p && p < e0) {
+          const uint32_t mb0 = movers_before(s0, n, m, bits, word_prefix, tile_off);
+          const uint32_t mbp = movers_before(p, n, m, bits, word_prefix, tile_off);
+          const uint32_t mbe = movers_before(e0, n, m, bits, word_prefix, tile_off);
+          if (mbp == mb0) {
+            const uint32_t entering = keys_lds ? keys_below(mk, m, ko + 1u) - keys_below(mk, m, ko)
+                                               : keys_below(sk, m, ko + 1u) - keys_below(sk, m, ko);
+            emptied = e0 - s0 == mbe - mb0 && entering == 0;
           }
         }
       }

@@ -1,8 +1,9 @@
 """The re-sort of a state already in the order of its previous keys
 (only the particles that changed cell are sorted and merged back) against
 the radix sort and the oracle: identical keys, permutation, cell ranges and
-state, bit for bit, for few movers (LDS bitonic path), many movers (the
-chunked radix path of the movers workgroup) and ensembles."""
+state, bit for bit, for few movers (sorted within one wave), hundreds of
+movers (bitonic stages across waves through LDS), many movers (the chunked
+radix path of the movers workgroup) and ensembles."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,7 @@ def fast_column(n_side=128, speed=30.0, seed=5, dt=1e-4):
 CASES = {
     "pack_few_movers": (lambda: compressed_pack(seed=3), 30),
     "fast_many_movers": (lambda: fast_column(), 12),
+    "hundreds_of_movers": (lambda: fast_column(speed=3.0), 12),
     "ensemble_7": (lambda: sc.ensemble(instances=7, counts=[3000, 700, 5200, 90, 1640, 4100, 2222], seed=6), 25),
     "pile_wrap": (lambda: sc.triangle_pile(), 40),
 }
@@ -69,6 +71,8 @@ def test_resort_equals_radix(gpu_ctx, name, graph, copy):
         b.close()
     if name == "fast_many_movers":
         assert max(movers) > 4096  # the chunked radix path of the movers workgroup ran
+    if name == "hundreds_of_movers":
+        assert 64 < max(movers) <= 4096  # the LDS bitonic stages across waves ran
 
 
 def test_mover_count_is_cell_changes(gpu_ctx):
