@@ -136,6 +136,32 @@ def test_sort_digit_patterns(gpu_ctx, n):
         assert _sort_case(gpu_ctx, keys, rng.permutation(n).astype(np.uint32)), name
 
 
+@pytest.mark.parametrize("n", [2, 4097, 65549, 300007])
+@pytest.mark.parametrize("shift", [(1, 0), (0, 3), (2, 1)], ids=["keys+4B", "vals+12B", "both"])
+def test_sort_unaligned_arrays(gpu_ctx, n, shift):
+    """Caller arrays that are not 16-byte aligned (views at an offset of 1-3
+    words) take the sort's one-word loads: the same stable order."""
+    s = sc.column_collapse(cols=4, rows=4)
+    s.instances[0] = sc.Instance(box=(1.0, 1.0), extent=(1.0, 1.0), grid=(11193546, 1))
+    gpu_ctx.load_scene(s)
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 11193546, size=n).astype(np.uint32)
+    keys[: n // 3] = np.sort(keys[: n // 3])  # a nearly sorted stretch too
+    vals = rng.permutation(n).astype(np.uint32)
+    dk, dv = shift
+    hk = dev(np.concatenate([np.zeros(dk, np.uint32), keys]))
+    hv = dev(np.concatenate([np.zeros(dv, np.uint32), vals]))
+    kview, vview = hk[dk:], hv[dv:]
+    assert (kview.data_ptr() % 16 != 0) or (vview.data_ptr() % 16 != 0)
+    api.sortParticles(gpu_ctx, kview, vview, n)
+    gpu_ctx.synchronize()
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(host(kview, np.uint32), keys[order])
+    assert np.array_equal(host(vview, np.uint32), vals[order])
+    assert np.array_equal(host(hk, np.uint32)[:dk], np.zeros(dk, np.uint32))  # nothing written before the views
+    assert np.array_equal(host(hv, np.uint32)[:dv], np.zeros(dv, np.uint32))
+
+
 @pytest.mark.parametrize("n", [0, 1, 3, 5000, 300007])
 def test_sort_full_32bit_keys(n):
     """Without parameters the sort covers all 32 key bits (4 passes)."""
