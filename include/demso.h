@@ -145,7 +145,9 @@ int dem_set_instances(dem_context* ctx, const dem_instance* inst, uint32_t num_i
 int dem_set_materials(dem_context* ctx, const dem_particle_props* types, uint32_t num_types);
 /* set_gravity (src/functions.cu:663-668). */
 int dem_set_gravity(dem_context* ctx, float gx, float gy);
-/* Pre-allocate sort scratch for up to max_particles (call before capture). */
+/* Pre-allocate sort scratch for up to max_particles (call before capture):
+ * two pairs of key/value arrays (16 bytes a particle) and the per-tile digit
+ * counts (256 words per 8,192 keys). */
 int dem_reserve(dem_context* ctx, uint32_t max_particles);
 
 /* ---- the five stages, reference signatures ------------------------------ */
