@@ -600,18 +600,26 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       // unchanged; a row's partial sum is flushed when the walk moves on to
       // a later row.  (With attraction every candidate within reach pulls,
       // so the rows are walked one candidate at a time instead.)
-      uint64_t h[3];
+      // The three rows are walked together, the k-th candidate of each
+      // loaded at once (three loads in flight; a row already walked
+      // re-reads the particle's own slot, which is never marked).
+      uint64_t h[3] = {0u, 0u, 0u};
+      const uint32_t len0 = re[0] - rs[0], len1 = re[1] - rs[1], len2 = re[2] - rs[2];
+      const uint32_t L = max(len0, max(len1, len2));
+      for (uint32_t k = 0; k < L; ++k) {
+        Cand b[3];
+        const uint32_t j0 = k < len0 ? rs[0] + k : i, j1 = k < len1 ? rs[1] + k : i, j2 = k < len2 ? rs[2] + k : i;
+        candidate_load<kRadius, kOne, false, kO>(c, j0, pos, type, radius, d, b[0]);
+        candidate_load<kRadius, kOne, false, kO>(c, j1, pos, type, radius, d, b[1]);
+        candidate_load<kRadius, kOne, false, kO>(c, j2, pos, type, radius, d, b[2]);
+        const uint32_t jy[3] = {j0, j1, j2};
 #pragma unroll
-      for (int y = 0; y < 3; ++y) {
-        h[y] = 0;
-        for (uint32_t j = rs[y]; j < re[y]; ++j) {
-          Cand b;
-          candidate_load<kRadius, kOne, false, kO>(c, j, pos, type, radius, d, b);
-          const float rx = b.p.x - q.p.x, ry = b.p.y - q.p.y;
+        for (int y = 0; y < 3; ++y) {
+          const float rx = b[y].p.x - q.p.x, ry = b[y].p.y - q.p.y;
           const float d2 = rx * rx + ry * ry;
-          const float R = q.rA + b.r;
+          const float R = q.rA + b[y].r;
           const bool maybe = !(d2 > (R * R) * 1.00000095367431640625f) || R < 1.0842022e-19f;
-          h[y] |= (uint64_t)(maybe && j != i) << (j - rs[y]);
+          h[y] |= (uint64_t)(maybe && jy[y] != i) << k;
         }
       }
       RowSum rsum;
