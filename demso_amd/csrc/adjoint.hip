@@ -13,8 +13,9 @@
 //   k_adj_collide   : reverse of the pair law (:235-307) over the 3x3
 //                     neighbourhood, one lane per particle gathering the
 //                     neighbours' force/moment adjoints (no atomics on state)
-//   k_adj_permute   : adjoint of the sort permutation (scatter; in place at
-//                     the moved slots after an in-place re-sort)
+//   k_adj_permute   : adjoint of the sort permutation (scatter; after an
+//                     in-place re-sort of more than kScatterMax particles, in
+//                     place at the moved slots instead)
 // Switches (contact, approach-only damping, stick/slip, wall hits) are
 // frozen at the forward's decisions.  In exact arithmetic the pair law is
 // F = s n + f_t t, tau = f_t with s = -kn (R - dist) + [vn<0] cn vn,
@@ -37,6 +38,10 @@ namespace dem {
 namespace {
 
 constexpr int kParams = 4;  // kn, kt, cn, mu
+// Up to this many particles the adjoint of an in-place step's permutation is
+// one full scatter (a launch costs more than copying the whole adjoint
+// state); beyond it, only the listed slots are saved and put back.
+constexpr uint32_t kScatterMax = 262144;
 
 __device__ __forceinline__ uint32_t clampi(uint32_t t, uint32_t count) { return t < count ? t : count - 1u; }
 
@@ -158,7 +163,7 @@ __global__ void k_store_slot(const float2* __restrict__ pos, const float2* __res
   somega[i] = omega[i];
   if (radius) sradius[i] = radius[i];
   stype[i] = type[i];
-  if (index) sindex[i] = index[i];
+  sindex[i] = index ? index[i] : i;  // without an index: the identity (an in-place step's, before its listed slots)
   skey[i] = key[i];
   sacc[i] = acc[i];
   salpha[i] = alpha[i];
@@ -667,18 +672,17 @@ hipError_t store_slot_hook(void* user, dem_sim* m, hipStream_t s, int dyn, int s
   auto* a = static_cast<dem_adjoint*>(user);
   const size_t off = a->slot_off;
   const uint32_t n = m->n;
-  if (list) {
-    // an in-place step: its index is the identity except at the listed slots
-    const hipError_t e = launch_shift_index(s, m->shift, a->sindex + off, n);
-    if (e != hipSuccess) return e;
-  }
   a->slot_inplace[off / n] = list ? 1 : 0;
   hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[dyn], m->vel[dyn], m->omega[dyn],
                      m->rad ? m->radius[stat] : nullptr, m->type[stat], list ? nullptr : m->sorted_idx,
                      m->sorted_key, m->acc, m->alpha,
                      a->spos + off, a->svel + off, a->somega + off, a->sradius ? a->sradius + off : nullptr,
                      a->stype + off, a->sindex + off, a->skey + off, a->sacc + off, a->salpha + off, n);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  // an in-place step: its index is the identity (just stored) except at the
+  // listed slots
+  if (e == hipSuccess && list) e = launch_shift_index(s, m->shift, a->sindex + off, n, true);
+  return e;
 }
 
 }  // namespace
@@ -846,7 +850,7 @@ int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_ve
       DEM_ADJ(k_adj_collide, c, S, (const uint32_t*)m->cell_start, (const uint32_t*)m->cell_end,
               (const double2*)a->lF, (const double*)a->lM, a->lxp, a->lvp, a->lwp, a->grad, n);
 #undef DEM_ADJ
-      if (a->slot_inplace[k]) {
+      if (a->slot_inplace[k] && n > kScatterMax) {
         hipLaunchKernelGGL(k_adj_perm_save, grid_of(n), dim3(256), 0, s, a->sindex + off, a->lxp, a->lvp, a->lwp,
                            a->lx, a->lv, a->lw, n);
         hipLaunchKernelGGL(k_adj_perm_put, grid_of(n), dim3(256), 0, s, a->sindex + off, a->lx, a->lv, a->lw,
@@ -855,6 +859,9 @@ int dem_adj_backward(dem_adjoint* a, const float* grad_pos, const float* grad_ve
         std::swap(a->lv, a->lvp);
         std::swap(a->lw, a->lwp);
       } else {
+        // the full scatter (for an in-place step too, whose stored index is
+        // the identity outside its listed slots: one launch instead of two,
+        // which is faster wherever launches cost more than the whole copy)
         hipLaunchKernelGGL(k_adj_permute, grid_of(n), dim3(256), 0, s, a->sindex + off, a->lxp, a->lvp, a->lwp,
                            a->lx, a->lv, a->lw, n);
       }

@@ -153,8 +153,10 @@ hipError_t launch_resort_shift_plan(hipStream_t s, const uint32_t* key, const Sh
 hipError_t launch_resort_shift_apply(hipStream_t s, const ShiftState& st, const ShiftScratch& sh, uint32_t num_cells,
                                      uint32_t n);
 // The sorted index of the last in-place re-sort: identity except at the
-// listed slots (index[new] = old).
-hipError_t launch_shift_index(hipStream_t s, const ShiftScratch& sh, uint32_t* index, uint32_t n);
+// listed slots (index[new] = old).  identity_done: index already holds the
+// identity (only the listed slots are written).
+hipError_t launch_shift_index(hipStream_t s, const ShiftScratch& sh, uint32_t* index, uint32_t n,
+                              bool identity_done = false);
 hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32_t* cell_end,
                           float2* s_pos, float2* s_vel, float* s_theta, float* s_omega,
                           uint32_t* s_id, uint32_t* s_loc, uint32_t* s_type, float* s_radius,

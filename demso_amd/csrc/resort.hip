@@ -804,10 +804,10 @@ hipError_t launch_resort_shift_apply(hipStream_t s, const ShiftState& st, const 
   return hipGetLastError();
 }
 
-hipError_t launch_shift_index(hipStream_t s, const ShiftScratch& sh, uint32_t* index, uint32_t n) {
+hipError_t launch_shift_index(hipStream_t s, const ShiftScratch& sh, uint32_t* index, uint32_t n, bool identity_done) {
   if (n == 0) return hipSuccess;
   if (n > sh.capacity || !sh.count || !sh.dst || !sh.src || !index) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_iota, dim3((n + 255u) / 256u), dim3(256), 0, s, index, n);
+  if (!identity_done) hipLaunchKernelGGL(k_iota, dim3((n + 255u) / 256u), dim3(256), 0, s, index, n);
   hipLaunchKernelGGL(k_shift_index, dim3(list_blocks(n)), dim3(256), 0, s, sh.dst, sh.src, sh.count, sh.capacity,
                      index, n);
   return hipGetLastError();
