@@ -537,7 +537,60 @@ __global__ __launch_bounds__(256) void k_adj_collide(StepConst c, AdjIn S, const
     const uint32_t x1 = clamp ? (uint32_t)gx : ((uint32_t)gx & (G0 - 1u));
     const uint32_t x2 = ((uint32_t)gx + 1u) & (clamp ? 0xFFFFFFFFu : G0 - 1u);
     const bool contiguous = clamp || (x0 + 1u == x1 && x1 + 1u == x2);
-    for (int y = -1; y <= 1; ++y) {
+    // Contiguous rows (the usual case): all three rows' candidate ranges are
+    // loaded at once; rows of at most 64 candidates are walked together
+    // marking the candidates the forward's prefilter lets through (three
+    // loads in flight), then the pair adjoint runs over the marks in row
+    // order.  Candidates out of contact contribute nothing, so the sums are
+    // the same, in the same order, as walking every candidate.
+    bool walked = false;
+    if (contiguous) {
+      uint32_t rs[3], re[3];
+#pragma unroll
+      for (int y = 0; y < 3; ++y) {
+        const int iy = gy + y - 1;
+        const bool ok = !clamp || (iy >= 0 && iy < (int)G1);
+        const uint32_t cy = clamp ? (ok ? (uint32_t)iy : 0u) : ((uint32_t)iy & (G1 - 1u));
+        const uint32_t row = in.cell_base + cy * G0;
+        const uint32_t a0 = row + (vx0 ? x0 : x1), a1 = row + x1, a2 = row + (vx2 ? x2 : x1);
+        const uint32_t s0r = cell_start[a0], s1r = cell_start[a1], s2r = cell_start[a2];
+        const uint32_t e0r = cell_end[a0], e1r = cell_end[a1], e2r = cell_end[a2];
+        const uint32_t s0 = ok && vx0 ? s0r : kEmpty, s1 = ok ? s1r : kEmpty, s2 = ok && vx2 ? s2r : kEmpty;
+        const bool n0 = s0 != kEmpty, n1 = s1 != kEmpty, n2 = s2 != kEmpty;
+        rs[y] = n0 ? s0 : (n1 ? s1 : (n2 ? s2 : 0u));
+        re[y] = (n0 || n1 || n2) ? min(n2 ? e2r : (n1 ? e1r : e0r), n) : 0u;
+        if (re[y] < rs[y]) re[y] = rs[y];
+      }
+      const uint32_t len0 = re[0] - rs[0], len1 = re[1] - rs[1], len2 = re[2] - rs[2];
+      if (len0 <= 64u && len1 <= 64u && len2 <= 64u) {
+        walked = true;
+        uint64_t h[3] = {0u, 0u, 0u};
+        const uint32_t L = max(len0, max(len1, len2));
+        for (uint32_t k = 0; k < L; ++k) {
+          const uint32_t jy[3] = {k < len0 ? rs[0] + k : i, k < len1 ? rs[1] + k : i, k < len2 ? rs[2] + k : i};
+          float2 pb[3];
+          float rb[3];
+#pragma unroll
+          for (int y = 0; y < 3; ++y) {
+            pb[y] = S.pos[jy[y]];
+            rb[y] = kRadius ? S.radius[jy[y]] : c.mat[clampi(S.type[jy[y]], c.num_types)].radius;
+          }
+#pragma unroll
+          for (int y = 0; y < 3; ++y) {
+            const float rx = pb[y].x - q.pA.x, ry = pb[y].y - q.pA.y;
+            const float d2 = rx * rx + ry * ry;
+            const float R = q.rA + rb[y];
+            const bool maybe = !(d2 > (R * R) * 1.00000095367431640625f) || R < 1.0842022e-19f;
+            h[y] |= (uint64_t)(maybe && jy[y] != i) << k;
+          }
+        }
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+          for (uint64_t hy = h[y]; hy; hy &= hy - 1u)
+            adj_candidate<kRadius>(c, S, lF, lM, grad, i, rs[y] + (uint32_t)__builtin_ctzll(hy), q, g);
+      }
+    }
+    for (int y = -1; y <= 1 && !walked; ++y) {
       const int iy = gy + y;
       uint32_t cy;
       if (clamp) {
