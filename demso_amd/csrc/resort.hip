@@ -427,15 +427,31 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
   // [lo, hi) found by two binary searches.
   const uint32_t tile = blockIdx.x - kMoverBlocks;
   const uint32_t tiles = gridDim.x - kMoverBlocks;
+  const uint32_t first = tile * kPlaceTile;
+  if (first >= n) return;  // (the grid has one block per tile below n)
+  const uint32_t last = min(first + kPlaceTile, n) - 1u;
+  // all loads of the thread's items first (every slot is written, and none
+  // of them depends on the movers); the block's particles lie in one tile of
+  // kResortTile: without movers there, its word prefixes are not written
+  // and tile_off gives them
+  const uint32_t rt = first / kResortTile, tile_base = tile_off[rt];
+  const bool tile_movers = tile_off[rt + 1u] != tile_base;
+  uint32_t kk[kPlaceItems], bw[kPlaceItems], wp[kPlaceItems];
+#pragma unroll
+  for (uint32_t it = 0; it < kPlaceItems; ++it) {
+    const uint32_t i = first + it * kPlaceThreads + tid;
+    const bool valid = i <= last;
+    kk[it] = valid ? prev_key[i] : 0u;  // a stayer's key (movers are skipped below)
+    bw[it] = valid ? bits[i >> 5] : 0xFFFFFFFFu;
+    const uint32_t wraw = valid ? word_prefix[i >> 5] : 0u;
+    wp[it] = valid ? (tile_movers ? wraw : tile_base) : 0u;
+  }
   if (clear) {
     // this tile's share of the next step's cellStart reset
     const uint32_t per = (clear_n + tiles - 1u) / tiles;
     const uint32_t c0 = min(tile * per, clear_n), c1 = min(c0 + per, clear_n);
     for (uint32_t k = c0 + tid; k < c1; k += kPlaceThreads) clear[k] = kEmpty;
   }
-  const uint32_t first = tile * kPlaceTile;
-  if (first >= n) return;
-  const uint32_t last = min(first + kPlaceTile, n) - 1u;
   uint32_t lo = 0, hi = m;
   if (m > kWindowMax) {
     if (tid == 0) s_lo = movers_below(sk, sp, 0, m, composite(prev_key[first], first));
@@ -448,20 +464,6 @@ __global__ __launch_bounds__(kPlaceThreads) void k_resort_place(
   if (staged)
     for (uint32_t j = tid; j < hi - lo; j += kPlaceThreads) win[j] = composite(sk[lo + j], sp[lo + j]);
   __syncthreads();
-  // all loads of the thread's items first, then the placements
-  // the block's particles lie in one tile of kResortTile: without movers
-  // there, its word prefixes are not written and tile_off gives them
-  const uint32_t rt = first / kResortTile, tile_base = tile_off[rt];
-  const bool tile_movers = tile_off[rt + 1u] != tile_base;
-  uint32_t kk[kPlaceItems], bw[kPlaceItems], wp[kPlaceItems];
-#pragma unroll
-  for (uint32_t it = 0; it < kPlaceItems; ++it) {
-    const uint32_t i = first + it * kPlaceThreads + tid;
-    const bool valid = i <= last;
-    kk[it] = valid ? prev_key[i] : 0u;  // a stayer's key (movers are skipped below)
-    bw[it] = valid ? bits[i >> 5] : 0xFFFFFFFFu;
-    wp[it] = valid ? (tile_movers ? word_prefix[i >> 5] : tile_base) : 0u;
-  }
 #pragma unroll
   for (uint32_t it = 0; it < kPlaceItems; ++it) {
     const uint32_t i = first + it * kPlaceThreads + tid;
@@ -603,10 +605,20 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
   const uint32_t first = tile * kPlaceTile;
   if (first >= n) return;
   const uint32_t last = min(first + kPlaceTile, n) - 1u;
+  // the block's words that do not depend on the movers go out with the
+  // window's loads: its first and last keys, the movers before its ends,
+  // and (the block's particles lie in one tile of kResortTile; without
+  // movers there, its word prefixes are not written and tile_off gives
+  // them) the tile's offsets
+  const uint32_t key_first = prev_key[first], key_last = prev_key[last];
+  const uint32_t mb_first = movers_before(first, n, m, bits, word_prefix, tile_off);
+  const uint32_t mb_end = movers_before(last + 1u, n, m, bits, word_prefix, tile_off);
+  const uint32_t rt = first / kResortTile, tile_base = tile_off[rt];
+  const bool tile_movers = tile_off[rt + 1u] != tile_base;
   uint32_t lo = 0, hi = m;
   if (m > kWindowMax) {
-    if (tid == 0) s_lo = movers_below(sk, sp, 0, m, composite(prev_key[first], first));
-    if (tid == 1) s_hi = movers_below(sk, sp, 0, m, composite(prev_key[last], last));
+    if (tid == 0) s_lo = movers_below(sk, sp, 0, m, composite(key_first, first));
+    if (tid == 1) s_hi = movers_below(sk, sp, 0, m, composite(key_last, last));
     __syncthreads();
     lo = s_lo;
     hi = s_hi;
@@ -622,7 +634,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
       b = tid == 0 ? lo : hi;
     } else {
       const uint32_t q = tid == 0 ? first : last;
-      const uint64_t x = composite(prev_key[q], q);
+      const uint64_t x = composite(tid == 0 ? key_first : key_last, q);
       uint32_t a = 0, e = m;
       while (a < e) {
         const uint32_t mid = (a + e) >> 1;
@@ -635,15 +647,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_shift_place(
     else s_bl = b;
   }
   __syncthreads();
-  {
-    const uint32_t mb_first = movers_before(first, n, m, bits, word_prefix, tile_off);
-    const uint32_t mb_end = movers_before(last + 1u, n, m, bits, word_prefix, tile_off);
-    if (mb_first == mb_end && s_bf == s_bl && s_bf == mb_first) return;  // uniform over the block
-  }
-  // the block's particles lie in one tile of kResortTile: without movers
-  // there, its word prefixes are not written and tile_off gives them
-  const uint32_t rt = first / kResortTile, tile_base = tile_off[rt];
-  const bool tile_movers = tile_off[rt + 1u] != tile_base;
+  if (mb_first == mb_end && s_bf == s_bl && s_bf == mb_first) return;  // uniform over the block
   uint32_t kk[kPlaceItems], bw[kPlaceItems], wp[kPlaceItems];
 #pragma unroll
   for (uint32_t it = 0; it < kPlaceItems; ++it) {
