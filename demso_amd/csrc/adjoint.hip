@@ -148,11 +148,14 @@ __global__ void k_adj_out(const uint32_t* __restrict__ id, const double2* __rest
 // sorted-state snapshot of one step: pos, vel, omega, radius, type, index
 // One step's slot: the sorted pre-integrate state, the step's sorted keys
 // and permutation, and collide's accelerations.
+// (quiet: the in-place step's runs of 64 slots whose acc and alpha collide
+// left unwritten as +0, or null)
 __global__ void k_store_slot(const float2* __restrict__ pos, const float2* __restrict__ vel,
                              const float* __restrict__ omega, const float* __restrict__ radius,
                              const uint32_t* __restrict__ type, const uint32_t* __restrict__ index,
                              const uint32_t* __restrict__ key, const float2* __restrict__ acc,
-                             const float* __restrict__ alpha, float2* __restrict__ spos, float2* __restrict__ svel,
+                             const float* __restrict__ alpha, const uint32_t* __restrict__ quiet,
+                             float2* __restrict__ spos, float2* __restrict__ svel,
                              float* __restrict__ somega, float* __restrict__ sradius, uint32_t* __restrict__ stype,
                              uint32_t* __restrict__ sindex, uint32_t* __restrict__ skey, float2* __restrict__ sacc,
                              float* __restrict__ salpha, uint32_t n) {
@@ -165,8 +168,9 @@ __global__ void k_store_slot(const float2* __restrict__ pos, const float2* __res
   stype[i] = type[i];
   sindex[i] = index ? index[i] : i;  // without an index: the identity (an in-place step's, before its listed slots)
   skey[i] = key[i];
-  sacc[i] = acc[i];
-  salpha[i] = alpha[i];
+  const bool still = quiet && quiet[i / 64u] != 0u;
+  sacc[i] = still ? make_float2(0.0f, 0.0f) : acc[i];
+  salpha[i] = still ? 0.0f : alpha[i];
 }
 
 // cellStart/cellEnd of sorted keys (cellStart pre-filled with EMPTY)
@@ -728,7 +732,7 @@ hipError_t store_slot_hook(void* user, dem_sim* m, hipStream_t s, int dyn, int s
   a->slot_inplace[off / n] = list ? 1 : 0;
   hipLaunchKernelGGL(k_store_slot, grid_of(n), dim3(256), 0, s, m->pos[dyn], m->vel[dyn], m->omega[dyn],
                      m->rad ? m->radius[stat] : nullptr, m->type[stat], list ? nullptr : m->sorted_idx,
-                     m->sorted_key, m->acc, m->alpha,
+                     m->sorted_key, m->acc, m->alpha, m->acc_lazy ? m->quiet : nullptr,
                      a->spos + off, a->svel + off, a->somega + off, a->sradius ? a->sradius + off : nullptr,
                      a->stype + off, a->sindex + off, a->skey + off, a->sacc + off, a->salpha + off, n);
   hipError_t e = hipGetLastError();

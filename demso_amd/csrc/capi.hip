@@ -653,9 +653,13 @@ hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev, const Ste
     co.react_m = m->react_m;
   }
   const float* radius = m->rad ? m->radius[b] : nullptr;
+  // collide leaves acc and alpha of its quiet runs unwritten; integrate
+  // reads the runs' flags
   e = dem::launch_collide(c, s, ctx->bs_collide, m->pos[b], m->vel[b], m->acc, m->omega[b], m->alpha, m->type[b],
-                          radius, m->cs[b], m->cell_end, m->pressure, n, collide_mode(ctx), control ? &co : nullptr);
+                          radius, m->cs[b], m->cell_end, m->pressure, n, collide_mode(ctx), control ? &co : nullptr,
+                          nullptr, m->quiet);
   if (e != hipSuccess) return e;
+  m->acc_lazy = true;
   if (control) {
     e = dem::launch_control_finish(s, co, m->react_bx, m->react_by, m->react_bm, n, ctx->d_reaction, ctx->d_control,
                                    true);
@@ -674,7 +678,7 @@ hipError_t sim_step_inplace(dem_sim* m, hipStream_t s, hipEvent_t* ev, const Ste
   nk.tile_count = m->resort.tile_count;
   e = dem::launch_integrate_to(c, s, ctx->bs_integrate, m->pos[b], m->vel[b], m->theta[b], m->omega[b], m->pos[b],
                                m->vel[b], m->theta[b], m->omega[b], m->acc, m->alpha, m->type[b], radius,
-                               m->pressure, n, &nk, nullptr, ctx->per_integrate);
+                               m->pressure, n, &nk, nullptr, ctx->per_integrate, m->quiet);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[6], s);
   m->sorted_key = m->skey[b];
@@ -798,10 +802,30 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
   m->cur = t;
   m->cell_start = m->cs[t];
   m->perm_list = false;
+  m->acc_lazy = false;  // these layouts write every acc and alpha
   return hipSuccess;
 }
 
 bool sim_inplace(const dem_sim* m, const StepHook*) { return !m->ctx->copy && !m->ctx->fuse; }
+
+namespace {
+// acc = alpha = +0 in the runs of 64 slots that collide marked quiet.
+__global__ void k_quiet_zero(const uint32_t* __restrict__ quiet, float2* __restrict__ acc, float* __restrict__ alpha,
+                             uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || quiet[i / 64u] == 0u) return;
+  acc[i] = make_float2(0.0f, 0.0f);
+  alpha[i] = 0.0f;
+}
+}  // namespace
+
+hipError_t sim_acc_complete(dem_sim* m, hipStream_t s) {
+  if (!m->acc_lazy || m->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_quiet_zero, dim3((m->n + 255u) / 256u), dim3(256), 0, s, m->quiet, m->acc, m->alpha, m->n);
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) m->acc_lazy = false;
+  return e;
+}
 
 // In place, the re-sort stays cheaper than the radix sort while it lists at
 // most n / kListedFrac particles.
@@ -916,6 +940,7 @@ int dem_sim_create(dem_context* ctx, uint32_t n, uint32_t num_cells, int with_ra
   if (!r) r = sim_alloc(m, &m->omega[kScratch], n);
   if (!r) r = sim_alloc(m, &m->acc, n);
   if (!r) r = sim_alloc(m, &m->alpha, n);
+  if (!r) r = sim_alloc(m, &m->quiet, (n + 63u) / 64u);
   if (!r) r = sim_alloc(m, &m->pressure, n);
   if (!r) r = sim_alloc(m, &m->react_x, n);
   if (!r) r = sim_alloc(m, &m->react_y, n);
@@ -1022,6 +1047,7 @@ int dem_sim_upload(dem_sim* m, const float* pos, const float* vel, const float* 
   DEM_HIP(ctx, hipStreamSynchronize(s), "upload sync");
   m->resort_ok = false;  // an uploaded state is in no known sorted order
   m->perm_list = false;
+  m->acc_lazy = false;
   m->h_count[0] = m->h_count[1] = 0;
   m->radix_run = m->radix_wait = 0;
   return DEM_OK;
@@ -1036,6 +1062,7 @@ int dem_sim_download(dem_sim* m, float* pos, float* vel, float* theta, float* om
   const int c = m->cur;
   const size_t n = m->n;
   hipStream_t s = ctx->stream;
+  if (acc || alpha) DEM_HIP(ctx, sim_acc_complete(m, s), "accelerations");
   if (index && m->perm_list) {
     // the last step re-sorted in place: its index is the identity except
     // at the slots it listed
@@ -1087,6 +1114,7 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
     DEM_HIP(ctx, hipGraphLaunch(resort ? m->exec_inc[m->cur] : m->exec[m->cur], ctx->stream), "hipGraphLaunch");
     m->resort_ok = true;
     m->perm_list = resort && sim_inplace(m);
+    m->acc_lazy = m->perm_list;
   }
   // Each replay runs two steps, which leaves the state in the same buffer;
   // the sorted keys are in skey[cur] and the sorted index in sidx.
@@ -1273,6 +1301,8 @@ int dem_gather_rows(dem_context* ctx, const uint32_t* follow_ids, uint32_t count
 
 int dem_sim_gather_rows(dem_sim* m, const uint32_t* follow_ids, uint32_t count, uint32_t* rows) {
   if (!m) return DEM_ERR_INVALID;
+  if (int r = ensure_device(m->ctx)) return r;
+  DEM_HIP(m->ctx, sim_acc_complete(m, m->ctx->stream), "accelerations");
   const int c = m->cur;
   return dem_gather_rows(m->ctx, follow_ids, count, m->loc[c], (const float*)m->pos[c],
                          (const float*)m->vel[c], (const float*)m->acc, m->theta[c], m->omega[c], m->alpha,
@@ -1547,6 +1577,7 @@ int dem_sim_autotune(dem_sim* m, uint32_t* block) {
   if (ttheta) (void)hipFree(ttheta);
   if (tomega) (void)hipFree(tomega);
   m->resort_ok = false;  // the cell arrays and keys now hold the tuning step's data
+  m->acc_lazy = false;   // and acc, alpha the tuning collide's (every slot written)
   if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
   // collide + integrate as one kernel or two, in whichever candidate mode is
   // fastest here (the split layout's reorder moves fewer bytes than the

@@ -96,6 +96,13 @@ struct dem_sim {
   // except at the listed slots, and sidx is rebuilt from the list on demand.
   dem::ShiftScratch shift;
   bool perm_list = false;
+  // In-place steps: collide marks the runs of 64 slots where no force acted
+  // (quiet, ceil(n / 64) words) and leaves their acc and alpha unwritten
+  // (+0 by definition, which integrate uses); acc_lazy says the arrays still
+  // need those zeros before anything outside the step reads them
+  // (sim_acc_complete).
+  uint32_t* quiet = nullptr;
+  bool acc_lazy = false;
   // host-mapped: [0] the mover count of the last re-sort, [1] the particles
   // the last in-place re-sort listed (slot or key changed)
   uint32_t* h_count = nullptr;
@@ -143,6 +150,9 @@ hipError_t sim_step_on(dem_sim* m, hipStream_t s, hipEvent_t* ev, bool resort, c
 // Sort method for the next step of m (auto mode: re-sort while the last
 // observed mover count is small).
 bool sim_use_resort(dem_sim* m);
+// Writes the +0 accelerations of the quiet runs of the last in-place step
+// into acc and alpha (nothing if they are complete).
+hipError_t sim_acc_complete(dem_sim* m, hipStream_t s);
 // Whether a re-sorted step of m runs in place (see sim_step_on).
 bool sim_inplace(const dem_sim* m, const StepHook* hook = nullptr);
 

@@ -167,12 +167,17 @@ hipError_t launch_reorder(hipStream_t s, int block, uint32_t* cell_start, uint32
 // launch_collide's mode: the candidate mode (0 one at a time, 1 one ahead, 2
 // the first of every row at once, 3 contact lists), plus kWideMode to force 64-bit addressing
 // (the plain mode), which states over 2^29 particles or 2^30 cells get anyway.
+// quiet (ceil(n / 64) words, or null): collide sets quiet[g] = 1 for each run
+// g of 64 slots where no force acted, and then stores neither acc nor alpha
+// there (both are +0); launch_integrate_to given the same array reads +0
+// for those runs instead of loading them.
 constexpr int kWideMode = 4;
 hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const float2* pos,
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
                           const uint32_t* cell_end, float* pressure, uint32_t n, int mode = 0,
-                          const struct ControlOut* ctl = nullptr, const uint32_t* gat = nullptr);
+                          const struct ControlOut* ctl = nullptr, const uint32_t* gat = nullptr,
+                          uint32_t* quiet = nullptr);
 // The next step's hash, fused into integrate: bit i of `bits` set when the
 // cell key of the new position differs from the key the particle is sorted
 // by (the hash of its position before integrate), and then key[i] = the new
@@ -193,7 +198,7 @@ hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, con
                                float2* vel_out, float* theta_out, float* omega_out, const float2* acc,
                                const float* alpha, const uint32_t* type, const float* radius, float* pressure,
                                uint32_t n, const NextKeys* next = nullptr, const uint32_t* gat = nullptr,
-                               int per = 1);
+                               int per = 1, const uint32_t* quiet = nullptr);
 // collide + integrate + the next step's hash in one kernel: forces from the
 // sorted state *_in, acc/alpha/pressure as collide writes them, the
 // integrated state to *_out (a different buffer).

@@ -722,7 +722,8 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       float2* __restrict__ acc, const float* __restrict__ omega, float* __restrict__ alpha, \
       const uint32_t* __restrict__ type, const float* __restrict__ radius,                  \
       const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,       \
-      float* __restrict__ pressure, ControlOut ctl, const uint32_t* __restrict__ gat, uint32_t n
+      float* __restrict__ pressure, ControlOut ctl, const uint32_t* __restrict__ gat,                \
+      uint32_t* __restrict__ quiet, uint32_t n
 
 template <bool kRadius, int kMat, int kMode, bool kControl, bool kO>
 __device__ __forceinline__ void collide_kernel_body(DEM_COLLIDE_PARAMS) {
@@ -748,12 +749,19 @@ __device__ __forceinline__ void collide_kernel_body(DEM_COLLIDE_PARAMS) {
   // A wave whose discs all feel no force stores zeros without the three
   // divisions: the sums start at +0 and are never -0, so fx, fy and moment
   // are +0 exactly when nothing acted, and +0 over a positive finite mass or
-  // inertia is +0.
-  const bool quiet = fx == 0.0f && fy == 0.0f && moment == 0.0f && own.mass > 0.0f && own.inertia > 0.0f &&
+  // inertia is +0.  With a quiet array (the owned simulation's in-place
+  // step, whose integrate reads it) the wave records that instead of
+  // storing the zeros: quiet[i / 64] = 1 when acc and alpha of the wave's 64
+  // slots are +0 and were not written, else 0.
+  const bool still = fx == 0.0f && fy == 0.0f && moment == 0.0f && own.mass > 0.0f && own.inertia > 0.0f &&
                      own.mass <= 3.402823466e38f && own.inertia <= 3.402823466e38f;
-  if (__ballot(!quiet) == 0) {
-    acc[i] = make_float2(0.0f, 0.0f);
-    alpha[i] = 0.0f;
+  const bool wave_still = __ballot(!still) == 0;
+  if (quiet && __lane_id() == 0) quiet[i / (uint32_t)kWave] = wave_still ? 1u : 0u;
+  if (wave_still) {
+    if (!quiet) {
+      acc[i] = make_float2(0.0f, 0.0f);
+      alpha[i] = 0.0f;
+    }
   } else {
     acc[i] = make_float2(fx / own.mass, fy / own.mass);
     alpha[i] = moment / own.inertia;
@@ -764,7 +772,7 @@ __device__ __forceinline__ void collide_kernel_body(DEM_COLLIDE_PARAMS) {
 template <bool kRadius, int kMat, int kMode, bool kControl, bool kO>
 __global__ __launch_bounds__(256) void k_collide(DEM_COLLIDE_PARAMS) {
   collide_kernel_body<kRadius, kMat, kMode, kControl, kO>(c, pos, vel, acc, omega, alpha, type, radius, cell_start,
-                                                          cell_end, pressure, ctl, gat, n);
+                                                          cell_end, pressure, ctl, gat, quiet, n);
 }
 
 // The one-material batched mode needed 65 VGPRs with 64-bit addresses; held to 64 (8
@@ -773,7 +781,7 @@ __global__ __launch_bounds__(256) void k_collide(DEM_COLLIDE_PARAMS) {
 template <bool kRadius>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_collide_batch(DEM_COLLIDE_PARAMS) {
   collide_kernel_body<kRadius, kOneMat, kBatch, false, true>(c, pos, vel, acc, omega, alpha, type, radius,
-                                                             cell_start, cell_end, pressure, ctl, gat, n);
+                                                             cell_start, cell_end, pressure, ctl, gat, quiet, n);
 }
 #undef DEM_COLLIDE_PARAMS
 
@@ -912,7 +920,8 @@ __global__ __launch_bounds__(256) void k_integrate(
     StepConst c, const float2* pos_in, const float2* vel_in, const float* theta_in, const float* omega_in,
     float2* pos_out, float2* vel_out, float* theta_out, float* omega_out, const float2* __restrict__ acc,
     const float* __restrict__ alpha, const uint32_t* __restrict__ type, const float* __restrict__ radius,
-    float* __restrict__ pressure, NextKeys nk, const uint32_t* __restrict__ gat, uint32_t n) {
+    float* __restrict__ pressure, NextKeys nk, const uint32_t* __restrict__ gat,
+    const uint32_t* __restrict__ quiet, uint32_t n) {
   constexpr bool kMulti = kMat == kInstances;
   // kPer particles a lane, as k_hash.  The inputs of all kPer particles are
   // loaded before any output is stored (the in and out arrays may be the
@@ -933,8 +942,11 @@ __global__ __launch_bounds__(256) void k_integrate(
     v[k] = vel_in[g];
     w[k] = omega_in[g];
     th[k] = theta_in[g];
-    a[k] = acc[i];
-    al[k] = alpha[i];
+    // a run of 64 slots that collide marked quiet has acc = alpha = +0 (not
+    // stored); the wave's 64 slots are one such run (one scalar load)
+    const bool still = quiet && quiet[__builtin_amdgcn_readfirstlane(i) / (uint32_t)kWave] != 0u;
+    a[k] = still ? make_float2(0.0f, 0.0f) : acc[i];
+    al[k] = still ? 0.0f : alpha[i];
     rad[k] = kRadius ? radius[i] : 0.0f;
   }
 #pragma unroll
@@ -1105,8 +1117,9 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
                           const float2* vel, float2* acc, const float* omega, float* alpha,
                           const uint32_t* type, const float* radius, const uint32_t* cell_start,
                           const uint32_t* cell_end, float* pressure, uint32_t n, int mode,
-                          const ControlOut* ctl, const uint32_t* gat) {
+                          const ControlOut* ctl, const uint32_t* gat, uint32_t* quiet) {
   if (n == 0) return hipSuccess;
+  if (quiet && block % kWave) return hipErrorInvalidValue;  // the flags cover aligned 64-particle runs
   const ControlOut co = ctl ? *ctl : ControlOut{};
   // 32-bit byte offsets reach every element while n * 8 B and cells * 4 B
   // stay within 4 GiB
@@ -1116,10 +1129,11 @@ hipError_t launch_collide(const StepConst& c, hipStream_t s, int block, const fl
   do {                                                                                                      \
     if (o32)                                                                                                \
       hipLaunchKernelGGL((collide_kernel<R, M, P, X, true>()), grid_for(n, block), dim3(block), 0, s, c, pos, \
-                         vel, acc, omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, n);   \
+                         vel, acc, omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, quiet, n); \
     else                                                                                                    \
       hipLaunchKernelGGL((collide_kernel<R, M, P, X, false>()), grid_for(n, block), dim3(block), 0, s, c,     \
-                         pos, vel, acc, omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, n); \
+                         pos, vel, acc, omega, alpha, type, radius, cell_start, cell_end, pressure, co, gat, quiet, \
+                         n);                                                                                \
   } while (0)
   DEM_DISPATCH(DEM_COLLIDE);
 #undef DEM_COLLIDE
@@ -1225,7 +1239,8 @@ hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, con
                                const float2* vel_in, const float* theta_in, const float* omega_in, float2* pos_out,
                                float2* vel_out, float* theta_out, float* omega_out, const float2* acc,
                                const float* alpha, const uint32_t* type, const float* radius, float* pressure,
-                               uint32_t n, const NextKeys* next, const uint32_t* gat, int per) {
+                               uint32_t n, const NextKeys* next, const uint32_t* gat, int per,
+                               const uint32_t* quiet) {
   if (n == 0) return hipSuccess;
   if (block % kWave) return hipErrorInvalidValue;  // waves must cover aligned 64-particle runs
   per = per_of(per);
@@ -1234,7 +1249,7 @@ hipError_t launch_integrate_to(const StepConst& c, hipStream_t s, int block, con
 #define DEM_INTEGRATE(R, M, X, P)                                                                         \
   hipLaunchKernelGGL((k_integrate<R, M, X, P>), grid, dim3(block), 0, s, c, pos_in, vel_in, theta_in,    \
                      omega_in, pos_out, vel_out, theta_out, omega_out, acc, alpha, type, radius, pressure, nk, \
-                     gat, n)
+                     gat, quiet, n)
 #define DEM_INTEGRATE_X(R, M, P)            \
   do {                                      \
     if (next) DEM_INTEGRATE(R, M, true, P); \
