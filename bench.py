@@ -49,6 +49,9 @@ METRIC = "particle-steps/s @4M discs; batched-ensemble and fwd+adjoint vs plain 
 #             type 4; W pos 8, vel 8, theta 4, omega 4 (the cell of the
 #             position read is recomputed, not loaded; the next key is
 #             written only for the few particles that change cell)
+#   (the reference algorithm's figures; in runs of 64 slots where no force
+#   acted, collide writes no acc/alpha and integrate reads none, so the
+#   bytes moved can be below these)
 # Copy (DEM_STEP_COPY): re-sort R previous key 4, W sorted key 4, index 4 and
 #   the next cellStart 4 per cell; reorder R key 4, index 4, pos 8, id 4,
 #   type 4, radius 4, W pos 8, id 4, loc 4, type 4, radius 4; collide and
