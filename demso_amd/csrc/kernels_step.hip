@@ -603,25 +603,37 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       // The three rows are walked together, the k-th candidate of each
       // loaded at once (three loads in flight; a row already walked
       // re-reads the particle's own slot, which is never marked).
-      uint64_t h[3] = {0u, 0u, 0u};
+      // (the marks are kept as 32-bit halves: bit k of a row's mask is the
+      // same bit for every lane, so setting it is one select and one or)
+      uint32_t hlo[3] = {0u, 0u, 0u}, hhi[3] = {0u, 0u, 0u};
       const uint32_t len0 = re[0] - rs[0], len1 = re[1] - rs[1], len2 = re[2] - rs[2];
       const uint32_t L = max(len0, max(len1, len2));
-      for (uint32_t k = 0; k < L; ++k) {
-        Cand b[3];
-        const uint32_t j0 = k < len0 ? rs[0] + k : i, j1 = k < len1 ? rs[1] + k : i, j2 = k < len2 ? rs[2] + k : i;
-        candidate_load<kRadius, kOne, false, kO>(c, j0, pos, type, radius, d, b[0]);
-        candidate_load<kRadius, kOne, false, kO>(c, j1, pos, type, radius, d, b[1]);
-        candidate_load<kRadius, kOne, false, kO>(c, j2, pos, type, radius, d, b[2]);
-        const uint32_t jy[3] = {j0, j1, j2};
+      // positions k of the rows' k-th candidates, in [k0, k1), marking into
+      // word w (bit k - 32 w)
+      auto mark = [&](uint32_t k0, uint32_t k1, uint32_t (&w)[3], uint32_t wbase) {
+        for (uint32_t k = k0; k < k1; ++k) {
+          Cand b[3];
+          const uint32_t j0 = k < len0 ? rs[0] + k : i, j1 = k < len1 ? rs[1] + k : i, j2 = k < len2 ? rs[2] + k : i;
+          candidate_load<kRadius, kOne, false, kO>(c, j0, pos, type, radius, d, b[0]);
+          candidate_load<kRadius, kOne, false, kO>(c, j1, pos, type, radius, d, b[1]);
+          candidate_load<kRadius, kOne, false, kO>(c, j2, pos, type, radius, d, b[2]);
+          const uint32_t jy[3] = {j0, j1, j2};
+          const uint32_t bit = 1u << (k - wbase);
 #pragma unroll
-        for (int y = 0; y < 3; ++y) {
-          const float rx = b[y].p.x - q.p.x, ry = b[y].p.y - q.p.y;
-          const float d2 = rx * rx + ry * ry;
-          const float R = q.rA + b[y].r;
-          const bool maybe = !(d2 > (R * R) * 1.00000095367431640625f) || R < 1.0842022e-19f;
-          h[y] |= (uint64_t)(maybe && jy[y] != i) << k;
+          for (int y = 0; y < 3; ++y) {
+            const float rx = b[y].p.x - q.p.x, ry = b[y].p.y - q.p.y;
+            const float d2 = rx * rx + ry * ry;
+            const float R = q.rA + b[y].r;
+            const bool maybe = !(d2 > (R * R) * 1.00000095367431640625f) || R < 1.0842022e-19f;
+            w[y] |= maybe && jy[y] != i ? bit : 0u;
+          }
         }
-      }
+      };
+      mark(0u, min(L, 32u), hlo, 0u);
+      if (L > 32u) mark(32u, L, hhi, 32u);
+      uint64_t h[3];
+#pragma unroll
+      for (int y = 0; y < 3; ++y) h[y] = ((uint64_t)hhi[y] << 32) | hlo[y];
       RowSum rsum;
       rsum.b0 = b0[0];
       rsum.b1 = b1[0];
