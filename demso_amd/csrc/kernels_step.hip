@@ -517,14 +517,19 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   body<kRadius>(q.mA, radius, i, q.rA, own.mass, own.inertia);
   own.r = q.rA;
   q.p = pos[i];
-  {
+  // The particle's own velocity and spin feed only the contact law (and the
+  // fused kernel's integrate): the contact-list walk loads them once it has
+  // marked a candidate (or at the end, where a caller uses them), the other
+  // modes up front.
+  bool have_dyn = false;
+  auto load_own_dyn = [&]() {
     const uint32_t iv = d.at<kO>(i);
     q.v = d.vel[iv];
     q.w = d.omega[iv];
-  }
+    have_dyn = true;
+  };
+  if (kMode != kList || kControl) load_own_dyn();
   own.p = q.p;
-  own.v = q.v;
-  own.w = q.w;
   q.den = pressure_den(q.rA);
   q.rden = 0.0;  // the contact-list walk sets it before its first contact
   q.dt = c.dt;
@@ -638,8 +643,12 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       rsum.b0 = b0[0];
       rsum.b1 = b1[0];
       int row = 0;
-      // the pressure terms' reciprocal, only where a candidate may touch
-      if (h[0] | h[1] | h[2]) q.rden = 1.0 / q.den;
+      // the particle's velocity, spin and the pressure terms' reciprocal,
+      // only where a candidate may touch
+      if (h[0] | h[1] | h[2]) {
+        if (!have_dyn) load_own_dyn();
+        q.rden = 1.0 / q.den;
+      }
       while (h[0] | h[1] | h[2]) {
         const bool in0 = h[0] != 0, in1 = !in0 && h[1] != 0;
         const int y = in0 ? 0 : (in1 ? 1 : 2);
@@ -662,6 +671,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
       fx += rsum.cfx;
       fy += rsum.cfy;
     } else {
+    if (!have_dyn) load_own_dyn();
 #pragma unroll
     for (int y = 0; y < 3; ++y) {
       RowSum rsum;
@@ -698,6 +708,7 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
   } else {
     // wrap mode at the grid's x edge: the three cells one by one, in the
     // reference order
+    if (!have_dyn) load_own_dyn();
     for (int y = -1; y <= 1; ++y) {
       const uint32_t cy = ((uint32_t)gy + (uint32_t)y) & ((uint32_t)G1 - 1u);
       const uint32_t row = in.cell_base + cy * (uint32_t)G0;
@@ -723,6 +734,9 @@ __device__ __forceinline__ void collide_particle(const StepConst& c, uint32_t i,
     ctl.react_y[i] = ry;
     ctl.react_m[i] = rm;
   }
+  if (!have_dyn) load_own_dyn();  // (dead, and dropped, where the caller does not use them)
+  own.v = q.v;
+  own.w = q.w;
   fx_out = fx;
   fy_out = fy;
   moment_out = moment;
