@@ -949,6 +949,7 @@ __global__ __launch_bounds__(256) void k_integrate(
     float* __restrict__ pressure, NextKeys nk, const uint32_t* __restrict__ gat,
     const uint32_t* __restrict__ quiet, uint32_t n) {
   constexpr bool kMulti = kMat == kInstances;
+  const bool in_place = !gat && omega_out == omega_in && theta_out == theta_in;
   // kPer particles a lane, as k_hash.  The inputs of all kPer particles are
   // loaded before any output is stored (the in and out arrays may be the
   // same, so the compiler could not move a later particle's loads above an
@@ -989,8 +990,10 @@ __global__ __launch_bounds__(256) void k_integrate(
         pressure[i] = pres;
       pos_out[i] = p;
       vel_out[i] = vv;
-      omega_out[i] = ww;
-      theta_out[i] = tt;
+      // in place, a spin or angle whose bits did not change is not stored
+      // again (a disc at rest: no spin, no moment)
+      if (!in_place || __float_as_uint(ww) != __float_as_uint(w[k])) omega_out[i] = ww;
+      if (!in_place || __float_as_uint(tt) != __float_as_uint(th[k])) theta_out[i] = tt;
       if (kNext) next_key(c, nk, p, p_old[k], in, i, n);
     };
     if constexpr (kMulti) {
