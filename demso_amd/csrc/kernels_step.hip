@@ -779,8 +779,18 @@ __device__ __forceinline__ void collide_kernel_body(DEM_COLLIDE_PARAMS) {
   // step, whose integrate reads it) the wave records that instead of
   // storing the zeros: quiet[i / 64] = 1 when acc and alpha of the wave's 64
   // slots are +0 and were not written, else 0.
-  const bool still = fx == 0.0f && fy == 0.0f && moment == 0.0f && own.mass > 0.0f && own.inertia > 0.0f &&
-                     own.mass <= 3.402823466e38f && own.inertia <= 3.402823466e38f;
+  // (with a radius array the inertia is t / 5, t = ((2 mass) r) r, and t / 5
+  // rounds to a positive finite value exactly when 3 * 2^-149 <= t <= FLT_MAX:
+  // the check needs no division, which only a wave with a force then does)
+  bool inertia_ok;
+  if constexpr (kRadius) {
+    const float t = ((2.0f * own.mass) * own.r) * own.r;
+    inertia_ok = t >= __uint_as_float(3u) && t <= 3.402823466e38f;
+  } else {
+    inertia_ok = own.inertia > 0.0f && own.inertia <= 3.402823466e38f;
+  }
+  const bool still = fx == 0.0f && fy == 0.0f && moment == 0.0f && own.mass > 0.0f &&
+                     own.mass <= 3.402823466e38f && inertia_ok;
   const bool wave_still = __ballot(!still) == 0;
   if (quiet && __lane_id() == 0) quiet[i / (uint32_t)kWave] = wave_still ? 1u : 0u;
   if (wave_still) {
