@@ -65,6 +65,24 @@ __device__ __forceinline__ float add_pressure(float p, float len, double den) {
   return (float)((double)p + (double)len / den);
 }
 
+// RN(sqrt(x)) < m, the Coulomb test of the shear law (length(Ftrial) < Ftmax,
+// src/particles_kernel.cuh:295), for x a sum of two
+// float squares (+0 or more, or NaN).  RN is monotone and the root of a
+// float is never a rounding midpoint, so the test is x < (m-)^2, m- the
+// midpoint below m, which lies in [m^2 (1 - 2^-23), m^2).  For m in
+// [2^-63, 2^63] t = RN(m m) is a normal float within 2^-24 of m^2, so
+// x < RN(t (1 - 2^-20)) proves the test true and x > RN(t (1 + 2^-20))
+// proves it false; only between the two (or for other m, or NaN) is the
+// correctly rounded root taken.
+__device__ __forceinline__ bool sqrt_below(float x, float m) {
+  if (m >= 0x1p-63f && m <= 0x1p63f) {
+    const float t = m * m;
+    if (x < t * (1.0f - 0x1p-20f)) return true;
+    if (x > t * (1.0f + 0x1p-20f)) return false;
+  }
+  return sqrtf(x) < m;
+}
+
 __device__ __forceinline__ double pressure_den(float r) {
   return 4 * M_PI * (double)r * (double)r;
 }
@@ -217,33 +235,44 @@ struct PairCtx {
   uint32_t tA;
 };
 
-// The pressure term with the particle's rounded reciprocal y = RN(1/den):
-// q = RN(a y), then one correction q' = RN(q + r y) with the exact remainder
-// r = a - q den (Markstein's theorem: q' is the correctly rounded a/den
-// when y is the correctly rounded reciprocal and q is within an ulp of the
-// quotient; a is a float length and den = 4 pi r^2 of a float r, so every
-// step stays clear of overflow and underflow in double).  A non-finite
-// length divides as the reference does.  Three FP64 operations per contact
-// instead of a division sequence.
-__device__ __forceinline__ float add_pressure_recip(float p, float len, double den, double rden) {
+// The pressure term with the particle's rounded reciprocal y = RN(1/den) and
+// two fused corrections: q1 = RN(a y) (off by more than an ulp for about 1%
+// of inputs), q2 = RN(q1 + RN(a - q1 den) y) (within a tiny fraction of an
+// ulp of a/den, so faithful), q3 = RN(q2 + (a - q2 den) y), where the
+// remainder of the faithful q2 is exact.  q3 is the correctly rounded a/den
+// (Markstein's theorem for y = RN(1/den) and a faithful quotient: the
+// remainder and den * ulp/2 lie on the grid ulp(den) ulp/2 and differ from
+// ulp/(2y) by less than one step of it, so q2 + r y and a/den lie on the same
+// side of every rounding midpoint, and a/den is never one).  a is a float
+// length and den = 4 pi r^2 of a float r, so no step overflows or underflows
+// in double; a non-finite length divides as the reference does.  Five FP64
+// operations per contact instead of a division sequence.
+__device__ __forceinline__ double pressure_quotient(float len, double den, double rden) {
   const double a = (double)len;
   double qd = a * rden;
   const double rem = fma(-qd, den, a);
-  if (__builtin_expect(__builtin_isfinite(rem), 1))
+  if (__builtin_expect(__builtin_isfinite(rem), 1)) {
     qd = fma(rem, rden, qd);
-  else
+    qd = fma(fma(-qd, den, a), rden, qd);
+  } else {
     qd = a / den;
-  return (float)((double)p + qd);
+  }
+  return qd;
+}
+
+__device__ __forceinline__ float add_pressure_recip(float p, float len, double den, double rden) {
+  return (float)((double)p + pressure_quotient(len, den, rden));
 }
 
 // collideSpheres' contact law (src/particles_kernel.cuh:255-303) for a
 // partner B of material tB at offset (rx, ry) and distance dist < R: the
-// force on A as collideSpheres returns it (ffx, ffy), its pressure term, and
-// the friction moment added to `moment`.
-template <bool kRcp = false>
-__device__ __forceinline__ void contact_force(const StepConst& c, const PairCtx& q, float rx, float ry, float dist,
-                                              float R, float rB, uint32_t tB, float2 vB, float wB, float& ffx,
-                                              float& ffy, float& moment, float& pres) {
+// force on A as collideSpheres returns it (ffx, ffy), the length of its
+// normal part (the pressure term's numerator) as the return value, and the
+// friction moment term mterm (+0 without friction; the moment is a sum that
+// starts at +0 and is never -0, so adding +0 leaves it unchanged).
+__device__ __forceinline__ float contact_terms(const StepConst& c, const PairCtx& q, float rx, float ry, float dist,
+                                               float R, float rB, uint32_t tB, float2 vB, float wB, float& ffx,
+                                               float& ffy, float& mterm) {
   const float4 pr = c.pair[q.tA * c.num_types + tB];
   const float kn = pr.x, kt = pr.y, cn = pr.z, mu = pr.w;
   const float nx = rx / dist, ny = ry / dist;
@@ -260,21 +289,34 @@ __device__ __forceinline__ void contact_force(const StepConst& c, const PairCtx&
     ffy += s2 * ny;
   }
   const float flen = sqrtf(ffx * ffx + ffy * ffy);
-  pres = kRcp ? add_pressure_recip(pres, flen, q.den, q.rden) : add_pressure(pres, flen, q.den);
+  mterm = 0.0f;
   if (mu != 0.0f) {
     const float sr = q.rA * q.w + rB * wB;
     const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
     float ftx = (kt * cvx) * q.dt, fty = (kt * cvy) * q.dt;
     const float ftmax = mu * flen;
-    if (!(sqrtf(ftx * ftx + fty * fty) < ftmax)) {
+    if (!sqrt_below(ftx * ftx + fty * fty, ftmax)) {
       const float cl = sqrtf(cvx * cvx + cvy * cvy);
       ftx = (ftmax * cvx) / cl;
       fty = (ftmax * cvy) / cl;
     }
     ffx += ftx;
     ffy += fty;
-    moment += q.rA * (ftx * tx + fty * ty);
+    mterm = q.rA * (ftx * tx + fty * ty);
   }
+  return flen;
+}
+
+// The contact law with its pressure term added to pres and its moment term
+// to moment.
+template <bool kRcp = false>
+__device__ __forceinline__ void contact_force(const StepConst& c, const PairCtx& q, float rx, float ry, float dist,
+                                              float R, float rB, uint32_t tB, float2 vB, float wB, float& ffx,
+                                              float& ffy, float& moment, float& pres) {
+  float mterm;
+  const float flen = contact_terms(c, q, rx, ry, dist, R, rB, tB, vB, wB, ffx, ffy, mterm);
+  pres = kRcp ? add_pressure_recip(pres, flen, q.den, q.rden) : add_pressure(pres, flen, q.den);
+  moment += mterm;
 }
 
 // attractSpheres (src/particles_kernel.cuh:310-342) for a partner with
@@ -902,7 +944,7 @@ __device__ __forceinline__ bool integrate_particle(const StepConst& c, const Mat
         const float cvx = vtx - sr * tx, cvy = vty - sr * ty;
         float ftx = (kt * cvx) * dt, fty = (kt * cvy) * dt;
         const float ftmax = m.mu * flen;
-        if (!(sqrtf(ftx * ftx + fty * fty) < ftmax)) {
+        if (!sqrt_below(ftx * ftx + fty * fty, ftmax)) {
           const float cl = sqrtf(cvx * cvx + cvy * cvy);
           ftx = (ftmax * cvx) / cl;
           fty = (ftmax * cvy) / cl;
