@@ -456,6 +456,9 @@ def main() -> int:
 
     scene, workload = make_scene(args.workload)
     ctx = api.Context(0)
+    if args.collide_mode:
+        ctx.set_step_variant(prefetch=args.collide_mode == "prefetch", batch=args.collide_mode == "batch",
+                             contact_lists=args.collide_mode == "list")
     if args.drop_in_only:
         line = stage_loop(ctx, scene, args.steps, args.warmup)
         line.update(workload=workload, particles=scene.n, cells=scene.total_cells)
@@ -470,9 +473,6 @@ def main() -> int:
 
         cell_tuning = tuning.autotune(ctx, scene, multipliers=(1.0, 1.25, 1.5, 2.0), steps=10)
         scene = tuning.with_cell_multiplier(scene, cell_tuning["multiplier"])
-    if args.collide_mode:
-        ctx.set_step_variant(prefetch=args.collide_mode == "prefetch", batch=args.collide_mode == "batch",
-                             contact_lists=args.collide_mode == "list")
     sim = api.Simulation(ctx, scene)
     n, cells = scene.n, scene.total_cells
 
