@@ -594,6 +594,8 @@ void sim_free(dem_sim* m) {
     if (m->exec_inc[k]) (void)hipGraphExecDestroy(m->exec_inc[k]);
   }
   if (m->h_count) (void)hipHostFree(m->h_count);
+  for (hipEvent_t& e : m->look)
+    if (e) (void)hipEventDestroy(e);
   if (m->cap_stream) (void)hipStreamDestroy(m->cap_stream);
   for (void* p : m->allocs) (void)hipFree(p);
   free_sort(m->sort);
@@ -840,11 +842,18 @@ bool sim_use_resort(dem_sim* m) {
   // shifts the particles between its old and new slot) has its state staged
   // and written back.  Past either limit the steps radix sort, and the
   // re-sort is tried again after 1, 2, 4 ... 64 radix-sorted steps (its
-  // counts are only measured by a re-sort).  The counts are the latest the
-  // device has written, which may trail the steps already enqueued.
+  // counts are only measured by a re-sort), or after 8, 16 ... 64 when the
+  // counts were at twice a limit or more.  The counts are the latest the
+  // device has written, which may trail the steps already enqueued: graph
+  // replays are queued faster than they run, so a trial's counts reach the
+  // host only after the choices that follow it are made, and a trial that
+  // lists most of the state costs about two radix-sorted steps (the collapse
+  // of a dense pack lists 2-3M of 4.2M discs a step for a thousand steps).
   const volatile uint32_t* h = m->h_count;
   const uint32_t movers = h[0], listed = h[1];
-  const bool cheap = movers <= std::max(4096u, m->n / 64u) && (!sim_inplace(m) || listed <= m->n / kListedFrac);
+  const uint32_t mover_limit = std::max(4096u, m->n / 64u), listed_limit = m->n / kListedFrac;
+  const bool inplace = sim_inplace(m);
+  const bool cheap = movers <= mover_limit && (!inplace || listed <= listed_limit);
   if (cheap) {
     m->radix_run = m->radix_wait = 0;
     return true;
@@ -854,7 +863,13 @@ bool sim_use_resort(dem_sim* m) {
     return false;
   }
   m->radix_run = 0;
-  m->radix_wait = m->radix_wait ? std::min(2u * m->radix_wait, 64u) : 1u;
+  if (m->radix_wait == 0) {
+    const bool far = movers / 2u >= mover_limit || (inplace && listed / 2u >= listed_limit);
+    m->radix_wait = far ? 8u : 1u;
+    if (far) return false;
+  } else {
+    m->radix_wait = std::min(2u * m->radix_wait, 64u);
+  }
   return true;
 }
 
@@ -1093,8 +1108,19 @@ int sim_step_auto(dem_sim* m) {
 int dem_sim_step(dem_sim* m, uint32_t steps) {
   if (!m) return DEM_ERR_INVALID;
   if (int r = sim_check(m)) return r;
-  for (uint32_t k = 0; k < steps; ++k)
+  // as dem_sim_step_graph: under the automatic sort choice at most
+  // kLookahead steps are queued, so the counts it reads stay recent
+  const bool look = m->sort_mode == 0 && steps > (uint32_t)dem_sim::kLookahead;
+  if (look)
+    for (hipEvent_t& e : m->look)
+      if (!e) DEM_HIP(m->ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  for (uint32_t k = 0; k < steps; ++k) {
+    hipEvent_t& slot = m->look[k % dem_sim::kLookahead];
+    if (look && k >= (uint32_t)dem_sim::kLookahead)
+      DEM_HIP(m->ctx, hipEventSynchronize(slot), "hipEventSynchronize");
     if (int r = sim_step_auto(m)) return r;
+    if (look) DEM_HIP(m->ctx, hipEventRecord(slot, m->ctx->stream), "hipEventRecord");
+  }
   return DEM_OK;
 }
 
@@ -1109,9 +1135,20 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
   if (steps == 0) return DEM_OK;
   if (!m->exec[0] || m->graph_gen != ctx->generation)
     if (int r = sim_capture(m)) return r;
-  for (uint32_t k = 0; k < steps; k += 2) {
+  // The automatic sort choice reads counts that the replays write: with
+  // every replay queued at once they would all be chosen from the counts of
+  // before the call.  It keeps at most kLookahead replays queued instead
+  // (enough to keep the GPU busy; the host waits on the oldest).
+  const bool look = m->sort_mode == 0;
+  if (look)
+    for (hipEvent_t& e : m->look)
+      if (!e) DEM_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  for (uint32_t k = 0, r = 0; k < steps; k += 2, ++r) {
+    hipEvent_t& slot = m->look[r % dem_sim::kLookahead];
+    if (look && r >= (uint32_t)dem_sim::kLookahead) DEM_HIP(ctx, hipEventSynchronize(slot), "hipEventSynchronize");
     const bool resort = sim_use_resort(m);
     DEM_HIP(ctx, hipGraphLaunch(resort ? m->exec_inc[m->cur] : m->exec[m->cur], ctx->stream), "hipGraphLaunch");
+    if (look) DEM_HIP(ctx, hipEventRecord(slot, ctx->stream), "hipEventRecord");
     m->resort_ok = true;
     m->perm_list = resort && sim_inplace(m);
     m->acc_lazy = m->perm_list;

@@ -116,6 +116,11 @@ struct dem_sim {
   hipGraphExec_t exec[2] = {nullptr, nullptr};      // two radix-sorted steps, by starting buffer
   hipGraphExec_t exec_inc[2] = {nullptr, nullptr};  // two re-sorted steps, by starting buffer
   uint64_t graph_gen = 0;
+  // sort_mode 0: events recorded after the last kLookahead replays of
+  // dem_sim_step_graph, which waits on the oldest before it picks the next
+  // replay (the counts it picks from then trail by at most that many)
+  static constexpr int kLookahead = 4;
+  hipEvent_t look[kLookahead] = {nullptr, nullptr, nullptr, nullptr};
   // reaction of each sorted particle on the controlled particle, and its
   // per-block sums (ceil(n / kReactBlock) each)
   float *react_x = nullptr, *react_y = nullptr, *react_m = nullptr;

@@ -244,9 +244,15 @@ int dem_sim_download(dem_sim* sim, float* pos, float* vel, float* theta, float* 
                      uint32_t* type, uint32_t* loc, float* radius, float* acc, float* alpha,
                      float* pressure, uint32_t* hash, uint32_t* index, uint32_t* cellStart,
                      uint32_t* cellEnd);
-/* Run `steps` steps with eager launches on the context stream. */
+/* Run `steps` steps with eager launches on the context stream.  Under the
+ * automatic sort choice (dem_sim_set_sort_mode 0) the call keeps at most 4
+ * steps queued, waiting for older ones to finish before it queues the next,
+ * so that each step's sort method is picked from recent counts; it returns
+ * with up to 4 steps still running (not synchronized). */
 int dem_sim_step(dem_sim* sim, uint32_t steps);
-/* Run `steps` steps by replaying a captured hipGraph of the step sequence. */
+/* Run `steps` steps by replaying a captured hipGraph of the step sequence
+ * (two steps a replay; under the automatic sort choice at most 4 replays are
+ * kept queued, as dem_sim_step does). */
 int dem_sim_step_graph(dem_sim* sim, uint32_t steps);
 /* Average device time per stage over `reps` eager steps, measured with HIP
  * events on the context stream: ms[0]=hash ms[1]=sort ms[2]=cell fill
@@ -271,7 +277,8 @@ int dem_sim_autotune(dem_sim* sim, uint32_t* block);
  * sort, bit for bit).  mode 0 (default): re-sort while the last re-sort's
  * counts (dem_sim_resort_counts) stay small (movers at most max(4096, n/64);
  * in place, at most n/4 particles listed), else radix sort and try the
- * re-sort again after 1, 2, 4 ... 64 radix-sorted steps; 1: radix sort every
+ * re-sort again after 1, 2, 4 ... 64 radix-sorted steps (8, 16 ... 64 when a
+ * count was twice its limit or more); 1: radix sort every
  * step; 2: re-sort whenever the state allows it.  The first step after
  * dem_sim_upload, dem_sim_set_by_id (positions), dem_sim_autotune or
  * dem_adj_backward always radix sorts. */
