@@ -1105,21 +1105,31 @@ int sim_step_auto(dem_sim* m) {
   return DEM_OK;
 }
 
+// The queue-bounding events only mark progress: no system-scope fence (a
+// cache writeback at every record; the counts the host reads live in
+// host-mapped memory, and a late count only delays a choice that gives the
+// same bits either way).
+constexpr unsigned kLookFlags = hipEventDisableTiming | hipEventDisableSystemFence;
+// Below this many particles a step takes tens of microseconds, and the
+// events and waits cost more (about 9% on 2,000-60,000-disc columns) than
+// a late choice does; the queue is bounded only for larger states.
+constexpr uint32_t kLookMinParticles = 1u << 18;
+
 int dem_sim_step(dem_sim* m, uint32_t steps) {
   if (!m) return DEM_ERR_INVALID;
   if (int r = sim_check(m)) return r;
-  // as dem_sim_step_graph: under the automatic sort choice at most
-  // kLookahead steps are queued, so the counts it reads stay recent
-  const bool look = m->sort_mode == 0 && steps > (uint32_t)dem_sim::kLookahead;
+  // as dem_sim_step_graph: under the automatic sort choice the steps are
+  // queued in groups, at most two ahead, so the counts it reads stay recent
+  constexpr uint32_t G = dem_sim::kLookGroup;
+  const bool look = m->sort_mode == 0 && m->n >= kLookMinParticles && steps > 2u * G;
   if (look)
     for (hipEvent_t& e : m->look)
-      if (!e) DEM_HIP(m->ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      if (!e) DEM_HIP(m->ctx, hipEventCreateWithFlags(&e, kLookFlags), "hipEventCreate");
   for (uint32_t k = 0; k < steps; ++k) {
-    hipEvent_t& slot = m->look[k % dem_sim::kLookahead];
-    if (look && k >= (uint32_t)dem_sim::kLookahead)
-      DEM_HIP(m->ctx, hipEventSynchronize(slot), "hipEventSynchronize");
+    hipEvent_t& slot = m->look[(k / G) & 1u];
+    if (look && k % G == 0 && k >= 2u * G) DEM_HIP(m->ctx, hipEventSynchronize(slot), "hipEventSynchronize");
     if (int r = sim_step_auto(m)) return r;
-    if (look) DEM_HIP(m->ctx, hipEventRecord(slot, m->ctx->stream), "hipEventRecord");
+    if (look && k % G == G - 1u) DEM_HIP(m->ctx, hipEventRecord(slot, m->ctx->stream), "hipEventRecord");
   }
   return DEM_OK;
 }
@@ -1137,18 +1147,20 @@ int dem_sim_step_graph(dem_sim* m, uint32_t steps) {
     if (int r = sim_capture(m)) return r;
   // The automatic sort choice reads counts that the replays write: with
   // every replay queued at once they would all be chosen from the counts of
-  // before the call.  It keeps at most kLookahead replays queued instead
-  // (enough to keep the GPU busy; the host waits on the oldest).
-  const bool look = m->sort_mode == 0;
+  // before the call.  It queues the replays in groups of kLookGroup instead,
+  // waiting before each group for the group two back to finish (the GPU
+  // keeps a group queued meanwhile); one event a group.
+  constexpr uint32_t G = dem_sim::kLookGroup;
+  const bool look = m->sort_mode == 0 && m->n >= kLookMinParticles;
   if (look)
     for (hipEvent_t& e : m->look)
-      if (!e) DEM_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      if (!e) DEM_HIP(ctx, hipEventCreateWithFlags(&e, kLookFlags), "hipEventCreate");
   for (uint32_t k = 0, r = 0; k < steps; k += 2, ++r) {
-    hipEvent_t& slot = m->look[r % dem_sim::kLookahead];
-    if (look && r >= (uint32_t)dem_sim::kLookahead) DEM_HIP(ctx, hipEventSynchronize(slot), "hipEventSynchronize");
+    hipEvent_t& slot = m->look[(r / G) & 1u];
+    if (look && r % G == 0 && r >= 2u * G) DEM_HIP(ctx, hipEventSynchronize(slot), "hipEventSynchronize");
     const bool resort = sim_use_resort(m);
     DEM_HIP(ctx, hipGraphLaunch(resort ? m->exec_inc[m->cur] : m->exec[m->cur], ctx->stream), "hipGraphLaunch");
-    if (look) DEM_HIP(ctx, hipEventRecord(slot, ctx->stream), "hipEventRecord");
+    if (look && r % G == G - 1u) DEM_HIP(ctx, hipEventRecord(slot, ctx->stream), "hipEventRecord");
     m->resort_ok = true;
     m->perm_list = resort && sim_inplace(m);
     m->acc_lazy = m->perm_list;

@@ -116,11 +116,12 @@ struct dem_sim {
   hipGraphExec_t exec[2] = {nullptr, nullptr};      // two radix-sorted steps, by starting buffer
   hipGraphExec_t exec_inc[2] = {nullptr, nullptr};  // two re-sorted steps, by starting buffer
   uint64_t graph_gen = 0;
-  // sort_mode 0: events recorded after the last kLookahead replays of
-  // dem_sim_step_graph, which waits on the oldest before it picks the next
-  // replay (the counts it picks from then trail by at most that many)
-  static constexpr int kLookahead = 4;
-  hipEvent_t look[kLookahead] = {nullptr, nullptr, nullptr, nullptr};
+  // sort_mode 0: an event recorded after every kLookGroup-th replay (or
+  // eager step); before it starts a group, the host waits on the event of
+  // the group two back, so at most 2 kLookGroup are queued and the counts it
+  // picks from trail by less than that
+  static constexpr int kLookGroup = 4;
+  hipEvent_t look[2] = {nullptr, nullptr};
   // reaction of each sorted particle on the controlled particle, and its
   // per-block sums (ceil(n / kReactBlock) each)
   float *react_x = nullptr, *react_y = nullptr, *react_m = nullptr;

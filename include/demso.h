@@ -245,14 +245,15 @@ int dem_sim_download(dem_sim* sim, float* pos, float* vel, float* theta, float* 
                      float* pressure, uint32_t* hash, uint32_t* index, uint32_t* cellStart,
                      uint32_t* cellEnd);
 /* Run `steps` steps with eager launches on the context stream.  Under the
- * automatic sort choice (dem_sim_set_sort_mode 0) the call keeps at most 4
- * steps queued, waiting for older ones to finish before it queues the next,
- * so that each step's sort method is picked from recent counts; it returns
- * with up to 4 steps still running (not synchronized). */
+ * automatic sort choice (dem_sim_set_sort_mode 0) and from 262,144
+ * particles up, the call queues the steps in groups of 4 and, before a
+ * group, waits for the group two back to finish, so that each step's sort
+ * method is picked from recent counts; it returns with up to 8 steps still
+ * running (not synchronized). */
 int dem_sim_step(dem_sim* sim, uint32_t steps);
 /* Run `steps` steps by replaying a captured hipGraph of the step sequence
- * (two steps a replay; under the automatic sort choice at most 4 replays are
- * kept queued, as dem_sim_step does). */
+ * (two steps a replay; under the automatic sort choice the replays are
+ * queued in groups of 4, as dem_sim_step queues steps). */
 int dem_sim_step_graph(dem_sim* sim, uint32_t steps);
 /* Average device time per stage over `reps` eager steps, measured with HIP
  * events on the context stream: ms[0]=hash ms[1]=sort ms[2]=cell fill
