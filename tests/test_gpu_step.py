@@ -160,13 +160,14 @@ def test_step_variants_equal(gpu_ctx, name, variant):
 def test_settling_4m_first_steps(gpu_ctx, oracle):
     """Full-size settling workload (4,194,304 discs, 11,193,546 cells):
     hash, stable permutation and cellStart bit-exact with the oracle after
-    steps 1 and 2 (eager) and 100 (graph replays of re-sorted steps), state
+    steps 1, 2 and 12 (eager; the last ten queued in groups under the
+    automatic sort choice) and 100 (graph replays of re-sorted steps), state
     within 1e-5, plus size-independent checks."""
     s = sc.settling()
     assert s.n == 4194304 and s.total_cells == 11193546
     sim = api.Simulation(gpu_ctx, s)
     ref = oracle.Sim(s, "f32")
-    for chunk, graph in ((1, False), (1, False), (98, True)):
+    for chunk, graph in ((1, False), (1, False), (10, False), (88, True)):
         sim.step_graph(chunk) if graph else sim.step(chunk)
         ref.step(chunk)
         g, o = sim.download(), ref.state()
