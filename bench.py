@@ -15,6 +15,8 @@ Prints ONE JSON line:
                    stream, eager steps of the same workload)
   cpu_baseline     the CPU oracle (a port of the reference kernels) timed
                    on this host on a bounded sample of the same workload
+  dense_settling   (default run) the same discs at their mean diameter,
+                   about two contacts a disc: particle-steps/s
 """
 from __future__ import annotations
 
@@ -180,6 +182,24 @@ def ensemble_speedup(ctx, steps: int = 50) -> dict:
     return {"value": batched / single["value"], "batched": batched, "one_by_one": single["value"],
             "unit": "particle-steps/s", "instances": len(scene.instances), "particles": scene.n,
             "steps": steps}
+
+
+def dense_settling(ctx, steps: int = 20) -> dict:
+    """The headline's 4,194,304 discs on a lattice at their mean diameter
+    (about two contacts a disc, so the contact law runs for most discs):
+    graph-replayed steps after the first (radix-sorted) one and autotuning,
+    next to the sparse headline state."""
+    from demso_amd import api, tuning
+    from demso_amd import scene as sc
+
+    scene = sc.settling_dense()
+    sim = api.Simulation(ctx, scene)
+    sim.step(1)
+    sim.autotune()
+    sec = tuning.time_graph_steps(sim, steps)
+    sim.close()
+    return {"value": scene.n / sec, "unit": "particle-steps/s", "ms_per_step": sec * 1e3, "steps": steps,
+            "particles": scene.n, "workload": "settling_dense (bench.py --workload settling_dense)"}
 
 
 def adjoint_ratio(ctx) -> dict:
@@ -552,6 +572,7 @@ def main() -> int:
         extra["drop_in_stage_loop"] = stage_loop(ctx, scene, max(10, min(args.steps, 50)), 3)
     if args.workload == "settling" and not args.no_extras:
         sim.close()
+        extra["dense_settling"] = dense_settling(ctx)
         extra["ensemble_speedup"] = ensemble_speedup(ctx)
         extra["adjoint_backward_over_forward"] = adjoint_ratio(ctx)
 
